@@ -1,0 +1,184 @@
+/*
+ * pmdd_engine.h -- C ABI of the MI355X frontier-enumeration engine.
+ *
+ * This is the seam that replaces the reference's GPU backend
+ * (src/pmdd/enum/gpu/).  The reference binds four C++ functions there,
+ * declared in src/pmdd/enum/gpu/cuda_wrappers.hpp:11-21:
+ *
+ *   topdown_cuda_enumerate(BDD*, maximization, problem_type, state_dominance,
+ *                          EnumerationStats*, std::string* reason, gpu_batch_size)
+ *   topdown_mdd_cuda_enumerate(MDD*, EnumerationStats*, std::string*, gpu_batch_size)
+ *   coupled_cuda_enumerate(MDD*, EnumerationStats*, std::string*, gpu_batch_size, gpu_max_prod)
+ *   coupled_bdd_cuda_enumerate(BDD*, maximization, problem_type, state_dominance,
+ *                              EnumerationStats*, std::string*, gpu_batch_size, gpu_max_prod)
+ *
+ * Here the diagram crosses the boundary as plain arrays (pmdd_dd_view), the
+ * result as a malloc'd point array (pmdd_frontier), errors as a negative code
+ * plus a message in a caller buffer (the reference returns NULL and sets
+ * *reason, src/pmdd/enum/gpu/enum_types.cuh:76-97), and the statistics as a
+ * POD struct (the GPU-filled subset of EnumerationStats,
+ * src/pmdd/util/stats.hpp:18-85).  No C++ type and no exception crosses it.
+ *
+ * Semantics (all integer, exact): maximisation everywhere; the frontier of a
+ * node is the set of nondominated root->node path-weight vectors with
+ * duplicates merged; the returned frontier is the nondominated set of all
+ * root->terminal path sums, unsorted (the caller sorts, as main.cpp:403 does).
+ */
+#ifndef PMDD_ENGINE_H_
+#define PMDD_ENGINE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---- */
+#define PMDD_OK 0
+#define PMDD_E_INVALID (-1)     /* malformed view / options */
+#define PMDD_E_NODEVICE (-2)    /* no HIP device ("No CUDA device found", enum.cu:37-39) */
+#define PMDD_E_HIP (-3)         /* a HIP runtime call failed */
+#define PMDD_E_NOMEM (-4)       /* device or host allocation failed */
+#define PMDD_E_OVERFLOW (-5)    /* an int32 path sum could overflow */
+#define PMDD_E_UNSUPPORTED (-6) /* objective count or method outside what the engine builds */
+#define PMDD_E_REJECTED (-7)    /* serving admission refused the request */
+
+/* ---- diagram kinds ---- */
+#define PMDD_DD_BDD 0
+#define PMDD_DD_MDD 1
+
+/* Flattened layered decision diagram.
+ *
+ * Layers 0..num_layers-1; layer 0 holds the root, the last layer the terminal.
+ * Nodes are numbered globally: layer l owns ids [layer_off[l], layer_off[l+1]).
+ * Arcs are stored once, grouped by head node (top-down CSR): the in-arcs of
+ * global node g are arcs [in_off[g], in_off[g+1]).  For a BDD the in-arcs of
+ * a node list the 1-arcs first, then the 0-arcs (the maximisation order of
+ * pack_bdd_layers, src/pmdd/enum/gpu/enum.cu:444-445,460-494).
+ *
+ * arc_src[a]  local index (within the previous layer) of the arc's tail
+ * arc_val[a]  BDD: arc type 0/1; MDD: arc value (the city for TSP)
+ * arc_w[a*d+o] objective o weight of arc a (int32, AoS like td_edge_weights)
+ * min_weight  per global node, knapsack used capacity (constraint 0) as set
+ *             by update_node_weights (knapsack_bdd.cpp:123-150); NULL when
+ *             the diagram is not a knapsack BDD.
+ */
+typedef struct pmdd_dd_view {
+    int32_t d;
+    int32_t num_layers;
+    int32_t kind; /* PMDD_DD_BDD or PMDD_DD_MDD */
+    int32_t root;     /* local index of the root in layer 0 */
+    int32_t terminal; /* local index of the terminal in the last layer */
+    int32_t reserved;
+    const int64_t *layer_off; /* [num_layers + 1] */
+    const int64_t *in_off;    /* [layer_off[num_layers] + 1] */
+    const int32_t *arc_src;   /* [num_arcs] */
+    const int32_t *arc_val;   /* [num_arcs] */
+    const int32_t *arc_w;     /* [num_arcs * d] */
+    const int32_t *min_weight; /* [layer_off[num_layers]] or NULL */
+} pmdd_dd_view;
+
+/* Options of one enumeration, mirroring the reference CLI
+ * (src/pmdd/util/cli_parser.hpp:10-30). */
+typedef struct pmdd_opts {
+    int32_t method;          /* 1 = top-down, 3 = dynamic layer cutset (coupled) */
+    int32_t problem_type;    /* 1 knapsack, 2 set packing, 3 TSP */
+    int32_t state_dominance; /* 0 or 1 */
+    int32_t count_pair_tests;/* 1 = count executed dominance pair tests (slower) */
+    int64_t max_cand;        /* --max-cand, candidates per expansion pass (default 20M) */
+    int64_t max_prod;        /* --max-prod, products per coupling batch (default 625K) */
+} pmdd_opts;
+
+#define PMDD_DEFAULT_MAX_CAND 20000000LL
+#define PMDD_DEFAULT_MAX_PROD 625000LL
+
+/* Result: count points of d int32 objectives, row-major (point i at
+ * points[i*d .. i*d+d-1]).  Allocated with malloc by the engine; release with
+ * pmdd_frontier_free. */
+typedef struct pmdd_frontier {
+    int32_t d;
+    int32_t reserved;
+    int64_t count;
+    int32_t *points;
+} pmdd_frontier;
+
+/* GPU-side statistics (subset of EnumerationStats that the GPU path fills,
+ * src/pmdd/util/stats.hpp:18-85).  The engine overwrites every scalar field.
+ * The per-layer arrays are caller-owned and optional: when non-NULL the engine
+ * writes min(std_capacity, num_layers) entries (method 1 only; method 3 leaves
+ * them at 0, as the reference clears them, enum.cu:610-613). */
+typedef struct pmdd_stats {
+    double wall_pack_transfer_s;
+    double wall_join_s;
+    double kernel_expand_td_s;
+    double kernel_dominance_s;
+    double kernel_total_s;
+    int64_t gpu_mem_peak_used_bytes;
+    int64_t gpu_mem_peak_reserved_bytes;
+    int64_t work_candidates_total;
+    int64_t work_candidates_peak;
+    int64_t work_frontier_survivors_total;
+    int64_t work_frontier_peak_points;
+    int64_t work_join_products_total;
+    int64_t dominance_filtered_total;
+    int64_t dominance_pair_tests; /* executed pair tests; 0 unless opts.count_pair_tests */
+    int32_t layer_coupling;
+    int32_t std_capacity;
+    double *std_candidates_per_layer;
+    double *std_frontier_survivors_per_layer;
+} pmdd_stats;
+
+/* Per-device engine context (device memory pools, streams, tuning table). */
+typedef struct pmdd_ctx pmdd_ctx;
+
+/* A diagram uploaded to device memory, reusable across enumerations. */
+typedef struct pmdd_resident pmdd_resident;
+
+/* Create a context on HIP device `device` (the reference hard-codes device 0,
+ * enum.cu:42-47).  Fails with PMDD_E_NODEVICE when no device is visible. */
+int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen);
+void pmdd_ctx_destroy(pmdd_ctx *ctx);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int pmdd_device_count(void);
+
+/* One-shot enumeration: upload, enumerate, copy the result back, free device
+ * memory.  Replaces the four cuda_wrappers.hpp entry points; method and
+ * problem type come from opts, the diagram kind from the view. */
+int pmdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *view, const pmdd_opts *opts,
+                   pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen);
+
+/* Named forms of the four reference entry points (cuda_wrappers.hpp:11-21).
+ * They fill a pmdd_opts and call pmdd_enumerate; the view kind must match. */
+int pmdd_topdown_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *bdd, int problem_type,
+                           int state_dominance, int64_t max_cand, pmdd_frontier *out,
+                           pmdd_stats *stats, char *err, size_t errlen);
+int pmdd_topdown_mdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *mdd, int64_t max_cand,
+                               pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen);
+int pmdd_coupled_mdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *mdd, int64_t max_cand,
+                               int64_t max_prod, pmdd_frontier *out, pmdd_stats *stats, char *err,
+                               size_t errlen);
+int pmdd_coupled_bdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *bdd, int problem_type,
+                               int state_dominance, int64_t max_cand, int64_t max_prod,
+                               pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen);
+
+/* Upload a diagram once (pack_bdd_layers' H2D step, enum.cu:438-593) and keep
+ * it resident in HBM; the view may be freed afterwards. */
+int pmdd_upload(pmdd_ctx *ctx, const pmdd_dd_view *view, pmdd_resident **out, char *err,
+                size_t errlen);
+/* Enumerate a resident diagram (inputs already in HBM). */
+int pmdd_enumerate_resident(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts,
+                            pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen);
+void pmdd_resident_free(pmdd_resident *dd);
+
+void pmdd_frontier_free(pmdd_frontier *f);
+
+/* Engine build identity, e.g. "pmdd-engine gfx950 d=2..8". */
+const char *pmdd_engine_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMDD_ENGINE_H_ */

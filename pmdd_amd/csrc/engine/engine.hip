@@ -1,0 +1,1393 @@
+// Frontier enumeration engine for MI355X (gfx950): host orchestration and the
+// C ABI declared in include/pmdd_engine.h.
+//
+// Replaces the reference GPU backend src/pmdd/enum/gpu/:
+//   enumerate_bdd_topdown      topdown.cu:552-1043      -> Run::topdown
+//   enumerate_mdd_topdown      enum.cu:164-289          -> Run::topdown
+//   coupled_bdd_cuda_enumerate enum.cu:595-756          -> Run::coupled
+//   enumerate_mdd_coupled      enum.cu:295-436          -> Run::coupled
+//   expand_layer_frontiers     bottomup.cu:242-533      -> Run::expand
+//   apply_knapsack_state_dominance topdown.cu:363-521   -> Run::state_dominance
+//   couple_cutsets_cuda        couple.cu:340-498        -> Run::couple
+// with kernels of its own design (kernels.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../../include/pmdd_engine.h"
+#include "kernels.hpp"
+
+using pmdd_k::ChunkDesc;
+using pmdd_k::kBlock;
+
+namespace {
+
+struct EngineError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw EngineError{code, msg}; }
+
+void hip_check(hipError_t e, const char *where) {
+    if (e == hipSuccess) {
+        return;
+    }
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PMDD_E_NOMEM
+                                                                                  : PMDD_E_HIP;
+    fail(code, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// Device buffer; grows on demand, contents not kept unless asked.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p != nullptr) {
+            (void)hipFree(p);
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t need, const char *what) {
+        if (need <= bytes && p != nullptr) {
+            return;
+        }
+        release();
+        size_t nb = std::max<size_t>(need, 256);
+        hip_check(hipMalloc(&p, nb), what);
+        bytes = nb;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Context and resident diagram
+// ---------------------------------------------------------------------------
+struct pmdd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // scratch reused across layers and calls
+    DevBuf chunks, run_off, nsurv, nalive, minkey, maxkey, chunk_out, which;
+    DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
+};
+
+struct LayerCSR {
+    // host copies (planning) and device arrays
+    std::vector<int64_t> off; // [N + 1] layer-local arc ids
+    std::vector<int32_t> src; // [E]
+    int64_t n_arcs = 0;
+    int64_t *d_off = nullptr;
+    int32_t *d_src = nullptr;
+    int32_t *d_w = nullptr; // SoA [d][E]
+};
+
+struct pmdd_resident {
+    int device = 0;
+    int d = 0;
+    int L = 0;
+    int kind = PMDD_DD_BDD;
+    int root = 0;
+    int terminal = 0;
+    std::vector<int64_t> layer_size;
+    std::vector<LayerCSR> td; // td[l]: arcs into layer l (l >= 1)
+    std::vector<LayerCSR> bu; // bu[l]: arcs out of layer l (l <= L-2), "src" = child in l+1
+    std::vector<std::vector<int32_t>> min_weight;    // per layer, knapsack only
+    std::vector<std::vector<int32_t>> single_parent; // per layer: parent index or -1
+    std::vector<std::vector<int32_t>> single_arc;    // per layer: arc value of that parent
+    DevBuf arena;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Upload: pack the view into per-layer CSR in both directions
+// (pack_bdd_layers enum.cu:438-593, pack_mdd_layers enum.cu:85-158).
+// ---------------------------------------------------------------------------
+void validate_view(const pmdd_dd_view *v) {
+    if (v == nullptr || v->layer_off == nullptr || v->in_off == nullptr) {
+        fail(PMDD_E_INVALID, "decision diagram view is NULL or incomplete");
+    }
+    if (v->d < 1 || v->d > 8) {
+        fail(PMDD_E_UNSUPPORTED,
+             "objective count " + std::to_string(v->d) + " outside the engine's 1..8");
+    }
+    if (v->num_layers < 2) {
+        fail(PMDD_E_INVALID, "decision diagram needs at least two layers");
+    }
+    if (v->kind != PMDD_DD_BDD && v->kind != PMDD_DD_MDD) {
+        fail(PMDD_E_INVALID, "unknown decision diagram kind");
+    }
+    const int L = v->num_layers;
+    if (v->layer_off[0] != 0) {
+        fail(PMDD_E_INVALID, "layer_off[0] must be 0");
+    }
+    for (int l = 0; l < L; ++l) {
+        if (v->layer_off[l + 1] <= v->layer_off[l]) {
+            fail(PMDD_E_INVALID, "layer " + std::to_string(l) + " has no nodes");
+        }
+        if (v->layer_off[l + 1] - v->layer_off[l] > INT32_MAX) {
+            fail(PMDD_E_UNSUPPORTED, "layer wider than 2^31 nodes");
+        }
+    }
+    if (v->root < 0 || v->root >= v->layer_off[1] || v->terminal < 0 ||
+        v->terminal >= v->layer_off[L] - v->layer_off[L - 1]) {
+        fail(PMDD_E_INVALID, "root or terminal index out of range");
+    }
+    const int64_t total = v->layer_off[L];
+    for (int64_t g = 0; g < v->layer_off[1]; ++g) {
+        if (v->in_off[g + 1] != v->in_off[g]) {
+            fail(PMDD_E_INVALID, "layer 0 nodes cannot have in-arcs");
+        }
+    }
+    for (int64_t g = 0; g < total; ++g) {
+        if (v->in_off[g + 1] < v->in_off[g]) {
+            fail(PMDD_E_INVALID, "in_off must be non-decreasing");
+        }
+    }
+    const int64_t A = v->in_off[total];
+    if (A > 0 && (v->arc_src == nullptr || v->arc_w == nullptr || v->arc_val == nullptr)) {
+        fail(PMDD_E_INVALID, "arc arrays missing");
+    }
+    for (int l = 1; l < L; ++l) {
+        const int64_t prev_n = v->layer_off[l] - v->layer_off[l - 1];
+        for (int64_t a = v->in_off[v->layer_off[l]]; a < v->in_off[v->layer_off[l + 1]]; ++a) {
+            if (v->arc_src[a] < 0 || v->arc_src[a] >= prev_n) {
+                fail(PMDD_E_INVALID, "arc source out of range in layer " + std::to_string(l));
+            }
+        }
+    }
+    // Every path sum stays inside int32: per objective, the sum over layers of
+    // the largest |arc weight| entering the layer bounds any partial or full
+    // path sum (the reference adds unchecked, bottomup.cu:124, couple.cu:93).
+    for (int o = 0; o < v->d; ++o) {
+        int64_t bound = 0;
+        for (int l = 1; l < L; ++l) {
+            int64_t mx = 0;
+            for (int64_t a = v->in_off[v->layer_off[l]]; a < v->in_off[v->layer_off[l + 1]]; ++a) {
+                mx = std::max<int64_t>(mx, std::llabs(static_cast<long long>(v->arc_w[a * v->d + o])));
+            }
+            bound += mx;
+        }
+        if (bound > INT32_MAX) {
+            fail(PMDD_E_OVERFLOW, "objective " + std::to_string(o) +
+                                      " path sums can exceed int32 (bound " +
+                                      std::to_string(bound) + ")");
+        }
+    }
+}
+
+pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
+    validate_view(v);
+    std::unique_ptr<pmdd_resident> r(new pmdd_resident());
+    const int L = v->num_layers;
+    const int d = v->d;
+    r->device = ctx->device;
+    r->d = d;
+    r->L = L;
+    r->kind = v->kind;
+    r->root = v->root;
+    r->terminal = v->terminal;
+    r->layer_size.resize(L);
+    for (int l = 0; l < L; ++l) {
+        r->layer_size[l] = v->layer_off[l + 1] - v->layer_off[l];
+    }
+    r->td.resize(L);
+    r->bu.resize(L);
+    std::vector<std::vector<int32_t>> td_w(L), bu_w(L);
+    for (int l = 1; l < L; ++l) {
+        const int64_t n = r->layer_size[l];
+        const int64_t g0 = v->layer_off[l];
+        const int64_t a0 = v->in_off[g0];
+        const int64_t E = v->in_off[g0 + n] - a0;
+        LayerCSR &c = r->td[l];
+        c.n_arcs = E;
+        c.off.resize(n + 1);
+        for (int64_t i = 0; i <= n; ++i) {
+            c.off[i] = v->in_off[g0 + i] - a0;
+        }
+        c.src.assign(v->arc_src + a0, v->arc_src + a0 + E);
+        td_w[l].resize(static_cast<size_t>(d) * E);
+        for (int64_t e = 0; e < E; ++e) {
+            for (int o = 0; o < d; ++o) {
+                td_w[l][o * E + e] = v->arc_w[(a0 + e) * d + o];
+            }
+        }
+        // bottom-up CSR of layer l-1: out-arcs grouped by tail, in the
+        // reference merge order (BDD: 1-arc first; MDD: by arc value)
+        LayerCSR &b = r->bu[l - 1];
+        const int64_t np = r->layer_size[l - 1];
+        b.n_arcs = E;
+        b.off.assign(np + 1, 0);
+        std::vector<int64_t> head(E);
+        for (int64_t i = 0; i < n; ++i) {
+            for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) {
+                head[e] = i;
+                b.off[c.src[e] + 1]++;
+            }
+        }
+        for (int64_t i = 0; i < np; ++i) {
+            b.off[i + 1] += b.off[i];
+        }
+        std::vector<int64_t> order(E);
+        std::vector<int64_t> pos(b.off.begin(), b.off.end() - 1);
+        for (int64_t e = 0; e < E; ++e) {
+            order[pos[c.src[e]]++] = e;
+        }
+        for (int64_t i = 0; i < np; ++i) {
+            std::stable_sort(order.begin() + b.off[i], order.begin() + b.off[i + 1],
+                             [&](int64_t x, int64_t y) {
+                                 const int32_t vx = v->arc_val[a0 + x], vy = v->arc_val[a0 + y];
+                                 return v->kind == PMDD_DD_BDD ? vx > vy : vx < vy;
+                             });
+        }
+        b.src.resize(E);
+        bu_w[l - 1].resize(static_cast<size_t>(d) * E);
+        for (int64_t k = 0; k < E; ++k) {
+            const int64_t e = order[k];
+            b.src[k] = static_cast<int32_t>(head[e]);
+            for (int o = 0; o < d; ++o) {
+                bu_w[l - 1][o * E + k] = td_w[l][o * E + e];
+            }
+        }
+    }
+    // knapsack metadata for state dominance (pack_bdd_layers enum.cu:496-540)
+    if (v->min_weight != nullptr) {
+        r->min_weight.resize(L);
+        r->single_parent.resize(L);
+        r->single_arc.resize(L);
+        for (int l = 0; l < L; ++l) {
+            const int64_t n = r->layer_size[l];
+            const int64_t g0 = v->layer_off[l];
+            r->min_weight[l].assign(v->min_weight + g0, v->min_weight + g0 + n);
+            r->single_parent[l].assign(n, -1);
+            r->single_arc[l].assign(n, -1);
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t a0 = v->in_off[g0 + i], a1 = v->in_off[g0 + i + 1];
+                if (a1 - a0 == 1) {
+                    r->single_parent[l][i] = v->arc_src[a0];
+                    r->single_arc[l][i] = v->arc_val[a0];
+                }
+            }
+        }
+    }
+
+    // one device arena for all CSR arrays
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t bytes = 0;
+    for (int l = 0; l < L; ++l) {
+        for (LayerCSR *c : {&r->td[l], &r->bu[l]}) {
+            if (c->off.empty()) {
+                continue;
+            }
+            bytes += align(c->off.size() * 8) + align(c->src.size() * 4) +
+                     align(static_cast<size_t>(d) * c->n_arcs * 4);
+        }
+    }
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    r->arena.ensure(bytes, "hipMalloc(diagram arena)");
+    char *base = r->arena.as<char>();
+    size_t at = 0;
+    for (int l = 0; l < L; ++l) {
+        for (int dir = 0; dir < 2; ++dir) {
+            LayerCSR *c = dir == 0 ? &r->td[l] : &r->bu[l];
+            const std::vector<int32_t> &w = dir == 0 ? td_w[l] : bu_w[l];
+            if (c->off.empty()) {
+                continue;
+            }
+            c->d_off = reinterpret_cast<int64_t *>(base + at);
+            at += align(c->off.size() * 8);
+            c->d_src = reinterpret_cast<int32_t *>(base + at);
+            at += align(c->src.size() * 4);
+            c->d_w = reinterpret_cast<int32_t *>(base + at);
+            at += align(static_cast<size_t>(d) * c->n_arcs * 4);
+            hip_check(hipMemcpyAsync(c->d_off, c->off.data(), c->off.size() * 8,
+                                     hipMemcpyHostToDevice, ctx->stream),
+                      "hipMemcpy(csr offsets)");
+            if (c->n_arcs > 0) {
+                hip_check(hipMemcpyAsync(c->d_src, c->src.data(), c->src.size() * 4,
+                                         hipMemcpyHostToDevice, ctx->stream),
+                          "hipMemcpy(csr sources)");
+                hip_check(hipMemcpyAsync(c->d_w, w.data(), w.size() * 4, hipMemcpyHostToDevice,
+                                         ctx->stream),
+                          "hipMemcpy(csr weights)");
+            }
+        }
+    }
+    hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
+    return r.release();
+}
+
+// ---------------------------------------------------------------------------
+// Frontier of one layer on the device: offsets per node + SoA points.
+// ---------------------------------------------------------------------------
+struct DevFrontier {
+    int64_t n_nodes = 0;
+    std::vector<int64_t> h_off; // [n_nodes + 1]
+    DevBuf off;
+    DevBuf pts;
+    int64_t cap = 0; // SoA stride
+    int64_t total() const { return h_off.empty() ? 0 : h_off.back(); }
+    int64_t size(int64_t v) const { return h_off[v + 1] - h_off[v]; }
+};
+
+// Grow a SoA point buffer to hold `need` points, keeping the first `keep`.
+void grow_points(DevFrontier &f, int d, int64_t need, int64_t keep, hipStream_t s) {
+    if (need <= f.cap && f.pts.p != nullptr) {
+        return;
+    }
+    const int64_t ncap = std::max<int64_t>(need, f.cap + f.cap / 2);
+    DevBuf nb;
+    nb.ensure(static_cast<size_t>(ncap) * d * 4, "hipMalloc(frontier points)");
+    if (keep > 0) {
+        for (int o = 0; o < d; ++o) {
+            hip_check(hipMemcpyAsync(nb.as<int32_t>() + o * ncap, f.pts.as<int32_t>() + o * f.cap,
+                                     static_cast<size_t>(keep) * 4, hipMemcpyDeviceToDevice, s),
+                      "hipMemcpy(frontier grow)");
+        }
+    }
+    std::swap(f.pts.p, nb.p);
+    std::swap(f.pts.bytes, nb.bytes);
+    f.cap = ncap;
+    hip_check(hipStreamSynchronize(s), "frontier grow sync");
+}
+
+void upload_offsets(DevFrontier &f, hipStream_t s) {
+    f.off.ensure(f.h_off.size() * 8, "hipMalloc(frontier offsets)");
+    hip_check(hipMemcpyAsync(f.off.p, f.h_off.data(), f.h_off.size() * 8, hipMemcpyHostToDevice,
+                             s),
+              "hipMemcpy(frontier offsets)");
+}
+
+// Frontier holding {0^d} at one node of a layer, empty elsewhere.
+void seed_frontier(DevFrontier &f, int d, int64_t n_nodes, int64_t node, hipStream_t s) {
+    f.n_nodes = n_nodes;
+    f.h_off.assign(n_nodes + 1, 0);
+    for (int64_t v = node + 1; v <= n_nodes; ++v) {
+        f.h_off[v] = 1;
+    }
+    f.cap = 0;
+    f.pts.release();
+    grow_points(f, d, 1, 0, s);
+    hip_check(hipMemsetAsync(f.pts.p, 0, static_cast<size_t>(f.cap) * d * 4, s),
+              "hipMemset(seed)");
+    upload_offsets(f, s);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel launchers, one instantiation per objective count.
+// ---------------------------------------------------------------------------
+struct Launch {
+    int chunk_cap;
+    void (*filter_expand)(dim3, hipStream_t, const ChunkDesc *, pmdd_k::ExpandSrc, int32_t *,
+                          int64_t, uint8_t *, int32_t *, int32_t *, int64_t *, int64_t *,
+                          unsigned long long *);
+    void (*filter_join)(dim3, hipStream_t, const ChunkDesc *, pmdd_k::JoinSrc, int32_t *, int64_t,
+                        uint8_t *, int32_t *, int32_t *, int64_t *, int64_t *,
+                        unsigned long long *);
+    void (*cross)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *, int64_t,
+                  uint8_t *, const int32_t *, int32_t *, const int64_t *, const int64_t *,
+                  unsigned long long *);
+    void (*compact)(dim3, hipStream_t, const ChunkDesc *, const int32_t *, const int64_t *,
+                    const int32_t *, int64_t, const uint8_t *, int32_t *, int64_t);
+    void (*state_dom)(dim3, hipStream_t, const int32_t *, const int64_t *, const int32_t *,
+                      int64_t, uint8_t *);
+    void (*compact_nodes)(dim3, hipStream_t, const int64_t *, const int32_t *, int64_t,
+                          const uint8_t *, const int64_t *, int32_t *, int64_t);
+    void (*vs_frontier)(dim3, hipStream_t, const int32_t *, int64_t, int64_t, const int32_t *,
+                        int64_t, int64_t, uint8_t *, uint8_t *, int64_t, unsigned long long *);
+};
+
+template <int D> struct LaunchImpl {
+    static constexpr int CH = pmdd_k::Cap<D>::CH;
+    static void filter_expand(dim3 g, hipStream_t s, const ChunkDesc *c, pmdd_k::ExpandSrc src,
+                              int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns,
+                              int32_t *na, int64_t *kmin, int64_t *kmax,
+                              unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, CH, pmdd_k::ExpandSrc>), g, dim3(kBlock),
+                           0, s, c, src, out, stride, alive, ns, na, kmin, kmax, pt);
+    }
+    static void filter_join(dim3 g, hipStream_t s, const ChunkDesc *c, pmdd_k::JoinSrc src,
+                            int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns, int32_t *na,
+                            int64_t *kmin, int64_t *kmax, unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, CH, pmdd_k::JoinSrc>), g, dim3(kBlock), 0,
+                           s, c, src, out, stride, alive, ns, na, kmin, kmax, pt);
+    }
+    static void cross(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
+                      const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
+                      int32_t *na, const int64_t *kmin, const int64_t *kmax,
+                      unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_cross_chunks<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
+                           stride, alive, ns, na, kmin, kmax, pt);
+    }
+    static void compact(dim3 g, hipStream_t s, const ChunkDesc *c, const int32_t *ns,
+                        const int64_t *out, const int32_t *src, int64_t ss, const uint8_t *alive,
+                        int32_t *dst, int64_t ds) {
+        hipLaunchKernelGGL((pmdd_k::k_compact_chunks<D>), g, dim3(kBlock), 0, s, c, ns, out, src,
+                           ss, alive, dst, ds);
+    }
+    static void state_dom(dim3 g, hipStream_t s, const int32_t *t, const int64_t *off,
+                          const int32_t *pts, int64_t stride, uint8_t *keep) {
+        hipLaunchKernelGGL((pmdd_k::k_state_dominance<D>), g, dim3(kBlock), 0, s, t, off, pts,
+                           stride, keep);
+    }
+    static void compact_nodes(dim3 g, hipStream_t s, const int64_t *oi, const int32_t *src,
+                              int64_t ss, const uint8_t *keep, const int64_t *oo, int32_t *dst,
+                              int64_t ds) {
+        hipLaunchKernelGGL((pmdd_k::k_compact_nodes<D>), g, dim3(kBlock), 0, s, oi, src, ss, keep,
+                           oo, dst, ds);
+    }
+    static void vs_frontier(dim3 g, hipStream_t s, const int32_t *B, int64_t bs, int64_t nb,
+                            const int32_t *R, int64_t rs, int64_t nr, uint8_t *kb, uint8_t *kr,
+                            int64_t nblk, unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_vs_frontier<D>), g, dim3(kBlock), 0, s, B, bs, nb, R, rs, nr,
+                           kb, kr, nblk, pt);
+    }
+    static Launch table() {
+        return Launch{CH, filter_expand, filter_join, cross, compact, state_dom, compact_nodes,
+                      vs_frontier};
+    }
+};
+
+Launch launch_for(int d) {
+    switch (d) {
+    case 1: return LaunchImpl<1>::table();
+    case 2: return LaunchImpl<2>::table();
+    case 3: return LaunchImpl<3>::table();
+    case 4: return LaunchImpl<4>::table();
+    case 5: return LaunchImpl<5>::table();
+    case 6: return LaunchImpl<6>::table();
+    case 7: return LaunchImpl<7>::table();
+    case 8: return LaunchImpl<8>::table();
+    default: fail(PMDD_E_UNSUPPORTED, "objective count outside 1..8");
+    }
+}
+
+template <class T> void h2d(DevBuf &b, const std::vector<T> &v, hipStream_t s, const char *what) {
+    b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T), what);
+    if (!v.empty()) {
+        hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s),
+                  what);
+    }
+}
+
+template <class T> void d2h(std::vector<T> &v, const DevBuf &b, size_t n, hipStream_t s,
+                            const char *what) {
+    v.resize(n);
+    if (n > 0) {
+        hip_check(hipMemcpyAsync(v.data(), b.p, n * sizeof(T), hipMemcpyDeviceToHost, s), what);
+    }
+    hip_check(hipStreamSynchronize(s), what);
+}
+
+// ---------------------------------------------------------------------------
+// One enumeration
+// ---------------------------------------------------------------------------
+struct Run {
+    pmdd_ctx *ctx;
+    pmdd_resident *dd;
+    pmdd_opts opts;
+    pmdd_stats *st;
+    Launch K;
+    int d;
+    hipStream_t s;
+    int64_t mem_base = 0;
+    unsigned long long *d_tests = nullptr;
+
+    Run(pmdd_ctx *c, pmdd_resident *r, const pmdd_opts &o, pmdd_stats *stats)
+        : ctx(c), dd(r), opts(o), st(stats), K(launch_for(r->d)), d(r->d), s(c->stream) {
+        if (opts.max_cand <= 0) {
+            opts.max_cand = PMDD_DEFAULT_MAX_CAND;
+        }
+        if (opts.max_prod <= 0) {
+            opts.max_prod = PMDD_DEFAULT_MAX_PROD;
+        }
+        size_t fr = 0, tot = 0;
+        hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+        mem_base = static_cast<int64_t>(tot) - static_cast<int64_t>(fr);
+        st->gpu_mem_peak_reserved_bytes = mem_base;
+        if (opts.count_pair_tests) {
+            ctx->counter.ensure(8, "hipMalloc(counter)");
+            d_tests = ctx->counter.as<unsigned long long>();
+            hip_check(hipMemsetAsync(d_tests, 0, 8, s), "hipMemset(counter)");
+        }
+    }
+
+    void sample_memory() {
+        size_t fr = 0, tot = 0;
+        hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+        const int64_t used = static_cast<int64_t>(tot) - static_cast<int64_t>(fr);
+        st->gpu_mem_peak_reserved_bytes = std::max(st->gpu_mem_peak_reserved_bytes, used);
+        st->gpu_mem_peak_used_bytes =
+            std::max(st->gpu_mem_peak_used_bytes, std::max<int64_t>(0, used - mem_base));
+    }
+
+    float elapsed_ms() {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b), "hipEventElapsedTime");
+        return ms;
+    }
+
+    // Filter `chunks` of the pass (candidates loaded by `load`), then compact
+    // the alive survivors into `next` after its current `next_total` points.
+    // Returns the alive count per chunk (host).
+    template <class LoadFn>
+    void filter_pass(const std::vector<ChunkDesc> &chunks, int64_t pass_cands, LoadFn load,
+                     std::vector<int32_t> &nalive_h) {
+        const int64_t nch = static_cast<int64_t>(chunks.size());
+        h2d(ctx->chunks, chunks, s, "hipMemcpy(chunk table)");
+        ctx->surv_pts.ensure(static_cast<size_t>(pass_cands) * d * 4, "hipMalloc(survivor buffer)");
+        ctx->alive.ensure(static_cast<size_t>(pass_cands), "hipMalloc(alive flags)");
+        ctx->nsurv.ensure(nch * 4, "hipMalloc(chunk counts)");
+        ctx->nalive.ensure(nch * 4, "hipMalloc(chunk counts)");
+        ctx->minkey.ensure(nch * 8, "hipMalloc(chunk keys)");
+        ctx->maxkey.ensure(nch * 8, "hipMalloc(chunk keys)");
+        hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+        load(dim3(static_cast<unsigned>(nch)));
+        hip_check(hipGetLastError(), "k_filter_chunks");
+        std::vector<int32_t> which;
+        for (int64_t c = 0; c < nch; ++c) {
+            if (chunks[c].group_count > 1) {
+                which.push_back(static_cast<int32_t>(c));
+            }
+        }
+        if (!which.empty()) {
+            h2d(ctx->which, which, s, "hipMemcpy(cross list)");
+            K.cross(dim3(static_cast<unsigned>(which.size())), s, ctx->which.as<int32_t>(),
+                    ctx->chunks.as<ChunkDesc>(), ctx->surv_pts.as<int32_t>(), pass_cands,
+                    ctx->alive.as<uint8_t>(), ctx->nsurv.as<int32_t>(), ctx->nalive.as<int32_t>(),
+                    ctx->minkey.as<int64_t>(), ctx->maxkey.as<int64_t>(), d_tests);
+            hip_check(hipGetLastError(), "k_cross_chunks");
+        }
+        hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+        d2h(nalive_h, ctx->nalive, nch, s, "chunk survivor counts");
+        const double ms = elapsed_ms();
+        st->kernel_dominance_s += ms * 1e-3;
+        st->kernel_total_s += ms * 1e-3;
+    }
+
+    void compact_pass(const std::vector<ChunkDesc> &chunks, int64_t pass_cands,
+                      const std::vector<int64_t> &chunk_out, int32_t *dst, int64_t dst_stride) {
+        const int64_t nch = static_cast<int64_t>(chunks.size());
+        h2d(ctx->chunk_out, chunk_out, s, "hipMemcpy(chunk outputs)");
+        hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+        K.compact(dim3(static_cast<unsigned>(nch)), s, ctx->chunks.as<ChunkDesc>(),
+                  ctx->nsurv.as<int32_t>(), ctx->chunk_out.as<int64_t>(),
+                  ctx->surv_pts.as<int32_t>(), pass_cands, ctx->alive.as<uint8_t>(), dst,
+                  dst_stride);
+        hip_check(hipGetLastError(), "k_compact_chunks");
+        hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+        hip_check(hipStreamSynchronize(s), "compaction sync");
+        const double ms = elapsed_ms();
+        st->kernel_expand_td_s += ms * 1e-3;
+        st->kernel_total_s += ms * 1e-3;
+    }
+
+    // One layer expansion (expand_layer_frontiers, bottomup.cu:242-533):
+    // frontier of every node of the target layer from its arcs' parents.
+    // Passes cover runs of whole nodes with at most max_cand candidates (a
+    // single node may exceed it, as in the reference).
+    void expand(const DevFrontier &prev, const LayerCSR &csr, int64_t n_nodes, DevFrontier &next,
+                int64_t *cand_out, std::vector<int64_t> *gamma_out) {
+        const int CH = K.chunk_cap;
+        // candidates per node and the node-local start of each arc's run
+        std::vector<int64_t> run_off(std::max<int64_t>(csr.n_arcs, 1));
+        std::vector<int64_t> gamma(n_nodes, 0);
+        int64_t C = 0;
+        for (int64_t v = 0; v < n_nodes; ++v) {
+            int64_t acc = 0;
+            for (int64_t e = csr.off[v]; e < csr.off[v + 1]; ++e) {
+                run_off[e] = acc;
+                acc += prev.size(csr.src[e]);
+            }
+            gamma[v] = acc;
+            C += acc;
+        }
+        *cand_out = C;
+        h2d(ctx->run_off, run_off, s, "hipMemcpy(run offsets)");
+
+        next.n_nodes = n_nodes;
+        next.h_off.assign(n_nodes + 1, 0);
+        int64_t next_total = 0;
+        if (next.cap < 1) {
+            grow_points(next, d, std::max<int64_t>(1024, std::min<int64_t>(C, 2 * prev.total())),
+                        0, s);
+        }
+
+        pmdd_k::ExpandSrc src;
+        src.prev_off = prev.off.as<int64_t>();
+        src.prev_pts = prev.pts.as<int32_t>();
+        src.prev_stride = prev.cap;
+        src.arc_off = csr.d_off;
+        src.arc_src = csr.d_src;
+        src.arc_w = csr.d_w;
+        src.w_stride = std::max<int64_t>(csr.n_arcs, 1);
+        src.run_off = ctx->run_off.as<int64_t>();
+
+        int64_t v = 0;
+        std::vector<ChunkDesc> chunks;
+        std::vector<int32_t> nalive;
+        std::vector<int64_t> chunk_out;
+        while (v < n_nodes) {
+            // plan one pass
+            const int64_t v0 = v;
+            int64_t pass_c = 0;
+            while (v < n_nodes && (pass_c == 0 || pass_c + gamma[v] <= opts.max_cand)) {
+                pass_c += gamma[v];
+                ++v;
+            }
+            const int64_t v1 = v;
+            chunks.clear();
+            int64_t base = 0;
+            for (int64_t u = v0; u < v1; ++u) {
+                const int64_t g = gamma[u];
+                if (g == 0) {
+                    continue;
+                }
+                const int32_t first = static_cast<int32_t>(chunks.size());
+                const int32_t cnt = static_cast<int32_t>(ceil_div(g, CH));
+                for (int32_t k = 0; k < cnt; ++k) {
+                    ChunkDesc c;
+                    c.start = static_cast<int64_t>(k) * CH;
+                    c.len = static_cast<int32_t>(std::min<int64_t>(CH, g - c.start));
+                    c.out_base = base + c.start;
+                    c.group = static_cast<int32_t>(u);
+                    c.group_first = first;
+                    c.group_count = cnt;
+                    chunks.push_back(c);
+                }
+                base += g;
+            }
+            if (chunks.empty()) {
+                for (int64_t u = v0; u < v1; ++u) {
+                    next.h_off[u + 1] = next_total;
+                }
+                continue;
+            }
+            filter_pass(
+                chunks, pass_c,
+                [&](dim3 grid) {
+                    K.filter_expand(grid, s, ctx->chunks.as<ChunkDesc>(), src,
+                                    ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>(),
+                                    ctx->nsurv.as<int32_t>(), ctx->nalive.as<int32_t>(),
+                                    ctx->minkey.as<int64_t>(), ctx->maxkey.as<int64_t>(), d_tests);
+                },
+                nalive);
+            // dense placement of the survivors, node by node
+            chunk_out.resize(chunks.size());
+            int64_t at = next_total;
+            size_t ci = 0;
+            for (int64_t u = v0; u < v1; ++u) {
+                while (ci < chunks.size() && chunks[ci].group == u) {
+                    chunk_out[ci] = at;
+                    at += nalive[ci];
+                    ++ci;
+                }
+                next.h_off[u + 1] = at;
+            }
+            grow_points(next, d, at, next_total, s);
+            compact_pass(chunks, pass_c, chunk_out, next.pts.as<int32_t>(), next.cap);
+            next_total = at;
+            sample_memory();
+        }
+        upload_offsets(next, s);
+        if (gamma_out != nullptr) {
+            gamma_out->swap(gamma);
+        }
+    }
+
+    // Knapsack state dominance on a top-down layer (filter_dominance_knapsack_cuda,
+    // topdown.cu:363-550; rule of filter_dominance_knapsack_cpu, dominance.cpp:24-86).
+    void state_dominance(int l, DevFrontier &f) {
+        const int64_t n = f.n_nodes;
+        if (n <= 1) {
+            return;
+        }
+        std::vector<int64_t> active;
+        for (int64_t v = 0; v < n; ++v) {
+            if (f.size(v) > 0) {
+                active.push_back(v);
+            }
+        }
+        if (active.size() <= 1) {
+            return;
+        }
+        const std::vector<int32_t> &mw = dd->min_weight[l];
+        std::stable_sort(active.begin(), active.end(),
+                         [&](int64_t a, int64_t b) { return mw[a] > mw[b]; });
+        const std::vector<int32_t> &sp = dd->single_parent[l];
+        const std::vector<int32_t> &sa = dd->single_arc[l];
+        std::vector<int32_t> targets;
+        for (size_t i = 0; i + 1 < active.size(); ++i) {
+            const int64_t n1 = active[i];
+            int32_t cmp[2] = {-1, -1};
+            int k = 0;
+            for (size_t j = i + 1; j < std::min(active.size(), i + 3); ++j) {
+                const int64_t n2 = active[j];
+                if (sp[n1] >= 0 && sp[n2] >= 0 && sp[n1] == sp[n2] && sa[n1] != sa[n2]) {
+                    continue;
+                }
+                cmp[k++] = static_cast<int32_t>(n2);
+            }
+            if (k == 0) {
+                continue;
+            }
+            targets.push_back(static_cast<int32_t>(n1));
+            targets.push_back(cmp[0]);
+            targets.push_back(cmp[1]);
+        }
+        if (targets.empty()) {
+            return;
+        }
+        const int64_t total = f.total();
+        ctx->keep_a.ensure(std::max<int64_t>(total, 1), "hipMalloc(keep flags)");
+        hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+        hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(total, 256))),
+                           dim3(256), 0, s, ctx->keep_a.as<uint8_t>(), total, uint8_t(1));
+        h2d(ctx->which, targets, s, "hipMemcpy(dominance targets)");
+        K.state_dom(dim3(static_cast<unsigned>(targets.size() / 3)), s, ctx->which.as<int32_t>(),
+                    f.off.as<int64_t>(), f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>());
+        hip_check(hipGetLastError(), "k_state_dominance");
+        // kept per node
+        ctx->join_meta.ensure(static_cast<size_t>(n) * 8, "hipMalloc(kept counts)");
+        hipLaunchKernelGGL(pmdd_k::k_count_kept, dim3(static_cast<unsigned>(n)), dim3(kBlock), 0,
+                           s, f.off.as<int64_t>(), ctx->keep_a.as<uint8_t>(),
+                           ctx->join_meta.as<int64_t>(), n);
+        hip_check(hipGetLastError(), "k_count_kept");
+        hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+        std::vector<int64_t> kept;
+        d2h(kept, ctx->join_meta, n, s, "kept counts");
+        const double ms = elapsed_ms();
+        st->kernel_dominance_s += ms * 1e-3;
+        st->kernel_total_s += ms * 1e-3;
+        DevFrontier g;
+        g.n_nodes = n;
+        g.h_off.assign(n + 1, 0);
+        for (int64_t v = 0; v < n; ++v) {
+            g.h_off[v + 1] = g.h_off[v] + kept[v];
+        }
+        const int64_t dropped = total - g.total();
+        if (dropped == 0) {
+            return;
+        }
+        st->dominance_filtered_total += dropped;
+        grow_points(g, d, std::max<int64_t>(g.total(), 1), 0, s);
+        upload_offsets(g, s);
+        K.compact_nodes(dim3(static_cast<unsigned>(n)), s, f.off.as<int64_t>(),
+                        f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>(),
+                        g.off.as<int64_t>(), g.pts.as<int32_t>(), g.cap);
+        hip_check(hipGetLastError(), "k_compact_nodes");
+        hip_check(hipStreamSynchronize(s), "state dominance sync");
+        std::swap(f.h_off, g.h_off);
+        std::swap(f.off.p, g.off.p);
+        std::swap(f.off.bytes, g.off.bytes);
+        std::swap(f.pts.p, g.pts.p);
+        std::swap(f.pts.bytes, g.pts.bytes);
+        std::swap(f.cap, g.cap);
+    }
+
+    static double pop_std(const std::vector<int64_t> &x) {
+        if (x.empty()) {
+            return 0.0;
+        }
+        double m = 0.0;
+        for (int64_t v : x) {
+            m += static_cast<double>(v);
+        }
+        m /= static_cast<double>(x.size());
+        double acc = 0.0;
+        for (int64_t v : x) {
+            acc += (static_cast<double>(v) - m) * (static_cast<double>(v) - m);
+        }
+        return std::sqrt(acc / static_cast<double>(x.size()));
+    }
+
+    void count_layer(int64_t cand, int64_t surv) {
+        st->work_candidates_total += cand;
+        st->work_candidates_peak = std::max(st->work_candidates_peak, cand);
+        st->work_frontier_survivors_total += surv;
+        st->work_frontier_peak_points = std::max(st->work_frontier_peak_points, surv);
+    }
+
+    void fetch_points(const DevFrontier &f, int64_t begin, int64_t count, pmdd_frontier *out) {
+        std::vector<int32_t> soa(static_cast<size_t>(count) * d);
+        for (int o = 0; o < d && count > 0; ++o) {
+            hip_check(hipMemcpyAsync(soa.data() + o * count, f.pts.as<int32_t>() + o * f.cap + begin,
+                                     static_cast<size_t>(count) * 4, hipMemcpyDeviceToHost, s),
+                      "hipMemcpy(result)");
+        }
+        hip_check(hipStreamSynchronize(s), "result sync");
+        out->d = d;
+        out->count = count;
+        out->points = static_cast<int32_t *>(std::malloc(sizeof(int32_t) * std::max<int64_t>(count * d, 1)));
+        if (out->points == nullptr) {
+            fail(PMDD_E_NOMEM, "host allocation of the result failed");
+        }
+        for (int64_t i = 0; i < count; ++i) {
+            for (int o = 0; o < d; ++o) {
+                out->points[i * d + o] = soa[o * count + i];
+            }
+        }
+    }
+
+    // Method 1 (enumerate_bdd_topdown / enumerate_mdd_topdown).
+    void topdown(pmdd_frontier *out) {
+        const int L = dd->L;
+        DevFrontier cur, nxt;
+        seed_frontier(cur, d, dd->layer_size[0], dd->root, s);
+        const bool sd = opts.state_dominance > 0 && opts.problem_type == 1 &&
+                        dd->kind == PMDD_DD_BDD && !dd->min_weight.empty();
+        for (int l = 1; l < L; ++l) {
+            int64_t cand = 0;
+            std::vector<int64_t> gamma;
+            expand(cur, dd->td[l], dd->layer_size[l], nxt, &cand, &gamma);
+            if (sd) {
+                state_dominance(l, nxt);
+            }
+            count_layer(cand, nxt.total());
+            if (st->std_candidates_per_layer != nullptr && l < st->std_capacity) {
+                st->std_candidates_per_layer[l] = pop_std(gamma);
+            }
+            if (st->std_frontier_survivors_per_layer != nullptr && l < st->std_capacity) {
+                std::vector<int64_t> sz(nxt.n_nodes);
+                for (int64_t v = 0; v < nxt.n_nodes; ++v) {
+                    sz[v] = nxt.size(v);
+                }
+                st->std_frontier_survivors_per_layer[l] = pop_std(sz);
+            }
+            std::swap(cur, nxt);
+        }
+        fetch_points(cur, cur.h_off[dd->terminal], cur.size(dd->terminal), out);
+    }
+
+    // Method 3 (coupled_bdd_cuda_enumerate enum.cu:595-756, enumerate_mdd_coupled
+    // enum.cu:295-436): expand top-down while its score is not above the
+    // bottom-up score, else bottom-up; the scores are sum |F(v)| * out-arcs(v)
+    // and trunc(1.5 * sum |F(v)| * in-arcs(v)); stop where the sweeps meet.
+    void coupled(pmdd_frontier *out) {
+        const int L = dd->L;
+        DevFrontier td, bu, tmp;
+        seed_frontier(td, d, dd->layer_size[0], dd->root, s);
+        seed_frontier(bu, d, dd->layer_size[L - 1], dd->terminal, s);
+        int lt = 0, lb = L - 1;
+        int64_t td_score = 0, bu_score = 0;
+        while (lt != lb) {
+            int64_t cand = 0;
+            if (td_score <= bu_score) {
+                ++lt;
+                expand(td, dd->td[lt], dd->layer_size[lt], tmp, &cand, nullptr);
+                std::swap(td, tmp);
+                count_layer(cand, td.total());
+                const LayerCSR &outs = dd->bu[lt];
+                int64_t score = 0;
+                for (int64_t v = 0; v < td.n_nodes; ++v) {
+                    const int64_t deg = outs.off.empty() ? 0 : outs.off[v + 1] - outs.off[v];
+                    score += td.size(v) * deg;
+                }
+                td_score = score;
+            } else {
+                --lb;
+                expand(bu, dd->bu[lb], dd->layer_size[lb], tmp, &cand, nullptr);
+                std::swap(bu, tmp);
+                count_layer(cand, bu.total());
+                const LayerCSR &ins = dd->td[lb];
+                int64_t score = 0;
+                for (int64_t v = 0; v < bu.n_nodes; ++v) {
+                    const int64_t deg = ins.off.empty() ? 0 : ins.off[v + 1] - ins.off[v];
+                    score += bu.size(v) * deg;
+                }
+                bu_score = static_cast<int64_t>(1.5 * static_cast<double>(score));
+            }
+        }
+        st->layer_coupling = lt;
+        const double t0 = now_s();
+        couple(td, bu, out);
+        st->wall_join_s += now_s() - t0;
+    }
+
+    // Cutset coupling (couple_cutsets_cuda, couple.cu:340-498): the result is
+    // the nondominated set of the union over cutset nodes of TD(v) + BU(v).
+    // Nodes with both frontiers non-empty, largest |TD|+|BU| first, go in
+    // batches of at most max_prod products (a single node may exceed it).  Each
+    // batch is filtered on its own, then merged into the running frontier R.
+    void couple(const DevFrontier &td, const DevFrontier &bu, pmdd_frontier *out) {
+        const int CH = K.chunk_cap;
+        const int64_t n = td.n_nodes;
+        std::vector<int64_t> nodes;
+        int64_t total_products = 0;
+        for (int64_t v = 0; v < n; ++v) {
+            const int64_t p = td.size(v) * bu.size(v);
+            total_products += p;
+            if (p > 0) {
+                nodes.push_back(v);
+            }
+        }
+        st->work_join_products_total += total_products;
+        std::stable_sort(nodes.begin(), nodes.end(), [&](int64_t a, int64_t b) {
+            return td.size(a) + bu.size(a) > td.size(b) + bu.size(b);
+        });
+
+        DevFrontier R, Rn, B;
+        R.n_nodes = Rn.n_nodes = B.n_nodes = 1;
+        int64_t r_count = 0;
+        size_t i = 0;
+        std::vector<ChunkDesc> chunks;
+        std::vector<int32_t> nalive;
+        std::vector<int64_t> chunk_out;
+        while (i < nodes.size()) {
+            int64_t bprod = 0;
+            size_t j = i;
+            while (j < nodes.size()) {
+                const int64_t p = td.size(nodes[j]) * bu.size(nodes[j]);
+                if (bprod + p > opts.max_prod && bprod > 0) {
+                    break;
+                }
+                bprod += p;
+                ++j;
+            }
+            const int32_t Kn = static_cast<int32_t>(j - i);
+            std::vector<int64_t> meta(4 * static_cast<size_t>(Kn) + 1);
+            int64_t *prod_off = meta.data();
+            int64_t *td_begin = meta.data() + Kn + 1;
+            int64_t *bu_begin = td_begin + Kn;
+            int64_t *bu_size = bu_begin + Kn;
+            prod_off[0] = 0;
+            for (int32_t k = 0; k < Kn; ++k) {
+                const int64_t v = nodes[i + k];
+                prod_off[k + 1] = prod_off[k] + td.size(v) * bu.size(v);
+                td_begin[k] = td.h_off[v];
+                bu_begin[k] = bu.h_off[v];
+                bu_size[k] = bu.size(v);
+            }
+            h2d(ctx->join_meta, meta, s, "hipMemcpy(batch table)");
+            pmdd_k::JoinSrc src;
+            src.prod_off = ctx->join_meta.as<int64_t>();
+            src.td_begin = src.prod_off + Kn + 1;
+            src.bu_begin = src.td_begin + Kn;
+            src.bu_size = src.bu_begin + Kn;
+            src.K = Kn;
+            src.td_pts = td.pts.as<int32_t>();
+            src.td_stride = td.cap;
+            src.bu_pts = bu.pts.as<int32_t>();
+            src.bu_stride = bu.cap;
+
+            // the whole batch is one group: chunks may straddle nodes
+            chunks.clear();
+            const int32_t cnt = static_cast<int32_t>(ceil_div(bprod, CH));
+            for (int32_t k = 0; k < cnt; ++k) {
+                ChunkDesc c;
+                c.start = static_cast<int64_t>(k) * CH;
+                c.len = static_cast<int32_t>(std::min<int64_t>(CH, bprod - c.start));
+                c.out_base = c.start;
+                c.group = 0;
+                c.group_first = 0;
+                c.group_count = cnt;
+                chunks.push_back(c);
+            }
+            filter_pass(
+                chunks, bprod,
+                [&](dim3 grid) {
+                    K.filter_join(grid, s, ctx->chunks.as<ChunkDesc>(), src,
+                                  ctx->surv_pts.as<int32_t>(), bprod, ctx->alive.as<uint8_t>(),
+                                  ctx->nsurv.as<int32_t>(), ctx->nalive.as<int32_t>(),
+                                  ctx->minkey.as<int64_t>(), ctx->maxkey.as<int64_t>(), d_tests);
+                },
+                nalive);
+            chunk_out.resize(chunks.size());
+            int64_t at = 0;
+            for (size_t c = 0; c < chunks.size(); ++c) {
+                chunk_out[c] = at;
+                at += nalive[c];
+            }
+            const int64_t b_count = at;
+            grow_points(B, d, std::max<int64_t>(b_count, 1), 0, s);
+            compact_pass(chunks, bprod, chunk_out, B.pts.as<int32_t>(), B.cap);
+
+            if (r_count == 0) {
+                std::swap(R.pts.p, B.pts.p);
+                std::swap(R.pts.bytes, B.pts.bytes);
+                std::swap(R.cap, B.cap);
+                r_count = b_count;
+            } else {
+                r_count = merge_running(R, r_count, B, b_count, Rn);
+            }
+            sample_memory();
+            i = j;
+        }
+        fetch_points(R, 0, r_count, out);
+        count_layer(0, r_count);
+    }
+
+    // R <- (R minus points strictly dominated by B) ++ (B minus points weakly
+    // dominated by R).  Returns the new size.
+    int64_t merge_running(DevFrontier &R, int64_t nr, DevFrontier &B, int64_t nb, DevFrontier &Rn) {
+        ctx->keep_a.ensure(std::max<int64_t>(nb, 1), "hipMalloc(keep flags)");
+        ctx->keep_b.ensure(std::max<int64_t>(nr, 1), "hipMalloc(keep flags)");
+        const int64_t blk_b = ceil_div(nb, kBlock);
+        const int64_t blk_r = ceil_div(nr, kBlock);
+        hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+        K.vs_frontier(dim3(static_cast<unsigned>(blk_b + blk_r)), s, B.pts.as<int32_t>(), B.cap,
+                      nb, R.pts.as<int32_t>(), R.cap, nr, ctx->keep_a.as<uint8_t>(),
+                      ctx->keep_b.as<uint8_t>(), blk_b, d_tests);
+        hip_check(hipGetLastError(), "k_vs_frontier");
+        hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+        std::vector<uint8_t> kb, kr;
+        d2h(kb, ctx->keep_a, nb, s, "keep flags");
+        d2h(kr, ctx->keep_b, nr, s, "keep flags");
+        const double ms = elapsed_ms();
+        st->kernel_dominance_s += ms * 1e-3;
+        st->kernel_total_s += ms * 1e-3;
+        // compact on the device with per-point chunk tables of one chunk per
+        // 2048 points: reuse k_compact_chunks
+        const int64_t CHC = 2048;
+        auto compact_into = [&](const DevFrontier &srcf, int64_t n, const std::vector<uint8_t> &keep,
+                                DevBuf &flags, int64_t out_at) -> int64_t {
+            std::vector<ChunkDesc> cs;
+            std::vector<int32_t> ns;
+            std::vector<int64_t> outs;
+            int64_t at = out_at;
+            for (int64_t b0 = 0; b0 < n; b0 += CHC) {
+                ChunkDesc c;
+                c.start = b0;
+                c.out_base = b0;
+                c.len = static_cast<int32_t>(std::min<int64_t>(CHC, n - b0));
+                c.group = 0;
+                c.group_first = 0;
+                c.group_count = 1;
+                cs.push_back(c);
+                ns.push_back(c.len);
+                outs.push_back(at);
+                for (int64_t k = b0; k < b0 + c.len; ++k) {
+                    at += keep[k];
+                }
+            }
+            if (cs.empty()) {
+                return at;
+            }
+            h2d(ctx->chunks, cs, s, "hipMemcpy(chunk table)");
+            h2d(ctx->nsurv, ns, s, "hipMemcpy(chunk counts)");
+            h2d(ctx->chunk_out, outs, s, "hipMemcpy(chunk outputs)");
+            K.compact(dim3(static_cast<unsigned>(cs.size())), s, ctx->chunks.as<ChunkDesc>(),
+                      ctx->nsurv.as<int32_t>(), ctx->chunk_out.as<int64_t>(),
+                      srcf.pts.as<int32_t>(), srcf.cap, flags.as<uint8_t>(), Rn.pts.as<int32_t>(),
+                      Rn.cap);
+            hip_check(hipGetLastError(), "k_compact_chunks");
+            hip_check(hipStreamSynchronize(s), "merge compaction sync");
+            return at;
+        };
+        int64_t kept_r = 0, kept_b = 0;
+        for (uint8_t x : kr) {
+            kept_r += x;
+        }
+        for (uint8_t x : kb) {
+            kept_b += x;
+        }
+        Rn.cap = 0;
+        Rn.pts.release();
+        grow_points(Rn, d, std::max<int64_t>(kept_r + kept_b, 1), 0, s);
+        int64_t at = compact_into(R, nr, kr, ctx->keep_b, 0);
+        at = compact_into(B, nb, kb, ctx->keep_a, at);
+        std::swap(R.pts.p, Rn.pts.p);
+        std::swap(R.pts.bytes, Rn.pts.bytes);
+        std::swap(R.cap, Rn.cap);
+        return at;
+    }
+
+    void finish() {
+        if (d_tests != nullptr) {
+            unsigned long long t = 0;
+            hip_check(hipMemcpyAsync(&t, d_tests, 8, hipMemcpyDeviceToHost, s), "hipMemcpy(counter)");
+            hip_check(hipStreamSynchronize(s), "counter sync");
+            st->dominance_pair_tests = static_cast<int64_t>(t);
+        }
+        sample_memory();
+    }
+};
+
+void write_err(char *err, size_t errlen, const std::string &msg) {
+    if (err != nullptr && errlen > 0) {
+        std::snprintf(err, errlen, "%s", msg.c_str());
+    }
+}
+
+void reset_stats(pmdd_stats *st) {
+    double *a = st->std_candidates_per_layer;
+    double *b = st->std_frontier_survivors_per_layer;
+    const int32_t cap = st->std_capacity;
+    std::memset(st, 0, sizeof(*st));
+    st->std_candidates_per_layer = a;
+    st->std_frontier_survivors_per_layer = b;
+    st->std_capacity = cap;
+    if (cap > 0) {
+        if (a != nullptr) {
+            std::fill(a, a + cap, 0.0);
+        }
+        if (b != nullptr) {
+            std::fill(b, b + cap, 0.0);
+        }
+    }
+}
+
+int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmdd_frontier *out,
+                    pmdd_stats *stats) {
+    if (opts == nullptr || out == nullptr) {
+        fail(PMDD_E_INVALID, "NULL options or output");
+    }
+    if (opts->method != 1 && opts->method != 3) {
+        fail(PMDD_E_UNSUPPORTED,
+             "GPU backend is unsupported for method " + std::to_string(opts->method) + ".");
+    }
+    if (dd->device != ctx->device) {
+        fail(PMDD_E_INVALID, "diagram was uploaded through a context of another device");
+    }
+    pmdd_stats local{};
+    pmdd_stats *st = stats != nullptr ? stats : &local;
+    reset_stats(st);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    out->d = dd->d;
+    out->count = 0;
+    out->points = nullptr;
+    Run run(ctx, dd, *opts, st);
+    if (opts->method == 1) {
+        run.topdown(out);
+    } else {
+        run.coupled(out);
+    }
+    run.finish();
+    return PMDD_OK;
+}
+
+template <class F> int guarded(char *err, size_t errlen, F &&f) {
+    try {
+        const int rc = f();
+        if (rc == PMDD_OK) {
+            write_err(err, errlen, "");
+        }
+        return rc;
+    } catch (const EngineError &e) {
+        write_err(err, errlen, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        write_err(err, errlen, "host allocation failed");
+        return PMDD_E_NOMEM;
+    } catch (const std::exception &e) {
+        write_err(err, errlen, e.what());
+        return PMDD_E_INVALID;
+    }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int pmdd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
+    return guarded(err, errlen, [&]() -> int {
+        if (out == nullptr) {
+            fail(PMDD_E_INVALID, "pmdd_ctx_create: NULL output");
+        }
+        *out = nullptr;
+        int n = 0;
+        const hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(PMDD_E_NODEVICE, std::string("hipGetDeviceCount failed: ") + hipGetErrorString(e));
+        }
+        if (n <= 0) {
+            fail(PMDD_E_NODEVICE, "No HIP device found");
+        }
+        if (device < 0 || device >= n) {
+            fail(PMDD_E_NODEVICE, "HIP device " + std::to_string(device) + " not present");
+        }
+        std::unique_ptr<pmdd_ctx> c(new pmdd_ctx());
+        c->device = device;
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreate(&c->ev_a), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
+        *out = c.release();
+        return PMDD_OK;
+    });
+}
+
+void pmdd_ctx_destroy(pmdd_ctx *ctx) {
+    if (ctx == nullptr) {
+        return;
+    }
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream != nullptr) {
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    {
+        // release scratch before the stream
+        pmdd_ctx *c = ctx;
+        c->chunks.release();
+        c->run_off.release();
+        c->nsurv.release();
+        c->nalive.release();
+        c->minkey.release();
+        c->maxkey.release();
+        c->chunk_out.release();
+        c->which.release();
+        c->surv_pts.release();
+        c->alive.release();
+        c->counter.release();
+        c->join_meta.release();
+        c->keep_a.release();
+        c->keep_b.release();
+    }
+    if (ctx->ev_a != nullptr) {
+        (void)hipEventDestroy(ctx->ev_a);
+    }
+    if (ctx->ev_b != nullptr) {
+        (void)hipEventDestroy(ctx->ev_b);
+    }
+    if (ctx->stream != nullptr) {
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int pmdd_upload(pmdd_ctx *ctx, const pmdd_dd_view *view, pmdd_resident **out, char *err,
+                size_t errlen) {
+    return guarded(err, errlen, [&]() -> int {
+        if (ctx == nullptr || out == nullptr) {
+            fail(PMDD_E_INVALID, "pmdd_upload: NULL argument");
+        }
+        *out = nullptr;
+        *out = upload(ctx, view);
+        return PMDD_OK;
+    });
+}
+
+void pmdd_resident_free(pmdd_resident *dd) { delete dd; }
+
+int pmdd_enumerate_resident(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts,
+                            pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen) {
+    return guarded(err, errlen, [&]() -> int {
+        if (ctx == nullptr || dd == nullptr) {
+            fail(PMDD_E_INVALID, "pmdd_enumerate_resident: NULL argument");
+        }
+        return run_enumeration(ctx, dd, opts, out, stats);
+    });
+}
+
+int pmdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *view, const pmdd_opts *opts,
+                   pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen) {
+    return guarded(err, errlen, [&]() -> int {
+        if (ctx == nullptr) {
+            fail(PMDD_E_INVALID, "pmdd_enumerate: NULL context");
+        }
+        const double t0 = now_s();
+        std::unique_ptr<pmdd_resident> r(upload(ctx, view));
+        const double pack_s = now_s() - t0;
+        const int rc = run_enumeration(ctx, r.get(), opts, out, stats);
+        if (stats != nullptr) {
+            stats->wall_pack_transfer_s = pack_s;
+        }
+        return rc;
+    });
+}
+
+static int named_call(pmdd_ctx *ctx, const pmdd_dd_view *view, int kind, int method,
+                      int problem_type, int sd, int64_t max_cand, int64_t max_prod,
+                      pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen) {
+    if (view != nullptr && view->kind != kind) {
+        write_err(err, errlen, kind == PMDD_DD_BDD ? "expected a BDD view" : "expected an MDD view");
+        return PMDD_E_INVALID;
+    }
+    pmdd_opts o{};
+    o.method = method;
+    o.problem_type = problem_type;
+    o.state_dominance = sd;
+    o.max_cand = max_cand;
+    o.max_prod = max_prod;
+    return pmdd_enumerate(ctx, view, &o, out, stats, err, errlen);
+}
+
+int pmdd_topdown_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *bdd, int problem_type,
+                           int state_dominance, int64_t max_cand, pmdd_frontier *out,
+                           pmdd_stats *stats, char *err, size_t errlen) {
+    return named_call(ctx, bdd, PMDD_DD_BDD, 1, problem_type, state_dominance, max_cand,
+                      PMDD_DEFAULT_MAX_PROD, out, stats, err, errlen);
+}
+
+int pmdd_topdown_mdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *mdd, int64_t max_cand,
+                               pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen) {
+    return named_call(ctx, mdd, PMDD_DD_MDD, 1, 3, 0, max_cand, PMDD_DEFAULT_MAX_PROD, out, stats,
+                      err, errlen);
+}
+
+int pmdd_coupled_mdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *mdd, int64_t max_cand,
+                               int64_t max_prod, pmdd_frontier *out, pmdd_stats *stats, char *err,
+                               size_t errlen) {
+    return named_call(ctx, mdd, PMDD_DD_MDD, 3, 3, 0, max_cand, max_prod, out, stats, err, errlen);
+}
+
+int pmdd_coupled_bdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *bdd, int problem_type,
+                               int state_dominance, int64_t max_cand, int64_t max_prod,
+                               pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen) {
+    return named_call(ctx, bdd, PMDD_DD_BDD, 3, problem_type, state_dominance, max_cand, max_prod,
+                      out, stats, err, errlen);
+}
+
+void pmdd_frontier_free(pmdd_frontier *f) {
+    if (f != nullptr) {
+        std::free(f->points);
+        f->points = nullptr;
+        f->count = 0;
+    }
+}
+
+const char *pmdd_engine_version(void) { return "pmdd-engine gfx950 d=1..8"; }
+
+} // extern "C"

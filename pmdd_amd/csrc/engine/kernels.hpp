@@ -1,0 +1,678 @@
+// Device kernels of the frontier engine (gfx950, wave64).
+//
+// Points are int32 vectors of D objectives stored structure-of-arrays in HBM
+// (objective o of point i at pts[o * stride + i]).  Larger is better.
+//
+// The segmented Pareto filter works on "groups" (the candidates of one node
+// during a layer expansion; the products of one coupling batch) cut into
+// chunks of at most CH candidates:
+//
+//   k_filter_chunks   one workgroup per chunk.  Loads its candidates straight
+//                     from the parent frontiers (parent point + arc weight) or
+//                     the cutset frontiers (TD point + BU point) into LDS,
+//                     sorts them by key = sum of objectives, descending, with a
+//                     bitonic network, then runs a sort-filter skyline: a
+//                     candidate survives iff no earlier (higher-key) candidate
+//                     is >= it in every objective.  Ties (equal points) keep
+//                     the first copy.  Survivors go to HBM in key order.
+//   k_cross_chunks    groups of several chunks: every local survivor is
+//                     checked against the local survivors of the group's other
+//                     chunks; it dies when one is >= it everywhere and either
+//                     differs or sits in an earlier chunk (keeps one copy of
+//                     equal points).  Chunks whose largest key is below this
+//                     chunk's smallest key are skipped: a point can only be
+//                     weakly dominated by a point of larger or equal key.
+//   k_compact_chunks  writes the alive survivors of each chunk densely.
+//
+// Why it is exact: q >= p everywhere and q != p implies key(q) > key(p), so
+// after sorting by key descending only earlier points can dominate a point;
+// and if an earlier point dominates p, so does some earlier survivor
+// (transitivity), hence checking survivors only is enough.  Across chunks the
+// check is symmetric, so chunk order does not matter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pmdd_k {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kWaves = kBlock / kWave;
+
+template <int D> struct Pad {
+    static constexpr int DP = D <= 4 ? 4 : 8; // survivor rows padded to 16 or 32 bytes
+};
+
+// Chunk capacity per objective count: LDS = CH*(4D + 8 + 4*DP) bytes <= 80 KB,
+// so two workgroups fit one CU's 160 KB.
+template <int D> struct Cap {
+    static constexpr int CH = D <= 4 ? 2048 : 1024;
+};
+
+struct ChunkDesc {
+    int64_t start;       // first candidate, as an index into the group's virtual list
+    int64_t out_base;    // slot of the chunk's first survivor in the pass buffer
+    int32_t group;       // node of the layer (expansion); unused for coupling
+    int32_t len;         // candidates in the chunk, 1..CH
+    int32_t group_first; // first chunk of the group
+    int32_t group_count; // chunks in the group
+};
+
+// Candidate source of a layer expansion: node v's candidates are the parent
+// frontiers of its arcs, concatenated in arc order, each shifted by its arc
+// weight.  run_off[a] = node-local index of arc a's first candidate.
+struct ExpandSrc {
+    const int64_t *prev_off; // [N_prev + 1] frontier offsets of the parent layer
+    const int32_t *prev_pts; // SoA, stride prev_stride
+    int64_t prev_stride;
+    const int64_t *arc_off; // [N + 1] arcs of each node (layer-local arc ids)
+    const int32_t *arc_src; // [E] parent node of each arc
+    const int32_t *arc_w;   // SoA [D][E]
+    int64_t w_stride;       // = E
+    const int64_t *run_off; // [E]
+};
+
+// Candidate source of a coupling batch: for batch node k the products
+// td[td_begin[k] + a] + bu[bu_begin[k] + b], numbered a * bu_size[k] + b
+// after prod_off[k].
+struct JoinSrc {
+    const int64_t *prod_off; // [K + 1]
+    const int64_t *td_begin; // [K]
+    const int64_t *bu_begin; // [K]
+    const int64_t *bu_size;  // [K]
+    int32_t K;
+    const int32_t *td_pts;
+    int64_t td_stride;
+    const int32_t *bu_pts;
+    int64_t bu_stride;
+};
+
+template <int D>
+__device__ __forceinline__ void load_candidate(const ExpandSrc &s, const ChunkDesc &c, int64_t t,
+                                               int32_t (&p)[D]) {
+    const int64_t q = c.start + t;
+    int64_t lo = s.arc_off[c.group];
+    int64_t hi = s.arc_off[c.group + 1] - 1;
+    // last arc whose run starts at or before q (empty runs share a start)
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (s.run_off[mid] <= q) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    const int64_t e = lo;
+    const int64_t pos = s.prev_off[s.arc_src[e]] + (q - s.run_off[e]);
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        p[o] = s.prev_pts[o * s.prev_stride + pos] + s.arc_w[o * s.w_stride + e];
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void load_candidate(const JoinSrc &s, const ChunkDesc &c, int64_t t,
+                                               int32_t (&p)[D]) {
+    const int64_t q = c.start + t;
+    int lo = 0;
+    int hi = s.K - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s.prod_off[mid] <= q) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    const int64_t local = q - s.prod_off[lo];
+    const int64_t bs = s.bu_size[lo];
+    const int64_t a = local / bs;
+    const int64_t b = local - a * bs;
+    const int64_t ti = s.td_begin[lo] + a;
+    const int64_t bi = s.bu_begin[lo] + b;
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        p[o] = s.td_pts[o * s.td_stride + ti] + s.bu_pts[o * s.bu_stride + bi];
+    }
+}
+
+// Sort word: key (sum of objectives, |key| < 2^35) biased to unsigned in the
+// high 48 bits, (0xFFFF - slot) in the low 16, so that descending order of
+// the word is key descending, then slot ascending.
+__device__ __forceinline__ uint64_t pack_key(int64_t key, int slot) {
+    return (static_cast<uint64_t>(key + (int64_t(1) << 47)) << 16) |
+           static_cast<uint64_t>(0xFFFF - slot);
+}
+__device__ __forceinline__ int unpack_slot(uint64_t w) { return 0xFFFF - static_cast<int>(w & 0xFFFF); }
+__device__ __forceinline__ int64_t unpack_key(uint64_t w) {
+    return static_cast<int64_t>(w >> 16) - (int64_t(1) << 47);
+}
+
+// Block-wide exclusive prefix of a 0/1 flag; returns the rank and writes the
+// block total to *total.  Uses wave ballots and one LDS slot per wave.
+__device__ __forceinline__ int block_rank(bool flag, int *wave_tot, int *total) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const uint64_t mask = __ballot(flag);
+    const int in_wave = __popcll(mask & ((lane == 0) ? 0ULL : (~0ULL >> (kWave - lane))));
+    if (lane == 0) {
+        wave_tot[wave] = __popcll(mask);
+    }
+    __syncthreads();
+    int before = 0;
+    int sum = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const int t = wave_tot[w];
+        before += (w < wave) ? t : 0;
+        sum += t;
+    }
+    *total = sum;
+    return before + in_wave;
+}
+
+// row >= m in every objective; rows are 16-byte aligned, padded to DP ints.
+template <int D, int DP>
+__device__ __forceinline__ bool ge_all(const int32_t *row, const int32_t (&m)[D]) {
+    const int4 a = *reinterpret_cast<const int4 *>(row);
+    bool r = a.x >= m[0];
+    if (D > 1) r &= a.y >= m[D > 1 ? 1 : 0];
+    if (D > 2) r &= a.z >= m[D > 2 ? 2 : 0];
+    if (D > 3) r &= a.w >= m[D > 3 ? 3 : 0];
+    if (D > 4) {
+        const int4 b = *reinterpret_cast<const int4 *>(row + 4);
+        r &= b.x >= m[D > 4 ? 4 : 0];
+        if (D > 5) r &= b.y >= m[D > 5 ? 5 : 0];
+        if (D > 6) r &= b.z >= m[D > 6 ? 6 : 0];
+        if (D > 7) r &= b.w >= m[D > 7 ? 7 : 0];
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// k_filter_chunks: per-chunk load, presort, sort-filter skyline.
+// ---------------------------------------------------------------------------
+template <int D, int CH, class Src>
+__global__ __launch_bounds__(kBlock) void k_filter_chunks(
+    const ChunkDesc *__restrict__ chunks, Src src, int32_t *__restrict__ out_pts,
+    int64_t out_stride, uint8_t *__restrict__ alive, int32_t *__restrict__ chunk_nsurv,
+    int32_t *__restrict__ chunk_nalive, int64_t *__restrict__ chunk_minkey,
+    int64_t *__restrict__ chunk_maxkey, unsigned long long *__restrict__ pair_tests) {
+    constexpr int DP = Pad<D>::DP;
+    __shared__ int32_t cs[D][CH];
+    __shared__ uint64_t sk[CH];
+    __shared__ __align__(16) int32_t S[CH * DP];
+    __shared__ int wave_tot[kWaves];
+
+    const ChunkDesc c = chunks[blockIdx.x];
+    const int n = c.len;
+    const int tid = threadIdx.x;
+
+    int npad = 1;
+    while (npad < n) {
+        npad <<= 1;
+    }
+    for (int t = tid; t < npad; t += kBlock) {
+        if (t < n) {
+            int32_t p[D];
+            load_candidate<D>(src, c, t, p);
+            int64_t key = 0;
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                cs[o][t] = p[o];
+                key += p[o];
+            }
+            sk[t] = pack_key(key, t);
+        } else {
+            sk[t] = 0; // below every real word
+        }
+    }
+    __syncthreads();
+
+    // bitonic sort, descending
+    for (int k = 2; k <= npad; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < (npad >> 1); t += kBlock) {
+                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                const int l = i + j;
+                const uint64_t a = sk[i];
+                const uint64_t b = sk[l];
+                const bool desc = (i & k) == 0;
+                if (desc ? (a < b) : (a > b)) {
+                    sk[i] = b;
+                    sk[l] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // sort-filter skyline, 256 candidates per step
+    int nS = 0;
+    unsigned long long tests = 0;
+    for (int base = 0; base < n; base += kBlock) {
+        const int i = base + tid;
+        const bool valid = i < n;
+        const int tcount = min(kBlock, n - base);
+        int32_t m[D];
+        if (valid) {
+            const int slot = unpack_slot(sk[i]);
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                m[o] = cs[o][slot];
+                S[(nS + tid) * DP + o] = m[o];
+            }
+        } else {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                m[o] = 0;
+            }
+        }
+        __syncthreads();
+        bool dead = !valid;
+        // against survivors of earlier steps
+        for (int s = 0; s < nS && !dead; ++s) {
+            dead = ge_all<D, DP>(&S[s * DP], m);
+            if (pair_tests != nullptr) {
+                ++tests;
+            }
+        }
+        // against earlier candidates of this step
+        for (int j = 0; j < tid && j < tcount && !dead; ++j) {
+            dead = ge_all<D, DP>(&S[(nS + j) * DP], m);
+            if (pair_tests != nullptr) {
+                ++tests;
+            }
+        }
+        __syncthreads();
+        int total = 0;
+        const int rank = block_rank(!dead, wave_tot, &total);
+        if (!dead) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                S[(nS + rank) * DP + o] = m[o];
+            }
+        }
+        nS += total;
+        __syncthreads();
+    }
+
+    for (int s = tid; s < nS; s += kBlock) {
+        const int64_t pos = c.out_base + s;
+#pragma unroll
+        for (int o = 0; o < D; ++o) {
+            out_pts[o * out_stride + pos] = S[s * DP + o];
+        }
+        alive[pos] = 1;
+    }
+    if (tid == 0) {
+        chunk_nsurv[blockIdx.x] = nS;
+        chunk_nalive[blockIdx.x] = nS;
+        int64_t kmax = 0, kmin = 0;
+#pragma unroll
+        for (int o = 0; o < D; ++o) {
+            kmax += S[o];
+            kmin += S[(nS - 1) * DP + o];
+        }
+        chunk_maxkey[blockIdx.x] = kmax;
+        chunk_minkey[blockIdx.x] = kmin;
+    }
+    if (pair_tests != nullptr) {
+        atomicAdd(pair_tests, tests);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_cross_chunks: symmetric check of a chunk's survivors against the other
+// chunks of its group.  One workgroup per chunk listed in `which`.
+// ---------------------------------------------------------------------------
+template <int D, int CH>
+__global__ __launch_bounds__(kBlock) void k_cross_chunks(
+    const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
+    const int32_t *__restrict__ pts, int64_t stride, uint8_t *__restrict__ alive,
+    const int32_t *__restrict__ chunk_nsurv, int32_t *__restrict__ chunk_nalive,
+    const int64_t *__restrict__ chunk_minkey, const int64_t *__restrict__ chunk_maxkey,
+    unsigned long long *__restrict__ pair_tests) {
+    constexpr int DP = Pad<D>::DP;
+    constexpr int R = CH / kBlock; // survivors held per thread
+    __shared__ __align__(16) int32_t T[kBlock * DP];
+    __shared__ int live_count;
+
+    const int me = which[blockIdx.x];
+    const ChunkDesc c = chunks[me];
+    const int nS = chunk_nsurv[me];
+    const int tid = threadIdx.x;
+    const int64_t my_min = chunk_minkey[me];
+
+    int32_t m[R][D];
+    bool dead[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int s = r * kBlock + tid;
+        dead[r] = s >= nS;
+        if (!dead[r]) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                m[r][o] = pts[o * stride + c.out_base + s];
+            }
+        } else {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                m[r][o] = 0;
+            }
+        }
+    }
+    unsigned long long tests = 0;
+
+    for (int g = 0; g < c.group_count; ++g) {
+        const int other = c.group_first + g;
+        if (other == me) {
+            continue;
+        }
+        if (chunk_maxkey[other] < my_min) {
+            continue; // every point there has a smaller key than every point here
+        }
+        const ChunkDesc oc = chunks[other];
+        const int on = chunk_nsurv[other];
+        const bool earlier = other < me;
+        for (int tb = 0; tb < on; tb += kBlock) {
+            const int tn = min(kBlock, on - tb);
+            if (tid < tn) {
+#pragma unroll
+                for (int o = 0; o < D; ++o) {
+                    T[tid * DP + o] = pts[o * stride + oc.out_base + tb + tid];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                for (int j = 0; j < tn && !dead[r]; ++j) {
+                    const int32_t *q = &T[j * DP];
+                    bool ge = true;
+                    bool eq = true;
+#pragma unroll
+                    for (int o = 0; o < D; ++o) {
+                        ge &= q[o] >= m[r][o];
+                        eq &= q[o] == m[r][o];
+                    }
+                    dead[r] = ge && (!eq || earlier);
+                    if (pair_tests != nullptr) {
+                        ++tests;
+                    }
+                }
+            }
+            // stop early once every survivor of this chunk is dead
+            bool any_live = false;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                any_live |= !dead[r];
+            }
+            const int live = __syncthreads_or(any_live);
+            if (!live) {
+                break;
+            }
+        }
+        bool any_live = false;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            any_live |= !dead[r];
+        }
+        if (!__syncthreads_or(any_live)) {
+            break;
+        }
+    }
+
+    int kept = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int s = r * kBlock + tid;
+        if (s < nS) {
+            if (dead[r]) {
+                alive[c.out_base + s] = 0;
+            } else {
+                ++kept;
+            }
+        }
+    }
+    if (tid == 0) {
+        live_count = 0;
+    }
+    __syncthreads();
+    atomicAdd(&live_count, kept);
+    __syncthreads();
+    if (tid == 0) {
+        chunk_nalive[me] = live_count;
+    }
+    if (pair_tests != nullptr) {
+        atomicAdd(pair_tests, tests);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_compact_chunks: alive survivors of each chunk to dense output at
+// chunk_out[c] (order kept).
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_compact_chunks(
+    const ChunkDesc *__restrict__ chunks, const int32_t *__restrict__ chunk_nsurv,
+    const int64_t *__restrict__ chunk_out, const int32_t *__restrict__ src, int64_t src_stride,
+    const uint8_t *__restrict__ alive, int32_t *__restrict__ dst, int64_t dst_stride) {
+    __shared__ int wave_tot[kWaves];
+    const ChunkDesc c = chunks[blockIdx.x];
+    const int nS = chunk_nsurv[blockIdx.x];
+    int64_t out = chunk_out[blockIdx.x];
+    for (int base = 0; base < nS; base += kBlock) {
+        const int s = base + threadIdx.x;
+        const bool a = s < nS && alive[c.out_base + s] != 0;
+        int total = 0;
+        const int rank = block_rank(a, wave_tot, &total);
+        if (a) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                dst[o * dst_stride + out + rank] = src[o * src_stride + c.out_base + s];
+            }
+        }
+        out += total;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Knapsack state dominance (method 1): a label of a target node is dropped
+// when some label of one of its (up to two) comparison nodes is >= it in every
+// objective.  Reads the unfiltered layer; writes keep flags per point.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_state_dominance(
+    const int32_t *__restrict__ targets, // [n_targets][3]: node, cmpA, cmpB (-1 = none)
+    const int64_t *__restrict__ off, const int32_t *__restrict__ pts, int64_t stride,
+    uint8_t *__restrict__ keep) {
+    constexpr int DP = Pad<D>::DP;
+    __shared__ __align__(16) int32_t T[kBlock * DP];
+    const int node = targets[3 * blockIdx.x];
+    const int64_t b0 = off[node];
+    const int64_t n = off[node + 1] - b0;
+    for (int64_t base = 0; base < n; base += kBlock) {
+        const int64_t i = base + threadIdx.x;
+        const bool valid = i < n;
+        int32_t m[D];
+#pragma unroll
+        for (int o = 0; o < D; ++o) {
+            m[o] = valid ? pts[o * stride + b0 + i] : 0;
+        }
+        bool dead = !valid;
+        for (int cidx = 1; cidx <= 2; ++cidx) {
+            const int cmp = targets[3 * blockIdx.x + cidx];
+            if (cmp < 0) {
+                continue;
+            }
+            const int64_t c0 = off[cmp];
+            const int64_t cn = off[cmp + 1] - c0;
+            for (int64_t tb = 0; tb < cn; tb += kBlock) {
+                const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), cn - tb));
+                __syncthreads();
+                if (static_cast<int>(threadIdx.x) < tn) {
+#pragma unroll
+                    for (int o = 0; o < D; ++o) {
+                        T[threadIdx.x * DP + o] = pts[o * stride + c0 + tb + threadIdx.x];
+                    }
+                }
+                __syncthreads();
+                for (int j = 0; j < tn && !dead; ++j) {
+                    dead = ge_all<D, DP>(&T[j * DP], m);
+                }
+            }
+        }
+        if (valid && dead) {
+            keep[b0 + i] = 0;
+        }
+    }
+}
+
+// Per-node copy of kept points into a dense frontier with new offsets.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_compact_nodes(
+    const int64_t *__restrict__ off_in, const int32_t *__restrict__ src, int64_t src_stride,
+    const uint8_t *__restrict__ keep, const int64_t *__restrict__ off_out,
+    int32_t *__restrict__ dst, int64_t dst_stride) {
+    __shared__ int wave_tot[kWaves];
+    const int node = blockIdx.x;
+    const int64_t b0 = off_in[node];
+    const int64_t n = off_in[node + 1] - b0;
+    int64_t out = off_out[node];
+    for (int64_t base = 0; base < n; base += kBlock) {
+        const int64_t i = base + threadIdx.x;
+        const bool a = i < n && keep[b0 + i] != 0;
+        int total = 0;
+        const int rank = block_rank(a, wave_tot, &total);
+        if (a) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                dst[o * dst_stride + out + rank] = src[o * src_stride + b0 + i];
+            }
+        }
+        out += total;
+        __syncthreads();
+    }
+}
+
+// Per-node count of kept points.
+__global__ __launch_bounds__(kBlock) void k_count_kept(const int64_t *__restrict__ off,
+                                                       const uint8_t *__restrict__ keep,
+                                                       int64_t *__restrict__ counts,
+                                                       int64_t n_nodes) {
+    __shared__ int wave_tot[kWaves];
+    const int node = blockIdx.x;
+    const int64_t b0 = off[node];
+    const int64_t n = off[node + 1] - b0;
+    int64_t kept = 0;
+    for (int64_t base = 0; base < n; base += kBlock) {
+        const int64_t i = base + threadIdx.x;
+        const bool a = i < n && keep[b0 + i] != 0;
+        int total = 0;
+        block_rank(a, wave_tot, &total);
+        kept += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        counts[node] = kept;
+    }
+}
+
+// Fill a byte array (used instead of memset so the engine never records a
+// memset node when captured into a graph).
+__global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i] = v;
+    }
+}
+
+// Per-chunk count of alive survivors.
+__global__ __launch_bounds__(kBlock) void k_count_chunks(const ChunkDesc *__restrict__ chunks,
+                                                         const int32_t *__restrict__ chunk_nsurv,
+                                                         const uint8_t *__restrict__ alive,
+                                                         int32_t *__restrict__ chunk_nalive) {
+    __shared__ int wave_tot[kWaves];
+    const ChunkDesc c = chunks[blockIdx.x];
+    const int nS = chunk_nsurv[blockIdx.x];
+    int kept = 0;
+    for (int base = 0; base < nS; base += kBlock) {
+        const int s = base + threadIdx.x;
+        const bool a = s < nS && alive[c.out_base + s] != 0;
+        int total = 0;
+        block_rank(a, wave_tot, &total);
+        kept += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        chunk_nalive[blockIdx.x] = kept;
+    }
+}
+
+// Coupling merge step.  Blocks [0, nblk_b): each new batch point b dies when
+// some running-frontier point r is >= b everywhere (equal points die: r holds
+// them already).  Blocks [nblk_b, ...): each running-frontier point r dies
+// when some batch point b is >= r everywhere and differs from it.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_vs_frontier(
+    const int32_t *__restrict__ B, int64_t bstride, int64_t nb, const int32_t *__restrict__ R,
+    int64_t rstride, int64_t nr, uint8_t *__restrict__ keep_b, uint8_t *__restrict__ keep_r,
+    int64_t nblk_b, unsigned long long *__restrict__ pair_tests) {
+    constexpr int DP = Pad<D>::DP;
+    __shared__ __align__(16) int32_t T[kBlock * DP];
+    const bool b_role = static_cast<int64_t>(blockIdx.x) < nblk_b;
+    const int64_t blk = b_role ? blockIdx.x : blockIdx.x - nblk_b;
+    const int32_t *mine = b_role ? B : R;
+    const int64_t mstride = b_role ? bstride : rstride;
+    const int64_t nm = b_role ? nb : nr;
+    const int32_t *other = b_role ? R : B;
+    const int64_t ostride = b_role ? rstride : bstride;
+    const int64_t no = b_role ? nr : nb;
+    const int64_t i = blk * kBlock + threadIdx.x;
+    const bool valid = i < nm;
+    int32_t m[D];
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        m[o] = valid ? mine[o * mstride + i] : 0;
+    }
+    bool dead = !valid;
+    unsigned long long tests = 0;
+    for (int64_t tb = 0; tb < no; tb += kBlock) {
+        const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), no - tb));
+        if (static_cast<int>(threadIdx.x) < tn) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                T[threadIdx.x * DP + o] = other[o * ostride + tb + threadIdx.x];
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < tn && !dead; ++j) {
+            const int32_t *q = &T[j * DP];
+            bool ge = true;
+            bool eq = true;
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                ge &= q[o] >= m[o];
+                eq &= q[o] == m[o];
+            }
+            dead = ge && (b_role || !eq);
+            if (pair_tests != nullptr) {
+                ++tests;
+            }
+        }
+        if (!__syncthreads_or(!dead)) {
+            break;
+        }
+    }
+    if (valid) {
+        (b_role ? keep_b : keep_r)[i] = dead ? 0 : 1;
+    }
+    if (pair_tests != nullptr) {
+        atomicAdd(pair_tests, tests);
+    }
+}
+
+} // namespace pmdd_k

@@ -1,0 +1,546 @@
+// Decision-diagram constructors and flattening.
+#include "dd.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+#include "word_map.hpp"
+
+namespace pmdd {
+
+pmdd_dd_view FlatDD::view() const {
+    pmdd_dd_view v;
+    std::memset(&v, 0, sizeof(v));
+    v.d = d;
+    v.num_layers = num_layers;
+    v.kind = kind;
+    v.root = root;
+    v.terminal = terminal;
+    v.layer_off = layer_off.data();
+    v.in_off = in_off.data();
+    v.arc_src = arc_src.data();
+    v.arc_val = arc_val.data();
+    v.arc_w = arc_w.data();
+    v.min_weight = min_weight.empty() ? nullptr : min_weight.data();
+    return v;
+}
+
+int64_t BinaryDD::width() const {
+    int64_t w = 0;
+    for (int l = 0; l < num_layers; ++l) {
+        w = std::max(w, layer_size(l));
+    }
+    return w;
+}
+
+int64_t BinaryDD::num_nodes() const {
+    int64_t n = 0;
+    for (int l = 0; l < num_layers; ++l) {
+        n += layer_size(l);
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// Knapsack
+// ---------------------------------------------------------------------------
+
+BinaryDD build_knapsack_exact(const KnapsackInstance &inst, int d) {
+    const int n = inst.n_vars;
+    const int nc = inst.n_cons;
+    BinaryDD bdd;
+    bdd.d = d;
+    bdd.num_layers = n + 1;
+    bdd.child.resize(n);
+    bdd.one_w.assign(n, std::vector<int32_t>(d, 0));
+    for (int l = 0; l < n; ++l) {
+        for (int o = 0; o < d; ++o) {
+            bdd.one_w[l][o] = inst.obj_coeffs[l][o];
+        }
+    }
+
+    // States of the current layer, one word per constraint (int32 stored in a
+    // uint64 so that the map can hash it).
+    std::vector<uint64_t> cur(nc, 0); // root: nothing used
+    WordKeyMap next_map(nc);
+    std::vector<uint64_t> key(nc);
+    std::vector<int64_t> used(nc);
+
+    for (int l = 0; l < n; ++l) {
+        const int64_t width = static_cast<int64_t>(cur.size()) / nc;
+        bdd.child[l].assign(width, {-1, -1});
+        next_map.clear();
+        for (int64_t i = 0; i < width; ++i) {
+            const uint64_t *s = cur.data() + i * nc;
+            // one-arc feasibility: every constraint stays within its rhs
+            bool feasible = true;
+            for (int c = 0; c < nc; ++c) {
+                used[c] = static_cast<int64_t>(static_cast<int32_t>(s[c])) + inst.coeffs[c][l];
+                feasible = feasible && used[c] <= inst.rhs[c];
+            }
+            if (l < n - 1) {
+                bool ins = false;
+                bdd.child[l][i][0] = static_cast<int32_t>(next_map.insert(s, &ins));
+                if (feasible) {
+                    for (int c = 0; c < nc; ++c) {
+                        key[c] = static_cast<uint64_t>(static_cast<uint32_t>(used[c]));
+                    }
+                    bdd.child[l][i][1] = static_cast<int32_t>(next_map.insert(key.data(), &ins));
+                }
+            } else {
+                bdd.child[l][i][0] = 0; // terminal
+                if (feasible) {
+                    bdd.child[l][i][1] = 0;
+                }
+            }
+        }
+        if (l < n - 1) {
+            const int64_t nn = next_map.size();
+            cur.assign(next_map.key(0), next_map.key(0) + nn * nc);
+        }
+    }
+    return bdd;
+}
+
+void reduce_bdd(BinaryDD *bdd) {
+    const int L = bdd->num_layers;
+    // map from old index to new index for the layer below the one processed
+    std::vector<int32_t> below_map(1, 0); // terminal layer keeps its single node
+    WordKeyMap pairs(1);
+    for (int l = L - 2; l >= 0; --l) {
+        std::vector<std::array<int32_t, 2>> &layer = bdd->child[l];
+        std::vector<int32_t> this_map(layer.size(), -1);
+        std::vector<std::array<int32_t, 2>> kept;
+        kept.reserve(layer.size());
+        pairs.clear();
+        for (size_t i = 0; i < layer.size(); ++i) {
+            int32_t c0 = layer[i][0] >= 0 ? below_map[layer[i][0]] : -1;
+            int32_t c1 = layer[i][1] >= 0 ? below_map[layer[i][1]] : -1;
+            if (c0 < 0 && c1 < 0) {
+                continue; // no way down: removed, its parents lose the arc
+            }
+            const uint64_t k = (static_cast<uint64_t>(static_cast<uint32_t>(c0)) << 32) |
+                               static_cast<uint32_t>(c1);
+            bool ins = false;
+            const int64_t id = pairs.insert(&k, &ins);
+            if (ins) {
+                kept.push_back({c0, c1});
+            }
+            this_map[i] = static_cast<int32_t>(id);
+        }
+        layer.swap(kept);
+        below_map.swap(this_map);
+    }
+}
+
+std::vector<std::vector<int32_t>> knapsack_min_weights(const BinaryDD &bdd,
+                                                        const KnapsackInstance &inst) {
+    const int L = bdd.num_layers;
+    std::vector<std::vector<int32_t>> mw(L);
+    mw[0].assign(bdd.layer_size(0), 0);
+    for (int l = 1; l < L; ++l) {
+        mw[l].assign(bdd.layer_size(l), INT_MAX);
+        const int32_t a = inst.coeffs[0][l - 1];
+        for (size_t i = 0; i < bdd.child[l - 1].size(); ++i) {
+            const int32_t p = mw[l - 1][i];
+            const int32_t c0 = bdd.child[l - 1][i][0];
+            const int32_t c1 = bdd.child[l - 1][i][1];
+            if (c0 >= 0) {
+                mw[l][c0] = std::min(mw[l][c0], p);
+            }
+            if (c1 >= 0) {
+                mw[l][c1] = std::min(mw[l][c1], p + a);
+            }
+        }
+    }
+    return mw;
+}
+
+// ---------------------------------------------------------------------------
+// Set packing as an independent-set BDD
+// ---------------------------------------------------------------------------
+
+BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
+                                std::vector<int32_t> *var_layer) {
+    const int n = inst.n_vars;
+    const int W = (n + 63) / 64;
+    // Conflict graph: variables sharing a constraint are adjacent.  compl[v]
+    // keeps the vertices still eligible after v is chosen (not v, not N(v)).
+    std::vector<uint64_t> compl_mask(static_cast<size_t>(n) * W, 0);
+    for (int v = 0; v < n; ++v) {
+        for (int w = 0; w < n; ++w) {
+            compl_mask[static_cast<size_t>(v) * W + w / 64] |= (1ULL << (w % 64));
+        }
+    }
+    auto clear_bit = [&](int v, int w) {
+        compl_mask[static_cast<size_t>(v) * W + w / 64] &= ~(1ULL << (w % 64));
+    };
+    for (int v = 0; v < n; ++v) {
+        clear_bit(v, v);
+    }
+    for (const std::vector<int32_t> &cons : inst.vars_cons) {
+        for (size_t i = 0; i < cons.size(); ++i) {
+            for (size_t j = i + 1; j < cons.size(); ++j) {
+                if (cons[i] != cons[j]) {
+                    clear_bit(cons[i], cons[j]);
+                    clear_bit(cons[j], cons[i]);
+                }
+            }
+        }
+    }
+
+    BinaryDD bdd;
+    bdd.d = d;
+    bdd.num_layers = n + 1;
+    bdd.child.resize(n);
+    bdd.one_w.assign(n, std::vector<int32_t>(d, 0));
+    if (var_layer != nullptr) {
+        var_layer->assign(n, -1);
+    }
+
+    std::vector<uint64_t> cur(W, 0);
+    for (int v = 0; v < n; ++v) {
+        cur[v / 64] |= (1ULL << (v % 64));
+    }
+    std::vector<int32_t> active(n);
+    for (int v = 0; v < n; ++v) {
+        active[v] = v;
+    }
+    std::vector<int64_t> counter(n, 0);
+    WordKeyMap next_map(W);
+    std::vector<uint64_t> s0(W), s1(W);
+
+    for (int l = 1; l <= n; ++l) {
+        const int64_t width = static_cast<int64_t>(cur.size()) / W;
+        // min-in-state: the active vertex contained in the fewest states,
+        // first in active-list order on ties
+        for (int32_t v : active) {
+            counter[v] = 0;
+        }
+        for (int64_t i = 0; i < width; ++i) {
+            const uint64_t *s = cur.data() + i * W;
+            for (int w = 0; w < W; ++w) {
+                uint64_t bits = s[w];
+                while (bits != 0) {
+                    const int b = __builtin_ctzll(bits);
+                    counter[w * 64 + b]++;
+                    bits &= bits - 1;
+                }
+            }
+        }
+        size_t sel = 0;
+        for (size_t i = 1; i < active.size(); ++i) {
+            if (counter[active[i]] < counter[active[sel]]) {
+                sel = i;
+            }
+        }
+        const int vertex = active[sel];
+        active[sel] = active.back();
+        active.pop_back();
+        if (var_layer != nullptr) {
+            (*var_layer)[l - 1] = vertex;
+        }
+        for (int o = 0; o < d; ++o) {
+            bdd.one_w[l - 1][o] = inst.objs[o][vertex];
+        }
+
+        next_map.clear();
+        bdd.child[l - 1].assign(width, {-1, -1});
+        const int vw = vertex / 64;
+        const uint64_t vb = 1ULL << (vertex % 64);
+        const uint64_t *cm = compl_mask.data() + static_cast<size_t>(vertex) * W;
+        for (int64_t i = 0; i < width; ++i) {
+            const uint64_t *s = cur.data() + i * W;
+            const bool was_set = (s[vw] & vb) != 0;
+            for (int w = 0; w < W; ++w) {
+                s0[w] = s[w];
+            }
+            s0[vw] &= ~vb;
+            bool ins = false;
+            bdd.child[l - 1][i][0] = static_cast<int32_t>(next_map.insert(s0.data(), &ins));
+            if (was_set) {
+                for (int w = 0; w < W; ++w) {
+                    s1[w] = s0[w] & cm[w];
+                }
+                bdd.child[l - 1][i][1] = static_cast<int32_t>(next_map.insert(s1.data(), &ins));
+            }
+        }
+        const int64_t nn = next_map.size();
+        cur.assign(next_map.key(0), next_map.key(0) + nn * W);
+    }
+    return bdd;
+}
+
+// ---------------------------------------------------------------------------
+// Flattening
+// ---------------------------------------------------------------------------
+
+void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *min_weight,
+                 FlatDD *out) {
+    const int L = bdd.num_layers;
+    const int d = bdd.d;
+    out->d = d;
+    out->kind = PMDD_DD_BDD;
+    out->num_layers = L;
+    out->root = 0;
+    out->terminal = 0;
+    out->layer_off.assign(L + 1, 0);
+    for (int l = 0; l < L; ++l) {
+        out->layer_off[l + 1] = out->layer_off[l] + bdd.layer_size(l);
+    }
+    const int64_t total = out->layer_off[L];
+    out->in_off.assign(total + 1, 0);
+    out->arc_src.clear();
+    out->arc_val.clear();
+    out->arc_w.clear();
+
+    // root: no in-arcs
+    int64_t arc = 0;
+    out->in_off[1] = 0;
+    for (int l = 1; l < L; ++l) {
+        const int64_t n_here = bdd.layer_size(l);
+        const std::vector<std::array<int32_t, 2>> &parents = bdd.child[l - 1];
+        // in-degree per head and arc type
+        std::vector<int64_t> cnt1(n_here, 0), cnt0(n_here, 0);
+        for (size_t p = 0; p < parents.size(); ++p) {
+            if (parents[p][1] >= 0) {
+                cnt1[parents[p][1]]++;
+            }
+            if (parents[p][0] >= 0) {
+                cnt0[parents[p][0]]++;
+            }
+        }
+        const int64_t base_node = out->layer_off[l];
+        std::vector<int64_t> pos1(n_here), pos0(n_here);
+        int64_t cursor = arc;
+        for (int64_t v = 0; v < n_here; ++v) {
+            out->in_off[base_node + v] = cursor;
+            pos1[v] = cursor;
+            pos0[v] = cursor + cnt1[v];
+            cursor += cnt1[v] + cnt0[v];
+        }
+        out->in_off[base_node + n_here] = cursor;
+        const int64_t layer_arcs = cursor - arc;
+        out->arc_src.resize(cursor);
+        out->arc_val.resize(cursor);
+        out->arc_w.resize(cursor * d);
+        // parents in ascending index order within each (head, type) group
+        for (size_t p = 0; p < parents.size(); ++p) {
+            for (int t = 1; t >= 0; --t) {
+                const int32_t h = parents[p][t];
+                if (h < 0) {
+                    continue;
+                }
+                const int64_t a = (t == 1) ? pos1[h]++ : pos0[h]++;
+                out->arc_src[a] = static_cast<int32_t>(p);
+                out->arc_val[a] = t;
+                for (int o = 0; o < d; ++o) {
+                    out->arc_w[a * d + o] = (t == 1) ? bdd.one_w[l - 1][o] : 0;
+                }
+            }
+        }
+        arc += layer_arcs;
+    }
+    if (min_weight != nullptr) {
+        out->min_weight.assign(total, 0);
+        for (int l = 0; l < L; ++l) {
+            std::copy((*min_weight)[l].begin(), (*min_weight)[l].end(),
+                      out->min_weight.begin() + out->layer_off[l]);
+        }
+    } else {
+        out->min_weight.clear();
+    }
+}
+
+static void fill_arc_structure(FlatDD *out) {
+    int64_t max_in = 0;
+    for (int l = 0; l < out->num_layers; ++l) {
+        const int64_t a0 = out->in_off[out->layer_off[l]];
+        const int64_t a1 = out->in_off[out->layer_off[l + 1]];
+        max_in = std::max(max_in, a1 - a0);
+    }
+    out->structure.max_num_in_arcs = max_in;
+    out->structure.total_num_in_arcs = out->num_arcs();
+    out->structure.num_layers = out->num_layers;
+    out->structure.d = out->d;
+}
+
+// ---------------------------------------------------------------------------
+// TSP
+// ---------------------------------------------------------------------------
+
+void build_tsp_exact(const TspInstance &inst, int d, FlatDD *out) {
+    const int n = inst.n_cities;
+    const int L = n + 2;
+    out->d = d;
+    out->kind = PMDD_DD_MDD;
+    out->num_layers = L;
+    out->root = 0;
+    out->terminal = 0;
+
+    struct ArcRec {
+        int32_t tail, head, val;
+    };
+    // layer sizes and arcs (arcs of index l go from layer l to layer l+1)
+    std::vector<int64_t> sizes(L, 0);
+    std::vector<std::vector<ArcRec>> arcs(L - 1);
+    std::vector<std::vector<int32_t>> weights(L - 1);
+
+    sizes[0] = 1;
+    sizes[1] = 1;
+    arcs[0].push_back({0, 0, 0});
+    weights[0].assign(d, 0);
+
+    // states of layer 1: visited {0}, last city 0
+    std::vector<uint64_t> cur = {1ULL, 0ULL}; // (mask, last)
+    WordKeyMap next_map(2);
+    uint64_t key[2];
+    for (int l = 1; l < n; ++l) {
+        next_map.clear();
+        const int64_t width = static_cast<int64_t>(cur.size()) / 2;
+        for (int64_t i = 0; i < width; ++i) {
+            const uint64_t mask = cur[2 * i];
+            const int last = static_cast<int>(cur[2 * i + 1]);
+            for (int v = 0; v < n; ++v) {
+                if (mask & (1ULL << v)) {
+                    continue;
+                }
+                key[0] = mask | (1ULL << v);
+                key[1] = static_cast<uint64_t>(v);
+                bool ins = false;
+                const int64_t h = next_map.insert(key, &ins);
+                arcs[l].push_back({static_cast<int32_t>(i), static_cast<int32_t>(h), v});
+                for (int o = 0; o < d; ++o) {
+                    weights[l].push_back(-inst.at(o, last, v));
+                }
+            }
+        }
+        sizes[l + 1] = next_map.size();
+        cur.assign(next_map.key(0), next_map.key(0) + next_map.size() * 2);
+    }
+    // closing arcs into the terminal
+    sizes[n + 1] = 1;
+    {
+        const int64_t width = static_cast<int64_t>(cur.size()) / 2;
+        for (int64_t i = 0; i < width; ++i) {
+            const int last = static_cast<int>(cur[2 * i + 1]);
+            arcs[n].push_back({static_cast<int32_t>(i), 0, 0});
+            for (int o = 0; o < d; ++o) {
+                weights[n].push_back(-inst.at(o, last, 0));
+            }
+        }
+    }
+
+    out->layer_off.assign(L + 1, 0);
+    for (int l = 0; l < L; ++l) {
+        out->layer_off[l + 1] = out->layer_off[l] + sizes[l];
+    }
+    const int64_t total = out->layer_off[L];
+    out->in_off.assign(total + 1, 0);
+    int64_t num_arcs = 0;
+    for (int l = 0; l < L - 1; ++l) {
+        num_arcs += static_cast<int64_t>(arcs[l].size());
+    }
+    out->arc_src.assign(num_arcs, 0);
+    out->arc_val.assign(num_arcs, 0);
+    out->arc_w.assign(num_arcs * d, 0);
+    int64_t cursor = 0;
+    for (int l = 1; l < L; ++l) {
+        const std::vector<ArcRec> &la = arcs[l - 1];
+        const int64_t base_node = out->layer_off[l];
+        std::vector<int64_t> cnt(sizes[l], 0);
+        for (const ArcRec &a : la) {
+            cnt[a.head]++;
+        }
+        std::vector<int64_t> pos(sizes[l]);
+        for (int64_t v = 0; v < sizes[l]; ++v) {
+            out->in_off[base_node + v] = cursor;
+            pos[v] = cursor;
+            cursor += cnt[v];
+        }
+        // stable: in-arcs of a head keep insertion order (in_arcs_list order)
+        for (size_t k = 0; k < la.size(); ++k) {
+            const int64_t a = pos[la[k].head]++;
+            out->arc_src[a] = la[k].tail;
+            out->arc_val[a] = la[k].val;
+            for (int o = 0; o < d; ++o) {
+                out->arc_w[a * d + o] = weights[l - 1][k * d + o];
+            }
+        }
+    }
+    out->in_off[total] = cursor;
+    out->min_weight.clear();
+}
+
+// ---------------------------------------------------------------------------
+// main()'s construction sequence
+// ---------------------------------------------------------------------------
+
+static bool check_d(int file_d, int expect_d, std::string *error) {
+    if (file_d < 1) {
+        *error = "Error - instance declares no objectives";
+        return false;
+    }
+    if (expect_d > 0 && file_d != expect_d) {
+        *error = "Error - instance has " + std::to_string(file_d) +
+                 " objectives but this run expects " + std::to_string(expect_d);
+        return false;
+    }
+    return true;
+}
+
+bool build_from_file(const std::string &path, int problem_type, int expect_d, FlatDD *out,
+                     std::string *error) {
+    if (problem_type == 1) {
+        KnapsackInstance inst;
+        if (!read_knapsack(path, &inst, error) || !check_d(inst.n_objs, expect_d, error)) {
+            return false;
+        }
+        const int d = inst.n_objs;
+        BinaryDD bdd = build_knapsack_exact(inst, d);
+        const int64_t original_width = bdd.width();
+        const int64_t original_nodes = bdd.num_nodes();
+        reduce_bdd(&bdd);
+        std::vector<std::vector<int32_t>> mw = knapsack_min_weights(bdd, inst);
+        flatten_bdd(bdd, &mw, out);
+        out->structure.original_width = original_width;
+        out->structure.original_num_nodes = original_nodes;
+        out->structure.reduced_width = bdd.width();
+        out->structure.reduced_num_nodes = bdd.num_nodes();
+        fill_arc_structure(out);
+        return true;
+    }
+    if (problem_type == 2) {
+        SetPackingInstance inst;
+        if (!read_setpacking(path, &inst, error) || !check_d(inst.n_objs, expect_d, error)) {
+            return false;
+        }
+        const int d = inst.n_objs;
+        BinaryDD bdd = build_setpacking_exact(inst, d, nullptr);
+        flatten_bdd(bdd, nullptr, out);
+        out->structure.original_width = bdd.width();
+        out->structure.original_num_nodes = bdd.num_nodes();
+        out->structure.reduced_width = bdd.width();
+        out->structure.reduced_num_nodes = bdd.num_nodes();
+        fill_arc_structure(out);
+        return true;
+    }
+    if (problem_type == 3) {
+        TspInstance inst;
+        if (!read_tsp(path, &inst, error) || !check_d(inst.n_objs, expect_d, error)) {
+            return false;
+        }
+        build_tsp_exact(inst, inst.n_objs, out);
+        out->structure.original_width = -1;
+        out->structure.original_num_nodes = -1;
+        out->structure.reduced_width = -1;
+        out->structure.reduced_num_nodes = -1;
+        fill_arc_structure(out);
+        return true;
+    }
+    *error = "Error - problem type not recognized";
+    return false;
+}
+
+} // namespace pmdd

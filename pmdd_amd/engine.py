@@ -1,0 +1,175 @@
+"""Python mirror of the engine's C ABI (libpmdd_engine.so).
+
+``Engine.enumerate`` plays the role of the reference's
+MultiobjEnum::pareto_frontier_{topdown,dynamic_layer_cutset}_cuda
+(src/pmdd/enum/multiobj_enum.cpp:72-124): it takes a decision diagram, a
+method (1 top-down, 3 dynamic layer cutset), the problem type and state
+dominance flag, plus the --max-cand / --max-prod budgets, and returns the
+frontier.  Errors raise PmddError carrying the engine's code and message
+(the reference returns NULL with a reason string instead).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from .abi import PmddError
+
+
+@dataclass
+class EnumerationResult:
+    points: np.ndarray  # (count, d) int32, lexicographically ascending
+    stats: dict = field(default_factory=dict)
+
+
+def sort_lexicographic_ascending(points: np.ndarray) -> np.ndarray:
+    """The order main() writes (ParetoFrontier::sort_lexicographic_ascending,
+    src/pmdd/enum/pareto_frontier.hpp:418-433)."""
+    if len(points) == 0:
+        return points
+    return points[np.lexsort(points.T[::-1])]
+
+
+def _stats_dict(st: abi.Stats, nl: int, std_c, std_s) -> dict:
+    out = {name: getattr(st, name) for name, _ in abi.Stats._fields_
+           if name not in ("std_candidates_per_layer", "std_frontier_survivors_per_layer",
+                           "std_capacity")}
+    if std_c is not None:
+        out["std_candidates_per_layer"] = list(std_c)
+        out["std_frontier_survivors_per_layer"] = list(std_s)
+    return out
+
+
+def _take_frontier(lib, fr: abi.Frontier) -> np.ndarray:
+    n, d = int(fr.count), int(fr.d)
+    if n > 0:
+        arr = np.ctypeslib.as_array(fr.points, shape=(n * d,)).copy().reshape(n, d)
+    else:
+        arr = np.zeros((0, max(d, 1)), dtype=np.int32)
+    lib.pmdd_frontier_free(C.byref(fr))
+    return arr
+
+
+class Engine:
+    """A per-device engine context (pmdd_ctx)."""
+
+    def __init__(self, device: int = 0):
+        lib = abi.engine_lib()
+        self._lib = lib
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib.pmdd_ctx_create(int(device), C.byref(h), err, len(err))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, err.value.decode())
+        self._h = h
+        self.device = device
+
+    @staticmethod
+    def device_count() -> int:
+        return int(abi.engine_lib().pmdd_device_count())
+
+    @staticmethod
+    def version() -> str:
+        return abi.engine_lib().pmdd_engine_version().decode()
+
+    def _opts(self, method, problem_type, state_dominance, max_cand, max_prod, count_pair_tests):
+        o = abi.Opts()
+        o.method = int(method)
+        o.problem_type = int(problem_type)
+        o.state_dominance = int(state_dominance)
+        o.count_pair_tests = 1 if count_pair_tests else 0
+        o.max_cand = int(max_cand)
+        o.max_prod = int(max_prod)
+        return o
+
+    def enumerate(self, dd, method: int = 1, state_dominance: int = 0,
+                  max_cand: int = abi.DEFAULT_MAX_CAND, max_prod: int = abi.DEFAULT_MAX_PROD,
+                  count_pair_tests: bool = False, sort: bool = True) -> EnumerationResult:
+        o = self._opts(method, dd.problem_type, state_dominance, max_cand, max_prod,
+                       count_pair_tests)
+        fr = abi.Frontier()
+        st = abi.Stats()
+        nl = dd.num_layers
+        std_c = (C.c_double * nl)()
+        std_s = (C.c_double * nl)()
+        st.std_capacity = nl
+        st.std_candidates_per_layer = std_c
+        st.std_frontier_survivors_per_layer = std_s
+        err = C.create_string_buffer(1024)
+        rc = self._lib.pmdd_enumerate(self._h, dd.view_ptr, C.byref(o), C.byref(fr),
+                                      C.byref(st), err, len(err))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, err.value.decode())
+        pts = _take_frontier(self._lib, fr)
+        if sort:
+            pts = sort_lexicographic_ascending(pts)
+        return EnumerationResult(pts, _stats_dict(st, nl, std_c, std_s))
+
+    def upload(self, dd) -> "ResidentDD":
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = self._lib.pmdd_upload(self._h, dd.view_ptr, C.byref(h), err, len(err))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, err.value.decode())
+        return ResidentDD(self, h, dd.problem_type, dd.num_layers, dd.d)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            self._lib.pmdd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class ResidentDD:
+    """A diagram resident in HBM (pmdd_resident); enumerations reuse it."""
+
+    def __init__(self, engine: Engine, handle: C.c_void_p, problem_type: int, num_layers: int,
+                 d: int):
+        self.engine = engine
+        self._h = handle
+        self.problem_type = problem_type
+        self.num_layers = num_layers
+        self.d = d
+
+    def enumerate(self, method: int = 1, state_dominance: int = 0,
+                  max_cand: int = abi.DEFAULT_MAX_CAND, max_prod: int = abi.DEFAULT_MAX_PROD,
+                  count_pair_tests: bool = False, sort: bool = True) -> EnumerationResult:
+        eng = self.engine
+        o = eng._opts(method, self.problem_type, state_dominance, max_cand, max_prod,
+                      count_pair_tests)
+        fr = abi.Frontier()
+        st = abi.Stats()
+        err = C.create_string_buffer(1024)
+        rc = eng._lib.pmdd_enumerate_resident(eng._h, self._h, C.byref(o), C.byref(fr),
+                                              C.byref(st), err, len(err))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, err.value.decode())
+        pts = _take_frontier(eng._lib, fr)
+        if sort:
+            pts = sort_lexicographic_ascending(pts)
+        return EnumerationResult(pts, _stats_dict(st, self.num_layers, None, None))
+
+    def close(self) -> None:
+        if self._h is not None:
+            self.engine._lib.pmdd_resident_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

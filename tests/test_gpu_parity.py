@@ -1,0 +1,144 @@
+"""GPU parity: the HIP engine, called through the C ABI, against the oracle.
+
+Every comparison is bit-exact on the frontier as a lexicographically sorted
+set of int32 vectors, the way the reference writes it (main.cpp:403).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import instance_path, problem_type_of
+from pmdd_amd import DecisionDiagram
+from pmdd_amd import instances as gen
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = sorted(os.listdir(os.path.join(os.path.dirname(__file__), "golden", "instances")))
+# the 15-city TSP fixtures are covered by the known-count tests below
+SMALL_FIXTURES = [f for f in FIXTURES if "ncities15" not in f]
+
+
+def _dd(name):
+    return DecisionDiagram.from_file(instance_path(name), problem_type_of(name))
+
+
+def _same(a, b):
+    return a.shape == b.shape and bool((a == b).all())
+
+
+@pytest.mark.parametrize("name", SMALL_FIXTURES)
+@pytest.mark.parametrize("method", [1, 3])
+def test_fixture_matches_oracle(engine, name, method):
+    print(f"case {name} method={method}", flush=True)
+    dd = _dd(name)
+    ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
+    sds = [0, 1] if problem_type_of(name) != 3 else [0]
+    for sd in sds:
+        got = engine.enumerate(dd, method=method, state_dominance=sd).points
+        assert _same(got, ref), f"{name} m{method} sd{sd}: {len(got)} vs {len(ref)}"
+
+
+@pytest.mark.parametrize("name", ["KP_p-3_n-30_ins-2.dat", "tsp-nobj3-ncities10-seed12870.dat",
+                                  "bp-100-20-3-10-10192.dat"])
+@pytest.mark.parametrize("max_cand", [1, 7, 1000])
+def test_candidate_budget_invariance(engine, name, max_cand):
+    print(f"case {name} max_cand={max_cand}", flush=True)
+    dd = _dd(name)
+    ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
+    for method in (1, 3):
+        got = engine.enumerate(dd, method=method, max_cand=max_cand).points
+        assert _same(got, ref)
+
+
+@pytest.mark.parametrize("name", ["KP_p-3_n-30_ins-2.dat", "tsp-nobj3-ncities10-seed12870.dat"])
+@pytest.mark.parametrize("max_prod", [1, 1000, 625000])
+def test_product_budget_invariance(engine, name, max_prod):
+    print(f"case {name} max_prod={max_prod}", flush=True)
+    dd = _dd(name)
+    ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
+    got = engine.enumerate(dd, method=3, max_prod=max_prod).points
+    assert _same(got, ref)
+
+
+def test_tsp15_seed108_known_count(engine):
+    """831 solutions: reports/report.tex:226 of the reference."""
+    dd = _dd("tsp-nobj3-ncities15-seed108.dat")
+    ref = oracle.enumerate_dd(dd, 3, 0, threads=8)
+    assert len(ref) == 831
+    for method in (1, 3):
+        got = engine.enumerate(dd, method=method).points
+        assert _same(got, ref)
+
+
+def test_tsp15_seed11417_known_count(engine):
+    """5372 solutions: reports/report.tex:227 of the reference."""
+    dd = _dd("tsp-nobj4-ncities15-seed11417.dat")
+    a = engine.enumerate(dd, method=3).points
+    assert len(a) == 5372
+    b = engine.enumerate(dd, method=1).points
+    assert _same(a, b)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_generated_small_vs_bruteforce(engine, tmp_path, seed):
+    cases = [
+        gen.knapsack(seed, 14 + seed, 3),
+        gen.knapsack(100 + seed, 12, 1 + seed % 8),
+        gen.setpacking(200 + seed, 20 + 2 * seed, 2 + seed % 4, m=6),
+        gen.tsp(300 + seed, 6 + seed % 3, 2 + seed % 6),
+    ]
+    for k, inst in enumerate(cases):
+        path = gen.write(inst, str(tmp_path / f"g{seed}_{k}.dat"))
+        print(f"case seed={seed} kind={inst['kind']} n={inst['n']} d={inst['d']}", flush=True)
+        bf = oracle.bruteforce(path, inst["kind"])
+        dd = DecisionDiagram.from_file(path, inst["kind"])
+        for method in (1, 3):
+            for sd in (0, 1):
+                got = engine.enumerate(dd, method=method, state_dominance=sd).points
+                assert _same(got, bf), (seed, k, method, sd)
+
+
+@pytest.mark.parametrize("d", [3, 5])
+def test_generated_knapsack_stats_match_oracle(engine, tmp_path, d):
+    """Order-independent work counters of method 1 without state dominance."""
+    inst = gen.knapsack(7 + d, 30, d)
+    path = gen.write(inst, str(tmp_path / "kp.dat"))
+    dd = DecisionDiagram.from_file(path, 1)
+    ref, st, std_c, std_s = oracle.enumerate_dd(dd, 1, 0, threads=4, with_stats=True)
+    res = engine.enumerate(dd, method=1)
+    assert _same(res.points, ref)
+    s = res.stats
+    assert s["work_candidates_total"] == st.work_candidates_total
+    assert s["work_candidates_peak"] == st.work_candidates_peak
+    assert s["work_frontier_survivors_total"] == st.work_frontier_survivors_total
+    assert s["work_frontier_peak_points"] == st.work_frontier_peak_points
+    np.testing.assert_allclose(s["std_candidates_per_layer"], std_c, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(s["std_frontier_survivors_per_layer"], std_s, rtol=1e-12,
+                               atol=1e-9)
+
+
+def test_coupled_layer_matches_oracle_gpu_rule(engine):
+    dd = _dd("tsp-nobj3-ncities10-seed12870.dat")
+    ref, st, _, _ = oracle.enumerate_dd(dd, 3, 0, threads=4, score_rule=1, with_stats=True)
+    res = engine.enumerate(dd, method=3)
+    assert _same(res.points, ref)
+    assert res.stats["layer_coupling"] == st.layer_coupling
+    assert res.stats["work_join_products_total"] == st.work_join_products_total
+
+
+def test_resident_reuse(engine):
+    dd = _dd("KP_p-3_n-20_ins-1.dat")
+    ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
+    r = engine.upload(dd)
+    for method in (1, 3, 1):
+        assert _same(r.enumerate(method=method).points, ref)
+    r.close()
+
+
+def test_pair_test_counter_is_deterministic(engine):
+    dd = _dd("KP_p-3_n-30_ins-2.dat")
+    a = engine.enumerate(dd, method=1, count_pair_tests=True).stats["dominance_pair_tests"]
+    b = engine.enumerate(dd, method=1, count_pair_tests=True).stats["dominance_pair_tests"]
+    assert a == b and a > 0
