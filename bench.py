@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark: exact Pareto-frontier enumeration on one MI355X.
+
+Workload (one "step"): the single-large-knapsack configuration -- a
+3-objective knapsack BDD with n=80 items and a 5-objective one with n=50
+(weights and profits int32 uniform [1,1000], capacity = half the weight sum,
+seeded SplitMix64), each enumerated with method 1 (top-down) and method 3
+(dynamic layer cutset), state dominance 0 and 1: eight enumerations.  The
+diagrams are built and uploaded to HBM before the timed region; every
+enumeration in the timed region is complete (all layers, coupling, result
+copied back to the host).
+
+Prints one JSON line.  `value` = exact Pareto points per second over the
+timed steps (sum of final frontier sizes / wall time).  `roofline` covers the
+dominant kernel (k_filter_chunks); `cpu_baseline` times the CPU restatement
+of the reference's OpenMP enumerators on a bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
+VALU_PEAK_TOPS = 39.3           # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, int32 lane-ops/s (T)
+
+WORKLOAD = [
+    # (name, seed, n, d)
+    ("kp_d3_n80", 80_3, 80, 3),
+    ("kp_d5_n50", 50_5, 50, 5),
+]
+CASES = [(1, 0), (1, 1), (3, 0), (3, 1)]  # (method, state_dominance)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(tmpdir):
+    from pmdd_amd import DecisionDiagram
+    from pmdd_amd import instances as gen
+    dds = []
+    for name, seed, n, d in WORKLOAD:
+        path = gen.write(gen.knapsack(seed, n, d), os.path.join(tmpdir, f"{name}.dat"))
+        dds.append((name, path, DecisionDiagram.from_file(path, 1)))
+    return dds
+
+
+def read_traffic():
+    """HBM bytes per k_filter_chunks launch from the committed rocprofv3 PMC
+    summary (profiles/k_filter_chunks_kp_traffic.txt), or None."""
+    p = os.path.join(ROOT, "profiles", "k_filter_chunks_kp_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    vals = {}
+    with open(p) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(dds, threads):
+    """Oracle (CPU restatement of the reference OpenMP path) on a bounded
+    sample: the n=80 3-objective knapsack, methods 1 and 3, no state
+    dominance."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    name, path, dd = dds[0]
+    pts = 0
+    t0 = time.perf_counter()
+    for method in (1, 3):
+        pts += len(oracle.enumerate_dd(dd, method, 0, threads=threads))
+    dt = time.perf_counter() - t0
+    return {
+        "value": pts / dt,
+        "unit": "Pareto pts/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{name} (seeded knapsack n=80 d=3) methods 1 and 3, state dominance 0; "
+                  f"{pts} Pareto points in {dt:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--skip-pair-count", action="store_true",
+                    help="skip the extra run that counts executed pair tests")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
+                          "(instances shard across GPUs without communication)"}))
+        sys.exit(2)
+
+    from pmdd_amd.engine import Engine
+
+    tmp = tempfile.mkdtemp(prefix="pmdd_bench_")
+    dds = build_workload(tmp)
+    eng = Engine(0)
+    res = [(name, eng.upload(dd)) for name, _, dd in dds]
+    log(f"engine {Engine.version()}; workload " + ", ".join(n for n, _ in res))
+
+    def step():
+        pts = 0
+        agg = {"kernel_filter_s": 0.0, "filter_launches": 0, "algorithmic_bytes": 0,
+               "work_frontier_survivors_total": 0, "kernel_total_s": 0.0}
+        per = {}
+        for name, r in res:
+            ref = None
+            for method, sd in CASES:
+                out = r.enumerate(method=method, state_dominance=sd)
+                if ref is None:
+                    ref = out.points
+                elif out.points.shape != ref.shape or not (out.points == ref).all():
+                    raise SystemExit(f"{name}: method {method} sd {sd} frontier differs")
+                pts += len(out.points)
+                for k in agg:
+                    agg[k] += out.stats[k]
+                per[f"{name}_m{method}_sd{sd}"] = len(out.points)
+        return pts, agg, per
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup} done")
+
+    total_pts = 0
+    tot = None
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        pts, agg, per = step()
+        total_pts += pts
+        if tot is None:
+            tot = dict(agg)
+        else:
+            for k in agg:
+                tot[k] += agg[k]
+        log(f"step {i + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / max(args.steps, 1)
+    enumerations = args.steps * len(res) * len(CASES)
+
+    # executed pair tests of one step (deterministic), from a counting run
+    pair_tests = None
+    if not args.skip_pair_count:
+        pair_tests = 0
+        for name, r in res:
+            for method, sd in CASES:
+                pair_tests += r.enumerate(method=method, state_dominance=sd,
+                                          count_pair_tests=True).stats["dominance_pair_tests"]
+        log("pair-test count done")
+
+    f_time = tot["kernel_filter_s"]
+    launches = max(tot["filter_launches"], 1)
+    alg_bytes_per_launch = tot["algorithmic_bytes"] / launches
+    avg_launch_s = f_time / launches
+    achieved_gbs = alg_bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = read_traffic()
+    roofline = {
+        "kernel": "k_filter_chunks",
+        "bound": "hbm",
+        "achieved": achieved_gbs,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved_gbs / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "avg_launch_ms": avg_launch_s * 1e3,
+        "launches_per_step": tot["filter_launches"] / max(args.steps, 1),
+        "algorithmic_bytes_per_launch": alg_bytes_per_launch,
+    }
+    if pair_tests is not None:
+        d_avg = sum(d for _, _, _, d in WORKLOAD) / len(WORKLOAD)
+        tops = pair_tests * d_avg * args.steps / max(tot["kernel_total_s"], 1e-12) / 1e12
+        roofline["valu"] = {
+            "achieved": tops, "peak": VALU_PEAK_TOPS, "unit": "T int32 lane-ops/s",
+            "frac": tops / VALU_PEAK_TOPS,
+            "pair_tests_per_step": pair_tests,
+            "note": "d compares per executed pair test over all engine kernel time",
+        }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(dds, threads)
+        log(f"cpu baseline done: {cpu['value']:.1f} pts/s on {threads} threads")
+
+    line = {
+        "metric": "exact Pareto pts/s, inst/s, p50/p99 latency: batched vs plain GPU",
+        "value": total_pts / elapsed,
+        "unit": "Pareto pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded SplitMix64 knapsack instances)",
+        "config": {
+            "workload": "single large knapsack BDDs: d=3 n=80 and d=5 n=50, methods 1 and 3, "
+                        "state dominance 0 and 1 (8 enumerations per step), plain GPU path",
+            "instances": [n for n, _, _ in dds],
+            "max_cand": 20_000_000,
+            "max_prod": 625_000,
+        },
+        "inst_per_s": enumerations / elapsed,
+        "frontier_points_processed_per_s": tot["work_frontier_survivors_total"] / elapsed,
+        "comparisons_per_s": (pair_tests * args.steps / elapsed) if pair_tests else None,
+        "pareto_points_per_step": total_pts / max(args.steps, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for _, r in res:
+        r.close()
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -123,6 +123,14 @@ typedef struct pmdd_stats {
     int64_t work_join_products_total;
     int64_t dominance_filtered_total;
     int64_t dominance_pair_tests; /* executed pair tests; 0 unless opts.count_pair_tests */
+    /* Engine-specific figures for roofline accounting (no reference counterpart):
+     * the fused filter kernel (k_filter_chunks) alone -- its total device time,
+     * launch count and candidates loaded -- and the compulsory HBM bytes of the
+     * expanded layers, 4*(d*(S_prev+S_next) + E*(d+1) + 2*(N+1) + (N_prev+1)). */
+    double kernel_filter_s;
+    int64_t filter_launches;
+    int64_t filter_candidates;
+    int64_t algorithmic_bytes;
     int32_t layer_coupling;
     int32_t std_capacity;
     double *std_candidates_per_layer;

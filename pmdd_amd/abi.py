@@ -86,6 +86,10 @@ class Stats(C.Structure):
         ("work_join_products_total", C.c_int64),
         ("dominance_filtered_total", C.c_int64),
         ("dominance_pair_tests", C.c_int64),
+        ("kernel_filter_s", C.c_double),
+        ("filter_launches", C.c_int64),
+        ("filter_candidates", C.c_int64),
+        ("algorithmic_bytes", C.c_int64),
         ("layer_coupling", C.c_int32),
         ("std_capacity", C.c_int32),
         ("std_candidates_per_layer", C.POINTER(C.c_double)),

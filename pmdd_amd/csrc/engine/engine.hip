@@ -104,7 +104,7 @@ double now_s() {
 struct pmdd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
     // scratch reused across layers and calls
     DevBuf chunks, run_off, nsurv, nalive, minkey, maxkey, chunk_out, which;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
@@ -583,6 +583,7 @@ struct Run {
         hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
         load(dim3(static_cast<unsigned>(nch)));
         hip_check(hipGetLastError(), "k_filter_chunks");
+        hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
         std::vector<int32_t> which;
         for (int64_t c = 0; c < nch; ++c) {
             if (chunks[c].group_count > 1) {
@@ -600,6 +601,11 @@ struct Run {
         hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
         d2h(nalive_h, ctx->nalive, nch, s, "chunk survivor counts");
         const double ms = elapsed_ms();
+        float f1_ms = 0.f;
+        hip_check(hipEventElapsedTime(&f1_ms, ctx->ev_a, ctx->ev_c), "hipEventElapsedTime");
+        st->kernel_filter_s += f1_ms * 1e-3;
+        st->filter_launches += 1;
+        st->filter_candidates += pass_cands;
         st->kernel_dominance_s += ms * 1e-3;
         st->kernel_total_s += ms * 1e-3;
     }
@@ -729,6 +735,9 @@ struct Run {
             sample_memory();
         }
         upload_offsets(next, s);
+        st->algorithmic_bytes +=
+            4 * (static_cast<int64_t>(d) * (prev.total() + next_total) + csr.n_arcs * (d + 1) +
+                 2 * (n_nodes + 1) + (prev.n_nodes + 1));
         if (gamma_out != nullptr) {
             gamma_out->swap(gamma);
         }
@@ -1038,6 +1047,11 @@ struct Run {
                 at += nalive[c];
             }
             const int64_t b_count = at;
+            int64_t batch_inputs = 0;
+            for (int32_t k = 0; k < Kn; ++k) {
+                batch_inputs += td.size(nodes[i + k]) + bu.size(nodes[i + k]);
+            }
+            st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + b_count);
             grow_points(B, d, std::max<int64_t>(b_count, 1), 0, s);
             compact_pass(chunks, bprod, chunk_out, B.pts.as<int32_t>(), B.cap);
 
@@ -1254,6 +1268,7 @@ int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&c->ev_a), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev_c), "hipEventCreate");
         *out = c.release();
         return PMDD_OK;
     });
@@ -1290,6 +1305,9 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
     }
     if (ctx->ev_b != nullptr) {
         (void)hipEventDestroy(ctx->ev_b);
+    }
+    if (ctx->ev_c != nullptr) {
+        (void)hipEventDestroy(ctx->ev_c);
     }
     if (ctx->stream != nullptr) {
         (void)hipStreamDestroy(ctx->stream);
