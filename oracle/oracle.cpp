@@ -269,6 +269,7 @@ struct OracleRun {
     // stats
     int64_t cand_total = 0, cand_peak = 0, surv_total = 0, surv_peak = 0;
     int64_t join_products = 0, filtered = 0;
+    int64_t max_node_frontier = 0, max_node_candidates = 0;
     int32_t layer_coupling = 0;
     std::vector<double> std_cand, std_surv;
 
@@ -329,6 +330,8 @@ struct OracleRun {
             c += cand[i];
             sizes[i] = fr[i]->size();
             s += sizes[i];
+            max_node_frontier = std::max(max_node_frontier, sizes[i]);
+            max_node_candidates = std::max(max_node_candidates, cand[i]);
         }
         cand_total += c;
         cand_peak = std::max(cand_peak, c);
@@ -819,6 +822,8 @@ typedef struct oracle_stats {
     int64_t dominance_filtered_total;
     int32_t layer_coupling;
     int32_t reserved;
+    int64_t max_node_frontier;
+    int64_t max_node_candidates;
 } oracle_stats;
 
 // method: 1 top-down, 2 bottom-up, 3 coupled.  score_rule picks the method-3
@@ -856,6 +861,8 @@ int oracle_enumerate(const pmdd_dd_view *v, int method, int problem_type, int st
         st->work_join_products_total = run.join_products;
         st->dominance_filtered_total = run.filtered;
         st->layer_coupling = run.layer_coupling;
+        st->max_node_frontier = run.max_node_frontier;
+        st->max_node_candidates = run.max_node_candidates;
     }
     if (std_cand != nullptr) {
         std::copy(run.std_cand.begin(), run.std_cand.end(), std_cand);

@@ -29,6 +29,8 @@ class OracleStats(C.Structure):
         ("dominance_filtered_total", C.c_int64),
         ("layer_coupling", C.c_int32),
         ("reserved", C.c_int32),
+        ("max_node_frontier", C.c_int64),
+        ("max_node_candidates", C.c_int64),
     ]
 
 

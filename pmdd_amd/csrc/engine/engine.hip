@@ -101,13 +101,21 @@ double now_s() {
 // ---------------------------------------------------------------------------
 // Context and resident diagram
 // ---------------------------------------------------------------------------
+// Device tables of one chunk list: descriptors, per-chunk survivor counts
+// (local and after cross checks), key range of the survivors, output slots.
+struct ChunkBufs {
+    DevBuf chunks, nsurv, nalive, minkey, maxkey, out;
+};
+
 struct pmdd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
     // scratch reused across layers and calls
-    DevBuf chunks, run_off, nsurv, nalive, minkey, maxkey, chunk_out, which;
+    ChunkBufs cb_main, cb_aux;
+    DevBuf run_off, which, which_a, which_b, which_c;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
+    DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
 };
 
 struct LayerCSR {
@@ -415,48 +423,62 @@ void seed_frontier(DevFrontier &f, int d, int64_t n_nodes, int64_t node, hipStre
 // ---------------------------------------------------------------------------
 // Kernel launchers, one instantiation per objective count.
 // ---------------------------------------------------------------------------
+using FilterFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const void *,
+                          int32_t *, int64_t, uint8_t *, int32_t *, int32_t *, int64_t *,
+                          int64_t *, unsigned long long *);
+using CrossFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
+                         int64_t, uint8_t *, const int32_t *, int32_t *, const int64_t *,
+                         const int64_t *, unsigned long long *);
+using VsSortedFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *,
+                            const int32_t *, int64_t, uint8_t *, const int32_t *, int32_t *,
+                            const int32_t *, int64_t, int64_t, unsigned long long *);
+
 struct Launch {
     int chunk_cap;
-    void (*filter_expand)(dim3, hipStream_t, const ChunkDesc *, pmdd_k::ExpandSrc, int32_t *,
-                          int64_t, uint8_t *, int32_t *, int32_t *, int64_t *, int64_t *,
-                          unsigned long long *);
-    void (*filter_join)(dim3, hipStream_t, const ChunkDesc *, pmdd_k::JoinSrc, int32_t *, int64_t,
-                        uint8_t *, int32_t *, int32_t *, int64_t *, int64_t *,
-                        unsigned long long *);
-    void (*cross)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *, int64_t,
-                  uint8_t *, const int32_t *, int32_t *, const int64_t *, const int64_t *,
-                  unsigned long long *);
+    FilterFn filter_expand;  // ExpandSrc, presorted in LDS
+    FilterFn filter_join;    // JoinSrc, presorted in LDS
+    FilterFn filter_sorted;  // SortedSrc, source already in key order
+    CrossFn cross_sym;       // unordered chunks of a group
+    CrossFn cross_one_sided; // chunks are descending key ranges
+    VsSortedFn vs_weak;
+    VsSortedFn vs_strict;
     void (*compact)(dim3, hipStream_t, const ChunkDesc *, const int32_t *, const int64_t *,
                     const int32_t *, int64_t, const uint8_t *, int32_t *, int64_t);
     void (*state_dom)(dim3, hipStream_t, const int32_t *, const int64_t *, const int32_t *,
                       int64_t, uint8_t *);
     void (*compact_nodes)(dim3, hipStream_t, const int64_t *, const int32_t *, int64_t,
                           const uint8_t *, const int64_t *, int32_t *, int64_t);
-    void (*vs_frontier)(dim3, hipStream_t, const int32_t *, int64_t, int64_t, const int32_t *,
-                        int64_t, int64_t, uint8_t *, uint8_t *, int64_t, unsigned long long *);
+    void (*gather_runs)(dim3, hipStream_t, const int64_t *, const int32_t *, int32_t,
+                        pmdd_k::ExpandSrc, int32_t *, int64_t);
+    void (*merge_pass)(dim3, hipStream_t, const int32_t *, int32_t *, int64_t, const int64_t *,
+                       const int32_t *, int32_t, int, int64_t);
 };
 
 template <int D> struct LaunchImpl {
     static constexpr int CH = pmdd_k::Cap<D>::CH;
-    static void filter_expand(dim3 g, hipStream_t s, const ChunkDesc *c, pmdd_k::ExpandSrc src,
-                              int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns,
-                              int32_t *na, int64_t *kmin, int64_t *kmax,
-                              unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, CH, pmdd_k::ExpandSrc>), g, dim3(kBlock),
-                           0, s, c, src, out, stride, alive, ns, na, kmin, kmax, pt);
+    template <class Src, bool PRE>
+    static void filter(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
+                       const void *src, int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns,
+                       int32_t *na, int64_t *kmin, int64_t *kmax, unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, CH, Src, PRE>), g, dim3(kBlock), 0, s,
+                           which, c, *static_cast<const Src *>(src), out, stride, alive, ns, na,
+                           kmin, kmax, pt);
     }
-    static void filter_join(dim3 g, hipStream_t s, const ChunkDesc *c, pmdd_k::JoinSrc src,
-                            int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns, int32_t *na,
-                            int64_t *kmin, int64_t *kmax, unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, CH, pmdd_k::JoinSrc>), g, dim3(kBlock), 0,
-                           s, c, src, out, stride, alive, ns, na, kmin, kmax, pt);
-    }
+    template <bool ONE>
     static void cross(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
                       int32_t *na, const int64_t *kmin, const int64_t *kmax,
                       unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_cross_chunks<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
-                           stride, alive, ns, na, kmin, kmax, pt);
+        hipLaunchKernelGGL((pmdd_k::k_cross_chunks<D, CH, ONE>), g, dim3(kBlock), 0, s, which, c,
+                           pts, stride, alive, ns, na, kmin, kmax, pt);
+    }
+    template <bool STRICT>
+    static void vs(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
+                   const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
+                   int32_t *na, const int32_t *B, int64_t bs, int64_t nb,
+                   unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_vs_sorted<D, CH, STRICT>), g, dim3(kBlock), 0, s, which, c,
+                           pts, stride, alive, ns, na, B, bs, nb, pt);
     }
     static void compact(dim3 g, hipStream_t s, const ChunkDesc *c, const int32_t *ns,
                         const int64_t *out, const int32_t *src, int64_t ss, const uint8_t *alive,
@@ -475,15 +497,33 @@ template <int D> struct LaunchImpl {
         hipLaunchKernelGGL((pmdd_k::k_compact_nodes<D>), g, dim3(kBlock), 0, s, oi, src, ss, keep,
                            oo, dst, ds);
     }
-    static void vs_frontier(dim3 g, hipStream_t s, const int32_t *B, int64_t bs, int64_t nb,
-                            const int32_t *R, int64_t rs, int64_t nr, uint8_t *kb, uint8_t *kr,
-                            int64_t nblk, unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_vs_frontier<D>), g, dim3(kBlock), 0, s, B, bs, nb, R, rs, nr,
-                           kb, kr, nblk, pt);
+    static void gather_runs(dim3 g, hipStream_t s, const int64_t *base, const int32_t *node,
+                            int32_t G, pmdd_k::ExpandSrc src, int32_t *out, int64_t os) {
+        hipLaunchKernelGGL((pmdd_k::k_gather_runs<D>), g, dim3(kBlock), 0, s, base, node, G, src,
+                           out, os);
+    }
+    static void merge_pass(dim3 g, hipStream_t s, const int32_t *in, int32_t *out, int64_t stride,
+                           const int64_t *run_start, const int32_t *grp_run, int32_t G, int pass,
+                           int64_t total) {
+        hipLaunchKernelGGL((pmdd_k::k_merge_pass<D>), g, dim3(kBlock), 0, s, in, out, stride,
+                           run_start, grp_run, G, pass, total);
     }
     static Launch table() {
-        return Launch{CH, filter_expand, filter_join, cross, compact, state_dom, compact_nodes,
-                      vs_frontier};
+        Launch L;
+        L.chunk_cap = CH;
+        L.filter_expand = filter<pmdd_k::ExpandSrc, false>;
+        L.filter_join = filter<pmdd_k::JoinSrc, false>;
+        L.filter_sorted = filter<pmdd_k::SortedSrc, true>;
+        L.cross_sym = cross<false>;
+        L.cross_one_sided = cross<true>;
+        L.vs_weak = vs<false>;
+        L.vs_strict = vs<true>;
+        L.compact = compact;
+        L.state_dom = state_dom;
+        L.compact_nodes = compact_nodes;
+        L.gather_runs = gather_runs;
+        L.merge_pass = merge_pass;
+        return L;
     }
 };
 
@@ -499,6 +539,14 @@ Launch launch_for(int d) {
     case 8: return LaunchImpl<8>::table();
     default: fail(PMDD_E_UNSUPPORTED, "objective count outside 1..8");
     }
+}
+
+inline int ceil_log2(int64_t x) {
+    int p = 0;
+    while ((int64_t(1) << p) < x) {
+        ++p;
+    }
+    return p;
 }
 
 template <class T> void h2d(DevBuf &b, const std::vector<T> &v, hipStream_t s, const char *what) {
@@ -566,75 +614,95 @@ struct Run {
         return ms;
     }
 
-    // Filter `chunks` of the pass (candidates loaded by `load`), then compact
-    // the alive survivors into `next` after its current `next_total` points.
-    // Returns the alive count per chunk (host).
-    template <class LoadFn>
-    void filter_pass(const std::vector<ChunkDesc> &chunks, int64_t pass_cands, LoadFn load,
-                     std::vector<int32_t> &nalive_h) {
-        const int64_t nch = static_cast<int64_t>(chunks.size());
-        h2d(ctx->chunks, chunks, s, "hipMemcpy(chunk table)");
-        ctx->surv_pts.ensure(static_cast<size_t>(pass_cands) * d * 4, "hipMalloc(survivor buffer)");
-        ctx->alive.ensure(static_cast<size_t>(pass_cands), "hipMalloc(alive flags)");
-        ctx->nsurv.ensure(nch * 4, "hipMalloc(chunk counts)");
-        ctx->nalive.ensure(nch * 4, "hipMalloc(chunk counts)");
-        ctx->minkey.ensure(nch * 8, "hipMalloc(chunk keys)");
-        ctx->maxkey.ensure(nch * 8, "hipMalloc(chunk keys)");
-        hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-        load(dim3(static_cast<unsigned>(nch)));
-        hip_check(hipGetLastError(), "k_filter_chunks");
-        hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-        std::vector<int32_t> which;
-        for (int64_t c = 0; c < nch; ++c) {
-            if (chunks[c].group_count > 1) {
-                which.push_back(static_cast<int32_t>(c));
-            }
-        }
-        if (!which.empty()) {
-            h2d(ctx->which, which, s, "hipMemcpy(cross list)");
-            K.cross(dim3(static_cast<unsigned>(which.size())), s, ctx->which.as<int32_t>(),
-                    ctx->chunks.as<ChunkDesc>(), ctx->surv_pts.as<int32_t>(), pass_cands,
-                    ctx->alive.as<uint8_t>(), ctx->nsurv.as<int32_t>(), ctx->nalive.as<int32_t>(),
-                    ctx->minkey.as<int64_t>(), ctx->maxkey.as<int64_t>(), d_tests);
-            hip_check(hipGetLastError(), "k_cross_chunks");
-        }
-        hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-        d2h(nalive_h, ctx->nalive, nch, s, "chunk survivor counts");
-        const double ms = elapsed_ms();
-        float f1_ms = 0.f;
-        hip_check(hipEventElapsedTime(&f1_ms, ctx->ev_a, ctx->ev_c), "hipEventElapsedTime");
-        st->kernel_filter_s += f1_ms * 1e-3;
-        st->filter_launches += 1;
-        st->filter_candidates += pass_cands;
-        st->kernel_dominance_s += ms * 1e-3;
-        st->kernel_total_s += ms * 1e-3;
+    // ---- chunk tables ----------------------------------------------------
+    void upload_table(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks) {
+        const size_t n = std::max<size_t>(chunks.size(), 1);
+        h2d(cb.chunks, chunks, s, "hipMemcpy(chunk table)");
+        cb.nsurv.ensure(n * 4, "hipMalloc(chunk counts)");
+        cb.nalive.ensure(n * 4, "hipMalloc(chunk counts)");
+        cb.minkey.ensure(n * 8, "hipMalloc(chunk keys)");
+        cb.maxkey.ensure(n * 8, "hipMalloc(chunk keys)");
     }
 
-    void compact_pass(const std::vector<ChunkDesc> &chunks, int64_t pass_cands,
-                      const std::vector<int64_t> &chunk_out, int32_t *dst, int64_t dst_stride) {
-        const int64_t nch = static_cast<int64_t>(chunks.size());
-        h2d(ctx->chunk_out, chunk_out, s, "hipMemcpy(chunk outputs)");
-        hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-        K.compact(dim3(static_cast<unsigned>(nch)), s, ctx->chunks.as<ChunkDesc>(),
-                  ctx->nsurv.as<int32_t>(), ctx->chunk_out.as<int64_t>(),
-                  ctx->surv_pts.as<int32_t>(), pass_cands, ctx->alive.as<uint8_t>(), dst,
-                  dst_stride);
+    // Launch a filter over the listed chunks (all chunks when `which` is empty
+    // and `count` > 0).
+    void run_filter(FilterFn f, ChunkBufs &cb, DevBuf &wbuf, const std::vector<int32_t> &which,
+                    int64_t count, const void *src, int32_t *out, int64_t stride, uint8_t *alive) {
+        const int32_t *w = nullptr;
+        int64_t n = count;
+        if (!which.empty()) {
+            h2d(wbuf, which, s, "hipMemcpy(chunk list)");
+            w = wbuf.as<int32_t>();
+            n = static_cast<int64_t>(which.size());
+        }
+        if (n <= 0) {
+            return;
+        }
+        f(dim3(static_cast<unsigned>(n)), s, w, cb.chunks.as<ChunkDesc>(), src, out, stride, alive,
+          cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), cb.minkey.as<int64_t>(),
+          cb.maxkey.as<int64_t>(), d_tests);
+        hip_check(hipGetLastError(), "k_filter_chunks");
+    }
+
+    void run_cross(CrossFn f, ChunkBufs &cb, DevBuf &wbuf, const std::vector<int32_t> &which,
+                   const int32_t *pts, int64_t stride, uint8_t *alive) {
+        if (which.empty()) {
+            return;
+        }
+        h2d(wbuf, which, s, "hipMemcpy(cross list)");
+        f(dim3(static_cast<unsigned>(which.size())), s, wbuf.as<int32_t>(),
+          cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(),
+          cb.nalive.as<int32_t>(), cb.minkey.as<int64_t>(), cb.maxkey.as<int64_t>(), d_tests);
+        hip_check(hipGetLastError(), "k_cross_chunks");
+    }
+
+    void run_compact(ChunkBufs &cb, int64_t nch, const std::vector<int64_t> &outpos,
+                     const int32_t *src, int64_t ss, const uint8_t *alive, int32_t *dst,
+                     int64_t ds) {
+        if (nch <= 0) {
+            return;
+        }
+        h2d(cb.out, outpos, s, "hipMemcpy(chunk outputs)");
+        K.compact(dim3(static_cast<unsigned>(nch)), s, cb.chunks.as<ChunkDesc>(),
+                  cb.nsurv.as<int32_t>(), cb.out.as<int64_t>(), src, ss, alive, dst, ds);
         hip_check(hipGetLastError(), "k_compact_chunks");
-        hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-        hip_check(hipStreamSynchronize(s), "compaction sync");
-        const double ms = elapsed_ms();
-        st->kernel_expand_td_s += ms * 1e-3;
-        st->kernel_total_s += ms * 1e-3;
+    }
+
+    // Merge the runs of each group (each run in descending key order) into one
+    // run per group; returns the buffer (a or b) holding the result.
+    int32_t *merge_runs(DevBuf &a, DevBuf &b, int64_t total, const std::vector<int64_t> &run_start,
+                        const std::vector<int32_t> &grp_run, int64_t max_runs) {
+        const int passes = ceil_log2(max_runs);
+        if (passes == 0 || total == 0) {
+            return a.as<int32_t>();
+        }
+        h2d(ctx->run_start, run_start, s, "hipMemcpy(run table)");
+        h2d(ctx->grp_run, grp_run, s, "hipMemcpy(group table)");
+        const int32_t G = static_cast<int32_t>(grp_run.size()) - 1;
+        int32_t *cur = a.as<int32_t>();
+        int32_t *oth = b.as<int32_t>();
+        const dim3 grid(static_cast<unsigned>(ceil_div(total, kBlock)));
+        for (int p = 0; p < passes; ++p) {
+            K.merge_pass(grid, s, cur, oth, total, ctx->run_start.as<int64_t>(),
+                         ctx->grp_run.as<int32_t>(), G, p, total);
+            hip_check(hipGetLastError(), "k_merge_pass");
+            std::swap(cur, oth);
+        }
+        return cur;
     }
 
     // One layer expansion (expand_layer_frontiers, bottomup.cu:242-533):
     // frontier of every node of the target layer from its arcs' parents.
     // Passes cover runs of whole nodes with at most max_cand candidates (a
-    // single node may exceed it, as in the reference).
+    // single node may exceed it, as in the reference).  A node with at most CH
+    // candidates is filtered by one workgroup; a larger node first has its
+    // parent runs (each in key order) merged, then is cut into blocks of
+    // descending keys, each block filtered locally and checked against the
+    // survivors of the earlier blocks.  Every node's frontier leaves in
+    // descending key order.
     void expand(const DevFrontier &prev, const LayerCSR &csr, int64_t n_nodes, DevFrontier &next,
                 int64_t *cand_out, std::vector<int64_t> *gamma_out) {
         const int CH = K.chunk_cap;
-        // candidates per node and the node-local start of each arc's run
         std::vector<int64_t> run_off(std::max<int64_t>(csr.n_arcs, 1));
         std::vector<int64_t> gamma(n_nodes, 0);
         int64_t C = 0;
@@ -668,12 +736,13 @@ struct Run {
         src.w_stride = std::max<int64_t>(csr.n_arcs, 1);
         src.run_off = ctx->run_off.as<int64_t>();
 
+        ChunkBufs &cb = ctx->cb_main;
         int64_t v = 0;
         std::vector<ChunkDesc> chunks;
-        std::vector<int32_t> nalive;
-        std::vector<int64_t> chunk_out;
+        std::vector<int32_t> which_small, which_big, which_cross, nalive;
+        std::vector<int64_t> chunk_out, big_base, run_start;
+        std::vector<int32_t> big_node, grp_run;
         while (v < n_nodes) {
-            // plan one pass
             const int64_t v0 = v;
             int64_t pass_c = 0;
             while (v < n_nodes && (pass_c == 0 || pass_c + gamma[v] <= opts.max_cand)) {
@@ -682,23 +751,54 @@ struct Run {
             }
             const int64_t v1 = v;
             chunks.clear();
-            int64_t base = 0;
+            which_small.clear();
+            which_big.clear();
+            which_cross.clear();
+            big_base.clear();
+            big_node.clear();
+            run_start.clear();
+            grp_run.clear();
+            int64_t base = 0, bigpos = 0, max_runs = 1;
             for (int64_t u = v0; u < v1; ++u) {
                 const int64_t g = gamma[u];
                 if (g == 0) {
                     continue;
                 }
                 const int32_t first = static_cast<int32_t>(chunks.size());
-                const int32_t cnt = static_cast<int32_t>(ceil_div(g, CH));
-                for (int32_t k = 0; k < cnt; ++k) {
+                if (g <= CH) {
                     ChunkDesc c;
-                    c.start = static_cast<int64_t>(k) * CH;
-                    c.len = static_cast<int32_t>(std::min<int64_t>(CH, g - c.start));
-                    c.out_base = base + c.start;
+                    c.start = 0;
+                    c.len = static_cast<int32_t>(g);
+                    c.out_base = base;
                     c.group = static_cast<int32_t>(u);
                     c.group_first = first;
-                    c.group_count = cnt;
+                    c.group_count = 1;
+                    which_small.push_back(first);
                     chunks.push_back(c);
+                } else {
+                    big_node.push_back(static_cast<int32_t>(u));
+                    big_base.push_back(bigpos);
+                    grp_run.push_back(static_cast<int32_t>(run_start.size()));
+                    for (int64_t e = csr.off[u]; e < csr.off[u + 1]; ++e) {
+                        run_start.push_back(bigpos + run_off[e]);
+                    }
+                    max_runs = std::max<int64_t>(max_runs, csr.off[u + 1] - csr.off[u]);
+                    const int32_t cnt = static_cast<int32_t>(ceil_div(g, CH));
+                    for (int32_t k = 0; k < cnt; ++k) {
+                        ChunkDesc c;
+                        c.start = bigpos + static_cast<int64_t>(k) * CH;
+                        c.len = static_cast<int32_t>(std::min<int64_t>(CH, g - int64_t(k) * CH));
+                        c.out_base = base + static_cast<int64_t>(k) * CH;
+                        c.group = static_cast<int32_t>(u);
+                        c.group_first = first;
+                        c.group_count = cnt;
+                        which_big.push_back(static_cast<int32_t>(chunks.size()));
+                        if (k > 0) {
+                            which_cross.push_back(static_cast<int32_t>(chunks.size()));
+                        }
+                        chunks.push_back(c);
+                    }
+                    bigpos += g;
                 }
                 base += g;
             }
@@ -708,15 +808,43 @@ struct Run {
                 }
                 continue;
             }
-            filter_pass(
-                chunks, pass_c,
-                [&](dim3 grid) {
-                    K.filter_expand(grid, s, ctx->chunks.as<ChunkDesc>(), src,
-                                    ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>(),
-                                    ctx->nsurv.as<int32_t>(), ctx->nalive.as<int32_t>(),
-                                    ctx->minkey.as<int64_t>(), ctx->maxkey.as<int64_t>(), d_tests);
-                },
-                nalive);
+            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            int32_t *sorted = nullptr;
+            if (!big_node.empty()) {
+                grp_run.push_back(static_cast<int32_t>(run_start.size()));
+                run_start.push_back(bigpos);
+                big_base.push_back(bigpos);
+                ctx->sort_a.ensure(static_cast<size_t>(bigpos) * d * 4, "hipMalloc(sort buffer)");
+                ctx->sort_b.ensure(static_cast<size_t>(bigpos) * d * 4, "hipMalloc(sort buffer)");
+                h2d(ctx->big_base, big_base, s, "hipMemcpy(big nodes)");
+                h2d(ctx->big_node, big_node, s, "hipMemcpy(big nodes)");
+                K.gather_runs(dim3(static_cast<unsigned>(ceil_div(bigpos, kBlock))), s,
+                              ctx->big_base.as<int64_t>(), ctx->big_node.as<int32_t>(),
+                              static_cast<int32_t>(big_node.size()), src,
+                              ctx->sort_a.as<int32_t>(), bigpos);
+                hip_check(hipGetLastError(), "k_gather_runs");
+                sorted = merge_runs(ctx->sort_a, ctx->sort_b, bigpos, run_start, grp_run, max_runs);
+            }
+            hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
+            upload_table(cb, chunks);
+            ctx->surv_pts.ensure(static_cast<size_t>(pass_c) * d * 4, "hipMalloc(survivor buffer)");
+            ctx->alive.ensure(static_cast<size_t>(pass_c), "hipMalloc(alive flags)");
+            run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
+                       ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
+            pmdd_k::SortedSrc ss{sorted, bigpos};
+            run_filter(K.filter_sorted, cb, ctx->which_b, which_big, 0, &ss,
+                       ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
+            hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+            run_cross(K.cross_one_sided, cb, ctx->which_c, which_cross, ctx->surv_pts.as<int32_t>(),
+                      pass_c, ctx->alive.as<uint8_t>());
+            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+            d2h(nalive, cb.nalive, chunks.size(), s, "chunk survivor counts");
+            add_time(ctx->ev_a, ctx->ev_d, &st->kernel_expand_td_s);
+            add_time(ctx->ev_d, ctx->ev_c, &st->kernel_filter_s);
+            add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
+            st->filter_launches += 1;
+            st->filter_candidates += pass_c;
+
             // dense placement of the survivors, node by node
             chunk_out.resize(chunks.size());
             int64_t at = next_total;
@@ -730,7 +858,13 @@ struct Run {
                 next.h_off[u + 1] = at;
             }
             grow_points(next, d, at, next_total, s);
-            compact_pass(chunks, pass_c, chunk_out, next.pts.as<int32_t>(), next.cap);
+            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            run_compact(cb, static_cast<int64_t>(chunks.size()), chunk_out,
+                        ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>(),
+                        next.pts.as<int32_t>(), next.cap);
+            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+            hip_check(hipStreamSynchronize(s), "compaction sync");
+            add_time(ctx->ev_a, ctx->ev_b, &st->kernel_expand_td_s);
             next_total = at;
             sample_memory();
         }
@@ -740,6 +874,15 @@ struct Run {
                  2 * (n_nodes + 1) + (prev.n_nodes + 1));
         if (gamma_out != nullptr) {
             gamma_out->swap(gamma);
+        }
+    }
+
+    void add_time(hipEvent_t a, hipEvent_t b, double *acc) {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        *acc += ms * 1e-3;
+        if (acc != &st->kernel_filter_s) {
+            st->kernel_total_s += ms * 1e-3;
         }
     }
 
@@ -955,8 +1098,11 @@ struct Run {
     // Cutset coupling (couple_cutsets_cuda, couple.cu:340-498): the result is
     // the nondominated set of the union over cutset nodes of TD(v) + BU(v).
     // Nodes with both frontiers non-empty, largest |TD|+|BU| first, go in
-    // batches of at most max_prod products (a single node may exceed it).  Each
-    // batch is filtered on its own, then merged into the running frontier R.
+    // batches of at most max_prod products (a single node may exceed it).  Per
+    // batch: products filtered per chunk, then against the running frontier R
+    // (kept in descending key order), the rest merged into key order and
+    // filtered by blocks into the batch frontier B; R loses the points that B
+    // strictly dominates and B is merged into R.
     void couple(const DevFrontier &td, const DevFrontier &bu, pmdd_frontier *out) {
         const int CH = K.chunk_cap;
         const int64_t n = td.n_nodes;
@@ -974,13 +1120,16 @@ struct Run {
             return td.size(a) + bu.size(a) > td.size(b) + bu.size(b);
         });
 
-        DevFrontier R, Rn, B;
-        R.n_nodes = Rn.n_nodes = B.n_nodes = 1;
+        DevFrontier R, B;
         int64_t r_count = 0;
         size_t i = 0;
-        std::vector<ChunkDesc> chunks;
-        std::vector<int32_t> nalive;
-        std::vector<int64_t> chunk_out;
+        std::vector<ChunkDesc> chunks, rchunks;
+        std::vector<int32_t> nalive, which, rnalive;
+        std::vector<int64_t> outpos, run_start, routpos;
+        std::vector<int32_t> grp_run;
+        ChunkBufs &cb = ctx->cb_main;
+        ChunkBufs &cr = ctx->cb_aux;
+        const std::vector<int32_t> none;
         while (i < nodes.size()) {
             int64_t bprod = 0;
             size_t j = i;
@@ -999,13 +1148,16 @@ struct Run {
             int64_t *bu_begin = td_begin + Kn;
             int64_t *bu_size = bu_begin + Kn;
             prod_off[0] = 0;
+            int64_t batch_inputs = 0;
             for (int32_t k = 0; k < Kn; ++k) {
                 const int64_t v = nodes[i + k];
                 prod_off[k + 1] = prod_off[k] + td.size(v) * bu.size(v);
                 td_begin[k] = td.h_off[v];
                 bu_begin[k] = bu.h_off[v];
                 bu_size[k] = bu.size(v);
+                batch_inputs += td.size(v) + bu.size(v);
             }
+            i = j;
             h2d(ctx->join_meta, meta, s, "hipMemcpy(batch table)");
             pmdd_k::JoinSrc src;
             src.prod_off = ctx->join_meta.as<int64_t>();
@@ -1018,7 +1170,7 @@ struct Run {
             src.bu_pts = bu.pts.as<int32_t>();
             src.bu_stride = bu.cap;
 
-            // the whole batch is one group: chunks may straddle nodes
+            // 1. products, chunk by chunk (chunks may straddle nodes)
             chunks.clear();
             const int32_t cnt = static_cast<int32_t>(ceil_div(bprod, CH));
             for (int32_t k = 0; k < cnt; ++k) {
@@ -1031,118 +1183,168 @@ struct Run {
                 c.group_count = cnt;
                 chunks.push_back(c);
             }
-            filter_pass(
-                chunks, bprod,
-                [&](dim3 grid) {
-                    K.filter_join(grid, s, ctx->chunks.as<ChunkDesc>(), src,
-                                  ctx->surv_pts.as<int32_t>(), bprod, ctx->alive.as<uint8_t>(),
-                                  ctx->nsurv.as<int32_t>(), ctx->nalive.as<int32_t>(),
-                                  ctx->minkey.as<int64_t>(), ctx->maxkey.as<int64_t>(), d_tests);
-                },
-                nalive);
-            chunk_out.resize(chunks.size());
-            int64_t at = 0;
+            upload_table(cb, chunks);
+            ctx->surv_pts.ensure(static_cast<size_t>(bprod) * d * 4, "hipMalloc(survivor buffer)");
+            ctx->alive.ensure(static_cast<size_t>(bprod), "hipMalloc(alive flags)");
+            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            run_filter(K.filter_join, cb, ctx->which_a, none, cnt, &src,
+                       ctx->surv_pts.as<int32_t>(), bprod, ctx->alive.as<uint8_t>());
+            hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+            // 2. against the running frontier
+            if (r_count > 0) {
+                K.vs_weak(dim3(static_cast<unsigned>(cnt)), s, nullptr, cb.chunks.as<ChunkDesc>(),
+                          ctx->surv_pts.as<int32_t>(), bprod, ctx->alive.as<uint8_t>(),
+                          cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), R.pts.as<int32_t>(),
+                          R.cap, r_count, d_tests);
+                hip_check(hipGetLastError(), "k_vs_sorted");
+            }
+            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+            d2h(nalive, cb.nalive, chunks.size(), s, "product survivor counts");
+            add_time(ctx->ev_a, ctx->ev_c, &st->kernel_filter_s);
+            add_time(ctx->ev_a, ctx->ev_b, &st->kernel_dominance_s);
+            st->filter_launches += 1;
+            st->filter_candidates += bprod;
+            outpos.resize(chunks.size());
+            run_start.clear();
+            int64_t xtotal = 0;
             for (size_t c = 0; c < chunks.size(); ++c) {
-                chunk_out[c] = at;
-                at += nalive[c];
+                outpos[c] = xtotal;
+                run_start.push_back(xtotal);
+                xtotal += nalive[c];
             }
-            const int64_t b_count = at;
-            int64_t batch_inputs = 0;
-            for (int32_t k = 0; k < Kn; ++k) {
-                batch_inputs += td.size(nodes[i + k]) + bu.size(nodes[i + k]);
+            if (xtotal == 0) {
+                continue;
             }
-            st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + b_count);
-            grow_points(B, d, std::max<int64_t>(b_count, 1), 0, s);
-            compact_pass(chunks, bprod, chunk_out, B.pts.as<int32_t>(), B.cap);
-
+            run_start.push_back(xtotal);
+            grp_run.assign({0, cnt});
+            // 3. remaining points to a dense buffer, then 4. into key order
+            ctx->sort_a.ensure(static_cast<size_t>(xtotal) * d * 4, "hipMalloc(sort buffer)");
+            ctx->sort_b.ensure(static_cast<size_t>(xtotal) * d * 4, "hipMalloc(sort buffer)");
+            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            run_compact(cb, cnt, outpos, ctx->surv_pts.as<int32_t>(), bprod,
+                        ctx->alive.as<uint8_t>(), ctx->sort_a.as<int32_t>(), xtotal);
+            int32_t *xs = merge_runs(ctx->sort_a, ctx->sort_b, xtotal, run_start, grp_run, cnt);
+            hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
+            // 5. blocks of descending keys: local filter + earlier blocks
+            chunks.clear();
+            which.clear();
+            const int32_t nb = static_cast<int32_t>(ceil_div(xtotal, CH));
+            for (int32_t k = 0; k < nb; ++k) {
+                ChunkDesc c;
+                c.start = static_cast<int64_t>(k) * CH;
+                c.len = static_cast<int32_t>(std::min<int64_t>(CH, xtotal - c.start));
+                c.out_base = c.start;
+                c.group = 0;
+                c.group_first = 0;
+                c.group_count = nb;
+                chunks.push_back(c);
+                if (k > 0) {
+                    which.push_back(k);
+                }
+            }
+            upload_table(cb, chunks);
+            pmdd_k::SortedSrc ss{xs, xtotal};
+            run_filter(K.filter_sorted, cb, ctx->which_a, none, nb, &ss,
+                       ctx->surv_pts.as<int32_t>(), xtotal, ctx->alive.as<uint8_t>());
+            hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+            run_cross(K.cross_one_sided, cb, ctx->which_c, which, ctx->surv_pts.as<int32_t>(),
+                      xtotal, ctx->alive.as<uint8_t>());
+            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+            d2h(nalive, cb.nalive, chunks.size(), s, "batch survivor counts");
+            add_time(ctx->ev_a, ctx->ev_d, &st->kernel_expand_td_s);
+            add_time(ctx->ev_d, ctx->ev_c, &st->kernel_filter_s);
+            add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
+            st->filter_launches += 1;
+            st->filter_candidates += xtotal;
+            outpos.resize(chunks.size());
+            int64_t btotal = 0;
+            for (size_t c = 0; c < chunks.size(); ++c) {
+                outpos[c] = btotal;
+                btotal += nalive[c];
+            }
+            B.cap = 0;
+            B.pts.release();
+            grow_points(B, d, std::max<int64_t>(btotal, 1), 0, s);
+            run_compact(cb, nb, outpos, ctx->surv_pts.as<int32_t>(), xtotal,
+                        ctx->alive.as<uint8_t>(), B.pts.as<int32_t>(), B.cap);
+            st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + btotal);
             if (r_count == 0) {
-                std::swap(R.pts.p, B.pts.p);
-                std::swap(R.pts.bytes, B.pts.bytes);
+                hip_check(hipStreamSynchronize(s), "batch sync");
+                std::swap(R.pts, B.pts);
                 std::swap(R.cap, B.cap);
-                r_count = b_count;
-            } else {
-                r_count = merge_running(R, r_count, B, b_count, Rn);
+                r_count = btotal;
+                continue;
             }
+            // 6. R minus what B strictly dominates, then R and B merged by key
+            rchunks.clear();
+            const int32_t rn = static_cast<int32_t>(ceil_div(r_count, CH));
+            for (int32_t k = 0; k < rn; ++k) {
+                ChunkDesc c;
+                c.start = static_cast<int64_t>(k) * CH;
+                c.len = static_cast<int32_t>(std::min<int64_t>(CH, r_count - c.start));
+                c.out_base = c.start;
+                c.group = 0;
+                c.group_first = 0;
+                c.group_count = rn;
+                rchunks.push_back(c);
+            }
+            upload_table(cr, rchunks);
+            std::vector<int32_t> rlen(rn);
+            for (int32_t k = 0; k < rn; ++k) {
+                rlen[k] = rchunks[k].len;
+            }
+            h2d(cr.nsurv, rlen, s, "hipMemcpy(frontier chunk sizes)");
+            ctx->keep_b.ensure(static_cast<size_t>(r_count), "hipMalloc(frontier flags)");
+            hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(r_count, 256))),
+                               dim3(256), 0, s, ctx->keep_b.as<uint8_t>(), r_count, uint8_t(1));
+            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            K.vs_strict(dim3(static_cast<unsigned>(rn)), s, nullptr, cr.chunks.as<ChunkDesc>(),
+                        R.pts.as<int32_t>(), R.cap, ctx->keep_b.as<uint8_t>(),
+                        cr.nsurv.as<int32_t>(), cr.nalive.as<int32_t>(), B.pts.as<int32_t>(),
+                        B.cap, btotal, d_tests);
+            hip_check(hipGetLastError(), "k_vs_sorted");
+            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+            d2h(rnalive, cr.nalive, rchunks.size(), s, "frontier survivor counts");
+            add_time(ctx->ev_a, ctx->ev_b, &st->kernel_dominance_s);
+            routpos.resize(rn);
+            int64_t ralive = 0;
+            for (int32_t k = 0; k < rn; ++k) {
+                routpos[k] = ralive;
+                ralive += rnalive[k];
+            }
+            const int64_t total_new = ralive + btotal;
+            DevFrontier M, Rn;
+            grow_points(M, d, total_new, 0, s);
+            grow_points(Rn, d, total_new, 0, s);
+            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            run_compact(cr, rn, routpos, R.pts.as<int32_t>(), R.cap, ctx->keep_b.as<uint8_t>(),
+                        M.pts.as<int32_t>(), M.cap);
+            for (int o = 0; o < d && btotal > 0; ++o) {
+                hip_check(hipMemcpyAsync(M.pts.as<int32_t>() + o * M.cap + ralive,
+                                         B.pts.as<int32_t>() + o * B.cap,
+                                         static_cast<size_t>(btotal) * 4, hipMemcpyDeviceToDevice,
+                                         s),
+                          "hipMemcpy(batch frontier)");
+            }
+            // both buffers share the stride M.cap == Rn.cap for the merge pass
+            std::vector<int64_t> rs = {0, ralive, total_new};
+            std::vector<int32_t> gr = {0, 2};
+            int32_t *merged = merge_runs(M.pts, Rn.pts, total_new, rs, gr, 2);
+            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+            hip_check(hipStreamSynchronize(s), "frontier merge sync");
+            add_time(ctx->ev_a, ctx->ev_b, &st->kernel_expand_td_s);
+            if (merged == M.pts.as<int32_t>()) {
+                std::swap(R.pts, M.pts);
+                R.cap = M.cap;
+            } else {
+                std::swap(R.pts, Rn.pts);
+                R.cap = Rn.cap;
+            }
+            r_count = total_new;
             sample_memory();
-            i = j;
         }
         fetch_points(R, 0, r_count, out);
         count_layer(0, r_count);
-    }
-
-    // R <- (R minus points strictly dominated by B) ++ (B minus points weakly
-    // dominated by R).  Returns the new size.
-    int64_t merge_running(DevFrontier &R, int64_t nr, DevFrontier &B, int64_t nb, DevFrontier &Rn) {
-        ctx->keep_a.ensure(std::max<int64_t>(nb, 1), "hipMalloc(keep flags)");
-        ctx->keep_b.ensure(std::max<int64_t>(nr, 1), "hipMalloc(keep flags)");
-        const int64_t blk_b = ceil_div(nb, kBlock);
-        const int64_t blk_r = ceil_div(nr, kBlock);
-        hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-        K.vs_frontier(dim3(static_cast<unsigned>(blk_b + blk_r)), s, B.pts.as<int32_t>(), B.cap,
-                      nb, R.pts.as<int32_t>(), R.cap, nr, ctx->keep_a.as<uint8_t>(),
-                      ctx->keep_b.as<uint8_t>(), blk_b, d_tests);
-        hip_check(hipGetLastError(), "k_vs_frontier");
-        hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-        std::vector<uint8_t> kb, kr;
-        d2h(kb, ctx->keep_a, nb, s, "keep flags");
-        d2h(kr, ctx->keep_b, nr, s, "keep flags");
-        const double ms = elapsed_ms();
-        st->kernel_dominance_s += ms * 1e-3;
-        st->kernel_total_s += ms * 1e-3;
-        // compact on the device with per-point chunk tables of one chunk per
-        // 2048 points: reuse k_compact_chunks
-        const int64_t CHC = 2048;
-        auto compact_into = [&](const DevFrontier &srcf, int64_t n, const std::vector<uint8_t> &keep,
-                                DevBuf &flags, int64_t out_at) -> int64_t {
-            std::vector<ChunkDesc> cs;
-            std::vector<int32_t> ns;
-            std::vector<int64_t> outs;
-            int64_t at = out_at;
-            for (int64_t b0 = 0; b0 < n; b0 += CHC) {
-                ChunkDesc c;
-                c.start = b0;
-                c.out_base = b0;
-                c.len = static_cast<int32_t>(std::min<int64_t>(CHC, n - b0));
-                c.group = 0;
-                c.group_first = 0;
-                c.group_count = 1;
-                cs.push_back(c);
-                ns.push_back(c.len);
-                outs.push_back(at);
-                for (int64_t k = b0; k < b0 + c.len; ++k) {
-                    at += keep[k];
-                }
-            }
-            if (cs.empty()) {
-                return at;
-            }
-            h2d(ctx->chunks, cs, s, "hipMemcpy(chunk table)");
-            h2d(ctx->nsurv, ns, s, "hipMemcpy(chunk counts)");
-            h2d(ctx->chunk_out, outs, s, "hipMemcpy(chunk outputs)");
-            K.compact(dim3(static_cast<unsigned>(cs.size())), s, ctx->chunks.as<ChunkDesc>(),
-                      ctx->nsurv.as<int32_t>(), ctx->chunk_out.as<int64_t>(),
-                      srcf.pts.as<int32_t>(), srcf.cap, flags.as<uint8_t>(), Rn.pts.as<int32_t>(),
-                      Rn.cap);
-            hip_check(hipGetLastError(), "k_compact_chunks");
-            hip_check(hipStreamSynchronize(s), "merge compaction sync");
-            return at;
-        };
-        int64_t kept_r = 0, kept_b = 0;
-        for (uint8_t x : kr) {
-            kept_r += x;
-        }
-        for (uint8_t x : kb) {
-            kept_b += x;
-        }
-        Rn.cap = 0;
-        Rn.pts.release();
-        grow_points(Rn, d, std::max<int64_t>(kept_r + kept_b, 1), 0, s);
-        int64_t at = compact_into(R, nr, kr, ctx->keep_b, 0);
-        at = compact_into(B, nb, kb, ctx->keep_a, at);
-        std::swap(R.pts.p, Rn.pts.p);
-        std::swap(R.pts.bytes, Rn.pts.bytes);
-        std::swap(R.cap, Rn.cap);
-        return at;
     }
 
     void finish() {
@@ -1269,6 +1471,7 @@ int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
         hip_check(hipEventCreate(&c->ev_a), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_c), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev_d), "hipEventCreate");
         *out = c.release();
         return PMDD_OK;
     });
@@ -1285,20 +1488,20 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
     {
         // release scratch before the stream
         pmdd_ctx *c = ctx;
-        c->chunks.release();
-        c->run_off.release();
-        c->nsurv.release();
-        c->nalive.release();
-        c->minkey.release();
-        c->maxkey.release();
-        c->chunk_out.release();
-        c->which.release();
-        c->surv_pts.release();
-        c->alive.release();
-        c->counter.release();
-        c->join_meta.release();
-        c->keep_a.release();
-        c->keep_b.release();
+        for (ChunkBufs *cb : {&c->cb_main, &c->cb_aux}) {
+            cb->chunks.release();
+            cb->nsurv.release();
+            cb->nalive.release();
+            cb->minkey.release();
+            cb->maxkey.release();
+            cb->out.release();
+        }
+        for (DevBuf *b : {&c->run_off, &c->which, &c->which_a, &c->which_b, &c->which_c,
+                          &c->surv_pts, &c->alive, &c->counter, &c->join_meta, &c->keep_a,
+                          &c->keep_b, &c->sort_a, &c->sort_b, &c->big_base, &c->big_node,
+                          &c->run_start, &c->grp_run}) {
+            b->release();
+        }
     }
     if (ctx->ev_a != nullptr) {
         (void)hipEventDestroy(ctx->ev_a);
@@ -1308,6 +1511,9 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
     }
     if (ctx->ev_c != nullptr) {
         (void)hipEventDestroy(ctx->ev_c);
+    }
+    if (ctx->ev_d != nullptr) {
+        (void)hipEventDestroy(ctx->ev_d);
     }
     if (ctx->stream != nullptr) {
         (void)hipStreamDestroy(ctx->stream);

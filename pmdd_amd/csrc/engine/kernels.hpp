@@ -138,6 +138,22 @@ __device__ __forceinline__ void load_candidate(const JoinSrc &s, const ChunkDesc
     }
 }
 
+// Candidates already materialised in key order (SoA); c.start is the
+// position of the chunk's first candidate in that buffer.
+struct SortedSrc {
+    const int32_t *pts;
+    int64_t stride;
+};
+
+template <int D>
+__device__ __forceinline__ void load_candidate(const SortedSrc &s, const ChunkDesc &c, int64_t t,
+                                               int32_t (&p)[D]) {
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        p[o] = s.pts[o * s.stride + c.start + t];
+    }
+}
+
 // Sort word: key (sum of objectives, |key| < 2^35) biased to unsigned in the
 // high 48 bits, (0xFFFF - slot) in the low 16, so that descending order of
 // the word is key descending, then slot ascending.
@@ -173,6 +189,17 @@ __device__ __forceinline__ int block_rank(bool flag, int *wave_tot, int *total) 
     return before + in_wave;
 }
 
+// Key (sum of objectives) of point i of a SoA buffer.
+template <int D>
+__device__ __forceinline__ int64_t key_at(const int32_t *pts, int64_t stride, int64_t i) {
+    int64_t k = 0;
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        k += pts[o * stride + i];
+    }
+    return k;
+}
+
 // row >= m in every objective; rows are 16-byte aligned, padded to DP ints.
 template <int D, int DP>
 __device__ __forceinline__ bool ge_all(const int32_t *row, const int32_t (&m)[D]) {
@@ -194,9 +221,10 @@ __device__ __forceinline__ bool ge_all(const int32_t *row, const int32_t (&m)[D]
 // ---------------------------------------------------------------------------
 // k_filter_chunks: per-chunk load, presort, sort-filter skyline.
 // ---------------------------------------------------------------------------
-template <int D, int CH, class Src>
+template <int D, int CH, class Src, bool PRESORTED>
 __global__ __launch_bounds__(kBlock) void k_filter_chunks(
-    const ChunkDesc *__restrict__ chunks, Src src, int32_t *__restrict__ out_pts,
+    const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks, Src src,
+    int32_t *__restrict__ out_pts,
     int64_t out_stride, uint8_t *__restrict__ alive, int32_t *__restrict__ chunk_nsurv,
     int32_t *__restrict__ chunk_nalive, int64_t *__restrict__ chunk_minkey,
     int64_t *__restrict__ chunk_maxkey, unsigned long long *__restrict__ pair_tests) {
@@ -206,7 +234,8 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
     __shared__ __align__(16) int32_t S[CH * DP];
     __shared__ int wave_tot[kWaves];
 
-    const ChunkDesc c = chunks[blockIdx.x];
+    const int ci = which != nullptr ? which[blockIdx.x] : static_cast<int>(blockIdx.x);
+    const ChunkDesc c = chunks[ci];
     const int n = c.len;
     const int tid = threadIdx.x;
 
@@ -231,8 +260,8 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
     }
     __syncthreads();
 
-    // bitonic sort, descending
-    for (int k = 2; k <= npad; k <<= 1) {
+    // bitonic sort, descending (skipped when the source is in key order)
+    for (int k = 2; !PRESORTED && k <= npad; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int t = tid; t < (npad >> 1); t += kBlock) {
                 const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
@@ -308,16 +337,16 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
         alive[pos] = 1;
     }
     if (tid == 0) {
-        chunk_nsurv[blockIdx.x] = nS;
-        chunk_nalive[blockIdx.x] = nS;
+        chunk_nsurv[ci] = nS;
+        chunk_nalive[ci] = nS;
         int64_t kmax = 0, kmin = 0;
 #pragma unroll
         for (int o = 0; o < D; ++o) {
             kmax += S[o];
             kmin += S[(nS - 1) * DP + o];
         }
-        chunk_maxkey[blockIdx.x] = kmax;
-        chunk_minkey[blockIdx.x] = kmin;
+        chunk_maxkey[ci] = kmax;
+        chunk_minkey[ci] = kmin;
     }
     if (pair_tests != nullptr) {
         atomicAdd(pair_tests, tests);
@@ -328,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
 // k_cross_chunks: symmetric check of a chunk's survivors against the other
 // chunks of its group.  One workgroup per chunk listed in `which`.
 // ---------------------------------------------------------------------------
-template <int D, int CH>
+template <int D, int CH, bool ONE_SIDED>
 __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
     const int32_t *__restrict__ pts, int64_t stride, uint8_t *__restrict__ alive,
@@ -366,7 +395,10 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     }
     unsigned long long tests = 0;
 
-    for (int g = 0; g < c.group_count; ++g) {
+    // ONE_SIDED: the group's chunks are consecutive key ranges in descending
+    // order, so only earlier chunks can dominate (and equal points there win).
+    const int g_end = ONE_SIDED ? (me - c.group_first) : c.group_count;
+    for (int g = 0; g < g_end; ++g) {
         const int other = c.group_first + g;
         if (other == me) {
             continue;
@@ -503,6 +535,11 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
             m[o] = valid ? pts[o * stride + b0 + i] : 0;
         }
         bool dead = !valid;
+        // frontiers are in descending key order: the last point of this block
+        // has the smallest key, and a comparison label can only be >= a point
+        // whose key does not exceed its own
+        const int64_t last = min(base + kBlock, n) - 1;
+        const int64_t block_min = key_at<D>(pts, stride, b0 + last);
         for (int cidx = 1; cidx <= 2; ++cidx) {
             const int cmp = targets[3 * blockIdx.x + cidx];
             if (cmp < 0) {
@@ -511,8 +548,13 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
             const int64_t c0 = off[cmp];
             const int64_t cn = off[cmp + 1] - c0;
             for (int64_t tb = 0; tb < cn; tb += kBlock) {
+                if (key_at<D>(pts, stride, c0 + tb) < block_min) {
+                    break;
+                }
+                if (!__syncthreads_or(!dead)) {
+                    break;
+                }
                 const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), cn - tb));
-                __syncthreads();
                 if (static_cast<int>(threadIdx.x) < tn) {
 #pragma unroll
                     for (int o = 0; o < D; ++o) {
@@ -669,6 +711,228 @@ __global__ __launch_bounds__(kBlock) void k_vs_frontier(
     }
     if (valid) {
         (b_role ? keep_b : keep_r)[i] = dead ? 0 : 1;
+    }
+    if (pair_tests != nullptr) {
+        atomicAdd(pair_tests, tests);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Big groups: runs in key order merged pairwise, one pass per level.
+//
+// Elements of several groups lie in one buffer; group g owns runs
+// [grp_run[g], grp_run[g+1]) and run r occupies [run_start[r], run_start[r+1]).
+// In pass p the runs of a group are taken in segments of 2^p runs; segment s is
+// merged with segment s^1 (stable: on equal keys the lower segment first).
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_gather_runs(
+    const int64_t *__restrict__ big_base, // [G + 1] element offsets of the groups
+    const int32_t *__restrict__ big_node, // [G] node of each group
+    int32_t G, ExpandSrc src, int32_t *__restrict__ out, int64_t out_stride) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= big_base[G]) {
+        return;
+    }
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (big_base[mid] <= q) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    ChunkDesc c;
+    c.start = 0;
+    c.group = big_node[lo];
+    int32_t p[D];
+    load_candidate<D>(src, c, q - big_base[lo], p);
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        out[o * out_stride + q] = p[o];
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_merge_pass(
+    const int32_t *__restrict__ in, int32_t *__restrict__ out, int64_t stride,
+    const int64_t *__restrict__ run_start, const int32_t *__restrict__ grp_run, int32_t G,
+    int pass, int64_t total) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= total) {
+        return;
+    }
+    // group of q: last g with run_start[grp_run[g]] <= q
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (run_start[grp_run[mid]] <= q) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    const int64_t gr0 = grp_run[lo];
+    const int64_t nr = grp_run[lo + 1] - gr0;
+    // run of q: last r in [gr0, gr0 + nr) with run_start[r] <= q
+    int64_t rl = gr0, rh = gr0 + nr - 1;
+    while (rl < rh) {
+        const int64_t mid = (rl + rh + 1) >> 1;
+        if (run_start[mid] <= q) {
+            rl = mid;
+        } else {
+            rh = mid - 1;
+        }
+    }
+    const int64_t seg = (rl - gr0) >> pass;
+    const int64_t partner = seg ^ 1;
+    auto run_pos = [&](int64_t r) { return run_start[gr0 + (r < nr ? r : nr)]; };
+    const int64_t a0 = run_pos(seg << pass);
+    const int64_t b0 = run_pos(partner << pass);
+    const int64_t b1 = run_pos((partner + 1) << pass);
+    int32_t v[D];
+    int64_t key = 0;
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        v[o] = in[o * stride + q];
+        key += v[o];
+    }
+    int64_t rank = q - a0;
+    if (b1 > b0) {
+        // partner elements placed before q: keys > key (q in the lower
+        // segment) or keys >= key (q in the upper segment)
+        const bool lower = (seg & 1) == 0;
+        int64_t l = b0, h = b1;
+        while (l < h) {
+            const int64_t mid = (l + h) >> 1;
+            const int64_t km = key_at<D>(in, stride, mid);
+            const bool before = lower ? (km > key) : (km >= key);
+            if (before) {
+                l = mid + 1;
+            } else {
+                h = mid;
+            }
+        }
+        rank += l - b0;
+    }
+    const int64_t dst = (a0 < b0 ? a0 : b0) + rank;
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        out[o * stride + dst] = v[o];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_vs_sorted: kill chunk survivors dominated by a set B in key order.
+// STRICT=false: x dies when some b >= x everywhere (equal points die).
+// STRICT=true:  x dies when some b >= x everywhere and b != x.
+// B is scanned from its largest key down and the scan stops at the first tile
+// whose largest key is too small to dominate any live point of the chunk.
+// ---------------------------------------------------------------------------
+template <int D, int CH, bool STRICT>
+__global__ __launch_bounds__(kBlock) void k_vs_sorted(
+    const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
+    const int32_t *__restrict__ pts, int64_t stride, uint8_t *__restrict__ alive,
+    const int32_t *__restrict__ chunk_nsurv, int32_t *__restrict__ chunk_nalive,
+    const int32_t *__restrict__ B, int64_t bstride, int64_t nb,
+    unsigned long long *__restrict__ pair_tests) {
+    constexpr int DP = Pad<D>::DP;
+    constexpr int R = CH / kBlock;
+    __shared__ __align__(16) int32_t T[kBlock * DP];
+    __shared__ unsigned long long min_word;
+    __shared__ int live_count;
+    const int me = which != nullptr ? which[blockIdx.x] : static_cast<int>(blockIdx.x);
+    const ChunkDesc c = chunks[me];
+    const int nS = chunk_nsurv[me];
+    const int tid = threadIdx.x;
+    int32_t m[R][D];
+    bool dead[R];
+    int64_t my_min = INT64_MAX;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int s = r * kBlock + tid;
+        dead[r] = s >= nS || alive[c.out_base + s] == 0;
+        int64_t k = 0;
+#pragma unroll
+        for (int o = 0; o < D; ++o) {
+            m[r][o] = dead[r] ? 0 : pts[o * stride + c.out_base + s];
+            k += m[r][o];
+        }
+        if (!dead[r] && k < my_min) {
+            my_min = k;
+        }
+    }
+    if (tid == 0) {
+        min_word = ~0ULL;
+    }
+    __syncthreads();
+    if (my_min != INT64_MAX) {
+        atomicMin(&min_word, static_cast<unsigned long long>(my_min + (int64_t(1) << 62)));
+    }
+    __syncthreads();
+    const bool any = min_word != ~0ULL;
+    const int64_t wg_min = any ? static_cast<int64_t>(min_word) - (int64_t(1) << 62) : 0;
+    unsigned long long tests = 0;
+    for (int64_t tb = 0; any && tb < nb; tb += kBlock) {
+        const int64_t top = key_at<D>(B, bstride, tb);
+        if (STRICT ? (top <= wg_min) : (top < wg_min)) {
+            break;
+        }
+        const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), nb - tb));
+        if (tid < tn) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                T[tid * DP + o] = B[o * bstride + tb + tid];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            for (int j = 0; j < tn && !dead[r]; ++j) {
+                const int32_t *q = &T[j * DP];
+                bool ge = true;
+                bool eq = true;
+#pragma unroll
+                for (int o = 0; o < D; ++o) {
+                    ge &= q[o] >= m[r][o];
+                    eq &= q[o] == m[r][o];
+                }
+                dead[r] = STRICT ? (ge && !eq) : ge;
+                if (pair_tests != nullptr) {
+                    ++tests;
+                }
+            }
+        }
+        bool any_live = false;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            any_live |= !dead[r];
+        }
+        if (!__syncthreads_or(any_live)) {
+            break;
+        }
+    }
+    int kept = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int s = r * kBlock + tid;
+        if (s < nS) {
+            if (dead[r]) {
+                alive[c.out_base + s] = 0;
+            } else {
+                ++kept;
+            }
+        }
+    }
+    if (tid == 0) {
+        live_count = 0;
+    }
+    __syncthreads();
+    atomicAdd(&live_count, kept);
+    __syncthreads();
+    if (tid == 0) {
+        chunk_nalive[me] = live_count;
     }
     if (pair_tests != nullptr) {
         atomicAdd(pair_tests, tests);
