@@ -116,6 +116,7 @@ struct pmdd_ctx {
     DevBuf run_off, which, which_a, which_b, which_c;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
+    DevBuf join_idx;
 };
 
 struct LayerCSR {
@@ -438,8 +439,7 @@ struct Launch {
     FilterFn filter_expand;  // ExpandSrc, presorted in LDS
     FilterFn filter_join;    // JoinSrc, presorted in LDS
     FilterFn filter_sorted;  // SortedSrc, source already in key order
-    CrossFn cross_sym;       // unordered chunks of a group
-    CrossFn cross_one_sided; // chunks are descending key ranges
+    CrossFn cross;           // chunks of a group are descending key blocks
     VsSortedFn vs_weak;
     VsSortedFn vs_strict;
     void (*compact)(dim3, hipStream_t, const ChunkDesc *, const int32_t *, const int64_t *,
@@ -452,7 +452,13 @@ struct Launch {
                         pmdd_k::ExpandSrc, int32_t *, int64_t);
     void (*merge_pass)(dim3, hipStream_t, const int32_t *, int32_t *, int64_t, const int64_t *,
                        const int32_t *, int32_t, int, int64_t);
+    void (*dense_vs)(dim3, hipStream_t, const int32_t *, int64_t, int64_t, uint8_t *, int32_t *,
+                     const int32_t *, int64_t, int64_t, unsigned long long *);
 };
+
+// points per lane of k_dense_vs_sorted; a slice is 64 * kDenseP points
+constexpr int kDenseP = 4;
+constexpr int kDenseSlice = 64 * kDenseP;
 
 template <int D> struct LaunchImpl {
     static constexpr int CH = pmdd_k::Cap<D>::CH;
@@ -464,13 +470,11 @@ template <int D> struct LaunchImpl {
                            which, c, *static_cast<const Src *>(src), out, stride, alive, ns, na,
                            kmin, kmax, pt);
     }
-    template <bool ONE>
     static void cross(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
-                      int32_t *na, const int64_t *kmin, const int64_t *kmax,
-                      unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_cross_chunks<D, CH, ONE>), g, dim3(kBlock), 0, s, which, c,
-                           pts, stride, alive, ns, na, kmin, kmax, pt);
+                      int32_t *na, const int64_t *, const int64_t *, unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_cross_chunks<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
+                           stride, alive, ns, na, pt);
     }
     template <bool STRICT>
     static void vs(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
@@ -508,14 +512,20 @@ template <int D> struct LaunchImpl {
         hipLaunchKernelGGL((pmdd_k::k_merge_pass<D>), g, dim3(kBlock), 0, s, in, out, stride,
                            run_start, grp_run, G, pass, total);
     }
+    static void dense_vs(dim3 g, hipStream_t s, const int32_t *X, int64_t xs, int64_t n,
+                         uint8_t *alive, int32_t *sa, const int32_t *R, int64_t rs, int64_t nr,
+                         unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_dense_vs_sorted<D, kDenseP>), g, dim3(pmdd_k::kWave), 0, s,
+                           X, xs, n, alive, sa, R, rs, nr, pt);
+    }
     static Launch table() {
         Launch L;
+        L.dense_vs = dense_vs;
         L.chunk_cap = CH;
         L.filter_expand = filter<pmdd_k::ExpandSrc, false>;
         L.filter_join = filter<pmdd_k::JoinSrc, false>;
         L.filter_sorted = filter<pmdd_k::SortedSrc, true>;
-        L.cross_sym = cross<false>;
-        L.cross_one_sided = cross<true>;
+        L.cross = cross;
         L.vs_weak = vs<false>;
         L.vs_strict = vs<true>;
         L.compact = compact;
@@ -579,6 +589,8 @@ struct Run {
     hipStream_t s;
     int64_t mem_base = 0;
     unsigned long long *d_tests = nullptr;
+    // PMDD_DEBUG_JOIN=1 prints per-batch coupling counts to stderr
+    const bool debug_join = std::getenv("PMDD_DEBUG_JOIN") != nullptr;
 
     Run(pmdd_ctx *c, pmdd_resident *r, const pmdd_opts &o, pmdd_stats *stats)
         : ctx(c), dd(r), opts(o), st(stats), K(launch_for(r->d)), d(r->d), s(c->stream) {
@@ -644,16 +656,44 @@ struct Run {
         hip_check(hipGetLastError(), "k_filter_chunks");
     }
 
-    void run_cross(CrossFn f, ChunkBufs &cb, DevBuf &wbuf, const std::vector<int32_t> &which,
-                   const int32_t *pts, int64_t stride, uint8_t *alive) {
-        if (which.empty()) {
+    // One-sided block filter over every group with more than one chunk (the
+    // chunks of a group are descending-key blocks), launched in waves of
+    // kWave chunks per group so that earlier chunks are final when read.
+    static constexpr int kCrossWave = 8;
+    void run_cross_waves(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks, const int32_t *pts,
+                         int64_t stride, uint8_t *alive) {
+        int32_t max_count = 1;
+        for (const ChunkDesc &c : chunks) {
+            max_count = std::max(max_count, c.group_count);
+        }
+        if (max_count <= 1) {
             return;
         }
-        h2d(wbuf, which, s, "hipMemcpy(cross list)");
-        f(dim3(static_cast<unsigned>(which.size())), s, wbuf.as<int32_t>(),
-          cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(),
-          cb.nalive.as<int32_t>(), cb.minkey.as<int64_t>(), cb.maxkey.as<int64_t>(), d_tests);
-        hip_check(hipGetLastError(), "k_cross_chunks");
+        const int waves = static_cast<int>(ceil_div(max_count, kCrossWave));
+        std::vector<int32_t> lists;
+        std::vector<size_t> start(waves + 1, 0);
+        for (int w = 0; w < waves; ++w) {
+            start[w] = lists.size();
+            for (size_t i = 0; i < chunks.size(); ++i) {
+                const ChunkDesc &c = chunks[i];
+                const int k = static_cast<int>(i) - c.group_first;
+                if (c.group_count > 1 && k > 0 && k / kCrossWave == w) {
+                    lists.push_back(static_cast<int32_t>(i));
+                }
+            }
+        }
+        start[waves] = lists.size();
+        h2d(ctx->which_c, lists, s, "hipMemcpy(wave lists)");
+        for (int w = 0; w < waves; ++w) {
+            const size_t n = start[w + 1] - start[w];
+            if (n == 0) {
+                continue;
+            }
+            K.cross(dim3(static_cast<unsigned>(n)), s, ctx->which_c.as<int32_t>() + start[w],
+                    cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(),
+                    cb.nalive.as<int32_t>(), nullptr, nullptr, d_tests);
+            hip_check(hipGetLastError(), "k_cross_chunks");
+        }
     }
 
     void run_compact(ChunkBufs &cb, int64_t nch, const std::vector<int64_t> &outpos,
@@ -835,8 +875,10 @@ struct Run {
             run_filter(K.filter_sorted, cb, ctx->which_b, which_big, 0, &ss,
                        ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-            run_cross(K.cross_one_sided, cb, ctx->which_c, which_cross, ctx->surv_pts.as<int32_t>(),
-                      pass_c, ctx->alive.as<uint8_t>());
+            if (!which_cross.empty()) {
+                run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), pass_c,
+                                ctx->alive.as<uint8_t>());
+            }
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
             d2h(nalive, cb.nalive, chunks.size(), s, "chunk survivor counts");
             add_time(ctx->ev_a, ctx->ev_d, &st->kernel_expand_td_s);
@@ -922,9 +964,13 @@ struct Run {
             if (k == 0) {
                 continue;
             }
-            targets.push_back(static_cast<int32_t>(n1));
-            targets.push_back(cmp[0]);
-            targets.push_back(cmp[1]);
+            // one workgroup per block of 256 labels of the target
+            for (int64_t b = 0; b < f.size(n1); b += kBlock) {
+                targets.push_back(static_cast<int32_t>(n1));
+                targets.push_back(cmp[0]);
+                targets.push_back(cmp[1]);
+                targets.push_back(static_cast<int32_t>(b));
+            }
         }
         if (targets.empty()) {
             return;
@@ -935,7 +981,7 @@ struct Run {
         hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(total, 256))),
                            dim3(256), 0, s, ctx->keep_a.as<uint8_t>(), total, uint8_t(1));
         h2d(ctx->which, targets, s, "hipMemcpy(dominance targets)");
-        K.state_dom(dim3(static_cast<unsigned>(targets.size() / 3)), s, ctx->which.as<int32_t>(),
+        K.state_dom(dim3(static_cast<unsigned>(targets.size() / 4)), s, ctx->which.as<int32_t>(),
                     f.off.as<int64_t>(), f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>());
         hip_check(hipGetLastError(), "k_state_dominance");
         // kept per node
@@ -1099,10 +1145,11 @@ struct Run {
     // the nondominated set of the union over cutset nodes of TD(v) + BU(v).
     // Nodes with both frontiers non-empty, largest |TD|+|BU| first, go in
     // batches of at most max_prod products (a single node may exceed it).  Per
-    // batch: products filtered per chunk, then against the running frontier R
-    // (kept in descending key order), the rest merged into key order and
-    // filtered by blocks into the batch frontier B; R loses the points that B
-    // strictly dominates and B is merged into R.
+    // batch: products filtered chunk by chunk, the survivors merged into key
+    // order, then in descending-key blocks filtered locally, against the
+    // running frontier R (kept in key order) and against earlier blocks,
+    // giving the batch frontier B; R loses the points B strictly dominates
+    // and B is merged into R.
     void couple(const DevFrontier &td, const DevFrontier &bu, pmdd_frontier *out) {
         const int CH = K.chunk_cap;
         const int64_t n = td.n_nodes;
@@ -1124,46 +1171,60 @@ struct Run {
         int64_t r_count = 0;
         size_t i = 0;
         std::vector<ChunkDesc> chunks, rchunks;
-        std::vector<int32_t> nalive, which, rnalive;
+        std::vector<int32_t> nalive, rnalive;
         std::vector<int64_t> outpos, run_start, routpos;
         std::vector<int32_t> grp_run;
         ChunkBufs &cb = ctx->cb_main;
         ChunkBufs &cr = ctx->cb_aux;
         const std::vector<int32_t> none;
+        // batches start small and double up to max_prod: the first batches
+        // build R cheaply, later ones are mostly filtered by it
+        int64_t batch_cap = std::min<int64_t>(opts.max_prod, int64_t(1) << 15);
         while (i < nodes.size()) {
             int64_t bprod = 0;
             size_t j = i;
             while (j < nodes.size()) {
                 const int64_t p = td.size(nodes[j]) * bu.size(nodes[j]);
-                if (bprod + p > opts.max_prod && bprod > 0) {
+                if (bprod + p > batch_cap && bprod > 0) {
                     break;
                 }
                 bprod += p;
                 ++j;
             }
+            batch_cap = std::min<int64_t>(opts.max_prod, batch_cap * 2);
             const int32_t Kn = static_cast<int32_t>(j - i);
             std::vector<int64_t> meta(4 * static_cast<size_t>(Kn) + 1);
+            std::vector<int64_t> idx;
             int64_t *prod_off = meta.data();
-            int64_t *td_begin = meta.data() + Kn + 1;
-            int64_t *bu_begin = td_begin + Kn;
-            int64_t *bu_size = bu_begin + Kn;
+            int64_t *row_base = meta.data() + Kn + 1;
+            int64_t *col_base = row_base + Kn;
+            int64_t *col_size = col_base + Kn;
             prod_off[0] = 0;
             int64_t batch_inputs = 0;
             for (int32_t k = 0; k < Kn; ++k) {
                 const int64_t v = nodes[i + k];
+                row_base[k] = static_cast<int64_t>(idx.size());
+                for (int64_t a = 0; a < td.size(v); ++a) {
+                    idx.push_back(td.h_off[v] + a);
+                }
+                col_base[k] = static_cast<int64_t>(idx.size());
+                for (int64_t b = 0; b < bu.size(v); ++b) {
+                    idx.push_back(bu.h_off[v] + b);
+                }
+                col_size[k] = bu.size(v);
                 prod_off[k + 1] = prod_off[k] + td.size(v) * bu.size(v);
-                td_begin[k] = td.h_off[v];
-                bu_begin[k] = bu.h_off[v];
-                bu_size[k] = bu.size(v);
                 batch_inputs += td.size(v) + bu.size(v);
             }
             i = j;
             h2d(ctx->join_meta, meta, s, "hipMemcpy(batch table)");
+            h2d(ctx->join_idx, idx, s, "hipMemcpy(batch rows)");
             pmdd_k::JoinSrc src;
             src.prod_off = ctx->join_meta.as<int64_t>();
-            src.td_begin = src.prod_off + Kn + 1;
-            src.bu_begin = src.td_begin + Kn;
-            src.bu_size = src.bu_begin + Kn;
+            src.row_base = src.prod_off + Kn + 1;
+            src.col_base = src.row_base + Kn;
+            src.col_size = src.col_base + Kn;
+            src.row_idx = ctx->join_idx.as<int64_t>();
+            src.col_idx = src.row_idx;
             src.K = Kn;
             src.td_pts = td.pts.as<int32_t>();
             src.td_stride = td.cap;
@@ -1180,7 +1241,7 @@ struct Run {
                 c.out_base = c.start;
                 c.group = 0;
                 c.group_first = 0;
-                c.group_count = cnt;
+                c.group_count = 1;
                 chunks.push_back(c);
             }
             upload_table(cb, chunks);
@@ -1190,18 +1251,9 @@ struct Run {
             run_filter(K.filter_join, cb, ctx->which_a, none, cnt, &src,
                        ctx->surv_pts.as<int32_t>(), bprod, ctx->alive.as<uint8_t>());
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-            // 2. against the running frontier
-            if (r_count > 0) {
-                K.vs_weak(dim3(static_cast<unsigned>(cnt)), s, nullptr, cb.chunks.as<ChunkDesc>(),
-                          ctx->surv_pts.as<int32_t>(), bprod, ctx->alive.as<uint8_t>(),
-                          cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), R.pts.as<int32_t>(),
-                          R.cap, r_count, d_tests);
-                hip_check(hipGetLastError(), "k_vs_sorted");
-            }
-            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
             d2h(nalive, cb.nalive, chunks.size(), s, "product survivor counts");
             add_time(ctx->ev_a, ctx->ev_c, &st->kernel_filter_s);
-            add_time(ctx->ev_a, ctx->ev_b, &st->kernel_dominance_s);
+            add_time(ctx->ev_a, ctx->ev_c, &st->kernel_dominance_s);
             st->filter_launches += 1;
             st->filter_candidates += bprod;
             outpos.resize(chunks.size());
@@ -1212,12 +1264,9 @@ struct Run {
                 run_start.push_back(xtotal);
                 xtotal += nalive[c];
             }
-            if (xtotal == 0) {
-                continue;
-            }
             run_start.push_back(xtotal);
             grp_run.assign({0, cnt});
-            // 3. remaining points to a dense buffer, then 4. into key order
+            // 2. survivors to a dense buffer and into key order
             ctx->sort_a.ensure(static_cast<size_t>(xtotal) * d * 4, "hipMalloc(sort buffer)");
             ctx->sort_b.ensure(static_cast<size_t>(xtotal) * d * 4, "hipMalloc(sort buffer)");
             hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
@@ -1225,49 +1274,107 @@ struct Run {
                         ctx->alive.as<uint8_t>(), ctx->sort_a.as<int32_t>(), xtotal);
             int32_t *xs = merge_runs(ctx->sort_a, ctx->sort_b, xtotal, run_start, grp_run, cnt);
             hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
-            // 5. blocks of descending keys: local filter + earlier blocks
+            // 3. against the running frontier, on the key-ordered points; the
+            // survivors stay in key order
+            // In stages over growing ranges of R (R holds its strongest
+            // points first); the survivors of a stage are compacted before the
+            // next, so a few hard points do not keep whole slices scanning.
+            int32_t *ys = xs;
+            int64_t ytotal = xtotal;
+            const int64_t stage_end[3] = {4096, 32768, INT64_MAX};
+            int64_t r_lo = 0;
+            for (int stage = 0; stage < 3 && r_lo < r_count && ytotal > 0; ++stage) {
+                const int64_t r_hi = std::min<int64_t>(r_count, stage_end[stage]);
+                const int64_t n_in = ytotal;
+                const int64_t nsl = ceil_div(n_in, kDenseSlice);
+                ctx->alive.ensure(static_cast<size_t>(n_in), "hipMalloc(alive flags)");
+                cr.nalive.ensure(static_cast<size_t>(nsl) * 4, "hipMalloc(slice counts)");
+                K.dense_vs(dim3(static_cast<unsigned>(nsl)), s, ys, n_in, n_in,
+                           ctx->alive.as<uint8_t>(), cr.nalive.as<int32_t>(),
+                           R.pts.as<int32_t>() + r_lo, R.cap, r_hi - r_lo, d_tests);
+                hip_check(hipGetLastError(), "k_dense_vs_sorted");
+                d2h(nalive, cr.nalive, static_cast<size_t>(nsl), s, "slice survivor counts");
+                std::vector<ChunkDesc> sl(nsl);
+                std::vector<int32_t> sln(nsl);
+                outpos.resize(nsl);
+                int64_t kept = 0;
+                for (int64_t k = 0; k < nsl; ++k) {
+                    sl[k].start = sl[k].out_base = k * kDenseSlice;
+                    sl[k].len = static_cast<int32_t>(std::min<int64_t>(kDenseSlice, n_in - k * kDenseSlice));
+                    sl[k].group = 0;
+                    sl[k].group_first = 0;
+                    sl[k].group_count = 1;
+                    sln[k] = sl[k].len;
+                    outpos[k] = kept;
+                    kept += nalive[k];
+                }
+                if (kept > 0 && kept < n_in) {
+                    upload_table(cr, sl);
+                    h2d(cr.nsurv, sln, s, "hipMemcpy(slice sizes)");
+                    int32_t *dst = (ys == ctx->sort_a.as<int32_t>()) ? ctx->sort_b.as<int32_t>()
+                                                                      : ctx->sort_a.as<int32_t>();
+                    run_compact(cr, nsl, outpos, ys, n_in, ctx->alive.as<uint8_t>(), dst, kept);
+                    ys = dst;
+                }
+                ytotal = kept;
+                r_lo = r_hi;
+            }
+            if (debug_join) {
+                std::fprintf(stderr, "join batch: nodes=%d products=%lld local=%lld after_R=%lld",
+                             Kn, static_cast<long long>(bprod), static_cast<long long>(xtotal),
+                             static_cast<long long>(ytotal));
+            }
+            // 4. descending-key blocks of the rest: local filter, earlier blocks
+            int64_t btotal = 0;
             chunks.clear();
-            which.clear();
-            const int32_t nb = static_cast<int32_t>(ceil_div(xtotal, CH));
+            const int32_t nb = static_cast<int32_t>(ceil_div(ytotal, CH));
             for (int32_t k = 0; k < nb; ++k) {
                 ChunkDesc c;
                 c.start = static_cast<int64_t>(k) * CH;
-                c.len = static_cast<int32_t>(std::min<int64_t>(CH, xtotal - c.start));
+                c.len = static_cast<int32_t>(std::min<int64_t>(CH, ytotal - c.start));
                 c.out_base = c.start;
                 c.group = 0;
                 c.group_first = 0;
                 c.group_count = nb;
                 chunks.push_back(c);
-                if (k > 0) {
-                    which.push_back(k);
-                }
             }
-            upload_table(cb, chunks);
-            pmdd_k::SortedSrc ss{xs, xtotal};
-            run_filter(K.filter_sorted, cb, ctx->which_a, none, nb, &ss,
-                       ctx->surv_pts.as<int32_t>(), xtotal, ctx->alive.as<uint8_t>());
-            hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-            run_cross(K.cross_one_sided, cb, ctx->which_c, which, ctx->surv_pts.as<int32_t>(),
-                      xtotal, ctx->alive.as<uint8_t>());
+            if (nb > 0) {
+                upload_table(cb, chunks);
+                pmdd_k::SortedSrc ss{ys, ytotal};
+                run_filter(K.filter_sorted, cb, ctx->which_a, none, nb, &ss,
+                           ctx->surv_pts.as<int32_t>(), ytotal, ctx->alive.as<uint8_t>());
+                hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+                run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), ytotal,
+                                ctx->alive.as<uint8_t>());
+                d2h(nalive, cb.nalive, chunks.size(), s, "batch survivor counts");
+                outpos.resize(chunks.size());
+                for (size_t c = 0; c < chunks.size(); ++c) {
+                    outpos[c] = btotal;
+                    btotal += nalive[c];
+                }
+            } else {
+                hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+            }
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-            d2h(nalive, cb.nalive, chunks.size(), s, "batch survivor counts");
+            hip_check(hipStreamSynchronize(s), "batch filter sync");
+            if (debug_join) {
+                std::fprintf(stderr, " batch_frontier=%lld R=%lld\n",
+                             static_cast<long long>(btotal), static_cast<long long>(r_count));
+            }
             add_time(ctx->ev_a, ctx->ev_d, &st->kernel_expand_td_s);
             add_time(ctx->ev_d, ctx->ev_c, &st->kernel_filter_s);
             add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
             st->filter_launches += 1;
             st->filter_candidates += xtotal;
-            outpos.resize(chunks.size());
-            int64_t btotal = 0;
-            for (size_t c = 0; c < chunks.size(); ++c) {
-                outpos[c] = btotal;
-                btotal += nalive[c];
+            st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + btotal);
+            if (btotal == 0) {
+                continue;
             }
             B.cap = 0;
             B.pts.release();
-            grow_points(B, d, std::max<int64_t>(btotal, 1), 0, s);
-            run_compact(cb, nb, outpos, ctx->surv_pts.as<int32_t>(), xtotal,
+            grow_points(B, d, btotal, 0, s);
+            run_compact(cb, nb, outpos, ctx->surv_pts.as<int32_t>(), ytotal,
                         ctx->alive.as<uint8_t>(), B.pts.as<int32_t>(), B.cap);
-            st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + btotal);
             if (r_count == 0) {
                 hip_check(hipStreamSynchronize(s), "batch sync");
                 std::swap(R.pts, B.pts);
@@ -1275,9 +1382,10 @@ struct Run {
                 r_count = btotal;
                 continue;
             }
-            // 6. R minus what B strictly dominates, then R and B merged by key
+            // 5. R minus what B strictly dominates, then R and B merged by key
             rchunks.clear();
             const int32_t rn = static_cast<int32_t>(ceil_div(r_count, CH));
+            std::vector<int32_t> rlen(rn);
             for (int32_t k = 0; k < rn; ++k) {
                 ChunkDesc c;
                 c.start = static_cast<int64_t>(k) * CH;
@@ -1285,14 +1393,11 @@ struct Run {
                 c.out_base = c.start;
                 c.group = 0;
                 c.group_first = 0;
-                c.group_count = rn;
+                c.group_count = 1;
                 rchunks.push_back(c);
+                rlen[k] = c.len;
             }
             upload_table(cr, rchunks);
-            std::vector<int32_t> rlen(rn);
-            for (int32_t k = 0; k < rn; ++k) {
-                rlen[k] = rchunks[k].len;
-            }
             h2d(cr.nsurv, rlen, s, "hipMemcpy(frontier chunk sizes)");
             ctx->keep_b.ensure(static_cast<size_t>(r_count), "hipMalloc(frontier flags)");
             hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(r_count, 256))),
@@ -1319,14 +1424,14 @@ struct Run {
             hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
             run_compact(cr, rn, routpos, R.pts.as<int32_t>(), R.cap, ctx->keep_b.as<uint8_t>(),
                         M.pts.as<int32_t>(), M.cap);
-            for (int o = 0; o < d && btotal > 0; ++o) {
+            for (int o = 0; o < d; ++o) {
                 hip_check(hipMemcpyAsync(M.pts.as<int32_t>() + o * M.cap + ralive,
                                          B.pts.as<int32_t>() + o * B.cap,
                                          static_cast<size_t>(btotal) * 4, hipMemcpyDeviceToDevice,
                                          s),
                           "hipMemcpy(batch frontier)");
             }
-            // both buffers share the stride M.cap == Rn.cap for the merge pass
+            // both buffers have stride total_new for the merge pass
             std::vector<int64_t> rs = {0, ralive, total_new};
             std::vector<int32_t> gr = {0, 2};
             int32_t *merged = merge_runs(M.pts, Rn.pts, total_new, rs, gr, 2);
@@ -1499,7 +1604,7 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
         for (DevBuf *b : {&c->run_off, &c->which, &c->which_a, &c->which_b, &c->which_c,
                           &c->surv_pts, &c->alive, &c->counter, &c->join_meta, &c->keep_a,
                           &c->keep_b, &c->sort_a, &c->sort_b, &c->big_base, &c->big_node,
-                          &c->run_start, &c->grp_run}) {
+                          &c->run_start, &c->grp_run, &c->join_idx}) {
             b->release();
         }
     }

@@ -74,14 +74,17 @@ struct ExpandSrc {
     const int64_t *run_off; // [E]
 };
 
-// Candidate source of a coupling batch: for batch node k the products
-// td[td_begin[k] + a] + bu[bu_begin[k] + b], numbered a * bu_size[k] + b
-// after prod_off[k].
+// Candidate source of a coupling batch: for batch node k, the products of its
+// kept TD rows and BU columns, td[row_idx[row_base[k] + a]] +
+// bu[col_idx[col_base[k] + b]], numbered a * col_size[k] + b after
+// prod_off[k].
 struct JoinSrc {
     const int64_t *prod_off; // [K + 1]
-    const int64_t *td_begin; // [K]
-    const int64_t *bu_begin; // [K]
-    const int64_t *bu_size;  // [K]
+    const int64_t *row_base; // [K]
+    const int64_t *col_base; // [K]
+    const int64_t *col_size; // [K]
+    const int64_t *row_idx;  // TD point of each kept row
+    const int64_t *col_idx;  // BU point of each kept column
     int32_t K;
     const int32_t *td_pts;
     int64_t td_stride;
@@ -127,11 +130,11 @@ __device__ __forceinline__ void load_candidate(const JoinSrc &s, const ChunkDesc
         }
     }
     const int64_t local = q - s.prod_off[lo];
-    const int64_t bs = s.bu_size[lo];
+    const int64_t bs = s.col_size[lo];
     const int64_t a = local / bs;
     const int64_t b = local - a * bs;
-    const int64_t ti = s.td_begin[lo] + a;
-    const int64_t bi = s.bu_begin[lo] + b;
+    const int64_t ti = s.row_idx[s.row_base[lo] + a];
+    const int64_t bi = s.col_idx[s.col_base[lo] + b];
 #pragma unroll
     for (int o = 0; o < D; ++o) {
         p[o] = s.td_pts[o * s.td_stride + ti] + s.bu_pts[o * s.bu_stride + bi];
@@ -354,105 +357,92 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
 }
 
 // ---------------------------------------------------------------------------
-// k_cross_chunks: symmetric check of a chunk's survivors against the other
-// chunks of its group.  One workgroup per chunk listed in `which`.
+// k_cross_chunks: the chunks of a group are consecutive blocks of descending
+// keys; every alive point of a chunk dies when an alive point of an earlier
+// chunk of its group is >= it in every objective.  One workgroup per chunk
+// listed in `which`.  Launched in waves of consecutive chunks so that the
+// earlier chunks are mostly final already; tiles keep only alive points.
+// Flags of chunks in the same wave may change while they are read: either
+// value is correct (a point killed by cross checks is itself dominated by an
+// earlier point that also dominates whatever it dominates).
 // ---------------------------------------------------------------------------
-template <int D, int CH, bool ONE_SIDED>
+template <int D, int CH>
 __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
-    const int32_t *__restrict__ pts, int64_t stride, uint8_t *__restrict__ alive,
+    const int32_t *__restrict__ pts, int64_t stride, uint8_t *alive,
     const int32_t *__restrict__ chunk_nsurv, int32_t *__restrict__ chunk_nalive,
-    const int64_t *__restrict__ chunk_minkey, const int64_t *__restrict__ chunk_maxkey,
     unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
-    constexpr int R = CH / kBlock; // survivors held per thread
+    constexpr int R = CH / kBlock; // points held per thread
     __shared__ __align__(16) int32_t T[kBlock * DP];
+    __shared__ int wave_tot[kWaves];
     __shared__ int live_count;
 
     const int me = which[blockIdx.x];
     const ChunkDesc c = chunks[me];
     const int nS = chunk_nsurv[me];
     const int tid = threadIdx.x;
-    const int64_t my_min = chunk_minkey[me];
+    const volatile uint8_t *valive = alive;
 
     int32_t m[R][D];
     bool dead[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int s = r * kBlock + tid;
-        dead[r] = s >= nS;
-        if (!dead[r]) {
+        dead[r] = s >= nS || valive[c.out_base + s] == 0;
 #pragma unroll
-            for (int o = 0; o < D; ++o) {
-                m[r][o] = pts[o * stride + c.out_base + s];
-            }
-        } else {
-#pragma unroll
-            for (int o = 0; o < D; ++o) {
-                m[r][o] = 0;
-            }
+        for (int o = 0; o < D; ++o) {
+            m[r][o] = dead[r] ? INT32_MAX : pts[o * stride + c.out_base + s];
         }
     }
     unsigned long long tests = 0;
-
-    // ONE_SIDED: the group's chunks are consecutive key ranges in descending
-    // order, so only earlier chunks can dominate (and equal points there win).
-    const int g_end = ONE_SIDED ? (me - c.group_first) : c.group_count;
-    for (int g = 0; g < g_end; ++g) {
-        const int other = c.group_first + g;
-        if (other == me) {
-            continue;
-        }
-        if (chunk_maxkey[other] < my_min) {
-            continue; // every point there has a smaller key than every point here
-        }
+    bool any_live = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        any_live |= !dead[r];
+    }
+    bool go = __syncthreads_or(any_live);
+    for (int other = c.group_first; go && other < me; ++other) {
         const ChunkDesc oc = chunks[other];
         const int on = chunk_nsurv[other];
-        const bool earlier = other < me;
-        for (int tb = 0; tb < on; tb += kBlock) {
-            const int tn = min(kBlock, on - tb);
-            if (tid < tn) {
+        for (int tb = 0; go && tb < on; tb += kBlock) {
+            const int i = tb + tid;
+            const bool a = i < on && valive[oc.out_base + i] != 0;
+            int32_t q[D];
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                q[o] = a ? pts[o * stride + oc.out_base + i] : 0;
+            }
+            int tn = 0;
+            const int slot = block_rank(a, wave_tot, &tn);
+            if (a) {
 #pragma unroll
                 for (int o = 0; o < D; ++o) {
-                    T[tid * DP + o] = pts[o * stride + oc.out_base + tb + tid];
+                    T[slot * DP + o] = q[o];
                 }
             }
             __syncthreads();
+            for (int j = 0; j < tn; ++j) {
+                const int32_t *row = &T[j * DP];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                for (int j = 0; j < tn && !dead[r]; ++j) {
-                    const int32_t *q = &T[j * DP];
-                    bool ge = true;
-                    bool eq = true;
-#pragma unroll
-                    for (int o = 0; o < D; ++o) {
-                        ge &= q[o] >= m[r][o];
-                        eq &= q[o] == m[r][o];
-                    }
-                    dead[r] = ge && (!eq || earlier);
-                    if (pair_tests != nullptr) {
-                        ++tests;
-                    }
+                for (int r = 0; r < R; ++r) {
+                    dead[r] |= ge_all<D, DP>(row, m[r]);
                 }
             }
-            // stop early once every survivor of this chunk is dead
-            bool any_live = false;
+            if (pair_tests != nullptr) {
+                int live = 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    live += (r * kBlock + tid < nS) ? 1 : 0;
+                }
+                tests += static_cast<unsigned long long>(live) * tn;
+            }
+            any_live = false;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 any_live |= !dead[r];
             }
-            const int live = __syncthreads_or(any_live);
-            if (!live) {
-                break;
-            }
-        }
-        bool any_live = false;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            any_live |= !dead[r];
-        }
-        if (!__syncthreads_or(any_live)) {
-            break;
+            go = __syncthreads_or(any_live);
         }
     }
 
@@ -518,58 +508,58 @@ __global__ __launch_bounds__(kBlock) void k_compact_chunks(
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_state_dominance(
-    const int32_t *__restrict__ targets, // [n_targets][3]: node, cmpA, cmpB (-1 = none)
+    const int32_t *__restrict__ targets, // [n][4]: node, cmpA, cmpB (-1 = none), first label
     const int64_t *__restrict__ off, const int32_t *__restrict__ pts, int64_t stride,
     uint8_t *__restrict__ keep) {
     constexpr int DP = Pad<D>::DP;
     __shared__ __align__(16) int32_t T[kBlock * DP];
-    const int node = targets[3 * blockIdx.x];
+    const int32_t *tg = targets + 4 * static_cast<int64_t>(blockIdx.x);
+    const int node = tg[0];
     const int64_t b0 = off[node];
     const int64_t n = off[node + 1] - b0;
-    for (int64_t base = 0; base < n; base += kBlock) {
-        const int64_t i = base + threadIdx.x;
-        const bool valid = i < n;
-        int32_t m[D];
+    const int64_t base = tg[3];
+    const int64_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    int32_t m[D];
 #pragma unroll
-        for (int o = 0; o < D; ++o) {
-            m[o] = valid ? pts[o * stride + b0 + i] : 0;
+    for (int o = 0; o < D; ++o) {
+        m[o] = valid ? pts[o * stride + b0 + i] : 0;
+    }
+    bool dead = !valid;
+    // frontiers are in descending key order: the last label of this block has
+    // the smallest key, and a comparison label can only be >= a label whose
+    // key does not exceed its own
+    const int64_t last = min(base + kBlock, n) - 1;
+    const int64_t block_min = key_at<D>(pts, stride, b0 + last);
+    for (int cidx = 1; cidx <= 2; ++cidx) {
+        const int cmp = tg[cidx];
+        if (cmp < 0) {
+            continue;
         }
-        bool dead = !valid;
-        // frontiers are in descending key order: the last point of this block
-        // has the smallest key, and a comparison label can only be >= a point
-        // whose key does not exceed its own
-        const int64_t last = min(base + kBlock, n) - 1;
-        const int64_t block_min = key_at<D>(pts, stride, b0 + last);
-        for (int cidx = 1; cidx <= 2; ++cidx) {
-            const int cmp = targets[3 * blockIdx.x + cidx];
-            if (cmp < 0) {
-                continue;
+        const int64_t c0 = off[cmp];
+        const int64_t cn = off[cmp + 1] - c0;
+        for (int64_t tb = 0; tb < cn; tb += kBlock) {
+            if (key_at<D>(pts, stride, c0 + tb) < block_min) {
+                break;
             }
-            const int64_t c0 = off[cmp];
-            const int64_t cn = off[cmp + 1] - c0;
-            for (int64_t tb = 0; tb < cn; tb += kBlock) {
-                if (key_at<D>(pts, stride, c0 + tb) < block_min) {
-                    break;
-                }
-                if (!__syncthreads_or(!dead)) {
-                    break;
-                }
-                const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), cn - tb));
-                if (static_cast<int>(threadIdx.x) < tn) {
+            if (!__syncthreads_or(!dead)) {
+                break;
+            }
+            const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), cn - tb));
+            if (static_cast<int>(threadIdx.x) < tn) {
 #pragma unroll
-                    for (int o = 0; o < D; ++o) {
-                        T[threadIdx.x * DP + o] = pts[o * stride + c0 + tb + threadIdx.x];
-                    }
-                }
-                __syncthreads();
-                for (int j = 0; j < tn && !dead; ++j) {
-                    dead = ge_all<D, DP>(&T[j * DP], m);
+                for (int o = 0; o < D; ++o) {
+                    T[threadIdx.x * DP + o] = pts[o * stride + c0 + tb + threadIdx.x];
                 }
             }
+            __syncthreads();
+            for (int j = 0; j < tn && !dead; ++j) {
+                dead = ge_all<D, DP>(&T[j * DP], m);
+            }
         }
-        if (valid && dead) {
-            keep[b0 + i] = 0;
-        }
+    }
+    if (valid && dead) {
+        keep[b0 + i] = 0;
     }
 }
 
@@ -887,22 +877,26 @@ __global__ __launch_bounds__(kBlock) void k_vs_sorted(
             }
         }
         __syncthreads();
+        for (int j = 0; j < tn; ++j) {
+            const int32_t *q = &T[j * DP];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            for (int j = 0; j < tn && !dead[r]; ++j) {
-                const int32_t *q = &T[j * DP];
-                bool ge = true;
-                bool eq = true;
+            for (int r = 0; r < R; ++r) {
+                if (STRICT) {
+                    bool ge = true;
+                    bool eq = true;
 #pragma unroll
-                for (int o = 0; o < D; ++o) {
-                    ge &= q[o] >= m[r][o];
-                    eq &= q[o] == m[r][o];
-                }
-                dead[r] = STRICT ? (ge && !eq) : ge;
-                if (pair_tests != nullptr) {
-                    ++tests;
+                    for (int o = 0; o < D; ++o) {
+                        ge &= q[o] >= m[r][o];
+                        eq &= q[o] == m[r][o];
+                    }
+                    dead[r] |= ge && !eq;
+                } else {
+                    dead[r] |= ge_all<D, DP>(q, m[r]);
                 }
             }
+        }
+        if (pair_tests != nullptr) {
+            tests += static_cast<unsigned long long>(R) * tn;
         }
         bool any_live = false;
 #pragma unroll
@@ -933,6 +927,105 @@ __global__ __launch_bounds__(kBlock) void k_vs_sorted(
     __syncthreads();
     if (tid == 0) {
         chunk_nalive[me] = live_count;
+    }
+    if (pair_tests != nullptr) {
+        atomicAdd(pair_tests, tests);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_dense_vs_sorted: points of a dense array in descending key order die when
+// some point of R (descending key order) is >= them in every objective (equal
+// points die).  One wave per slice of 64*P consecutive points, so the early
+// exit waits for 64*P points only; each slice stops scanning R at the first
+// tile whose largest key is below the slice's smallest key.  Writes the alive
+// flag of every point and the alive count of every slice.
+// ---------------------------------------------------------------------------
+template <int D, int P>
+__global__ __launch_bounds__(kWave) void k_dense_vs_sorted(
+    const int32_t *__restrict__ X, int64_t xstride, int64_t n, uint8_t *__restrict__ alive,
+    int32_t *__restrict__ slice_alive, const int32_t *__restrict__ R, int64_t rstride, int64_t nr,
+    unsigned long long *__restrict__ pair_tests) {
+    constexpr int DP = Pad<D>::DP;
+    __shared__ __align__(16) int32_t T[kWave * DP];
+    __shared__ unsigned long long min_word;
+    const int tid = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kWave * P;
+    int32_t m[P][D];
+    bool dead[P];
+    int64_t my_min = INT64_MAX;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int64_t i = base + p * kWave + tid;
+        dead[p] = i >= n;
+        int64_t k = 0;
+#pragma unroll
+        for (int o = 0; o < D; ++o) {
+            m[p][o] = dead[p] ? INT32_MAX : X[o * xstride + i];
+            k += m[p][o];
+        }
+        if (!dead[p] && k < my_min) {
+            my_min = k;
+        }
+    }
+    if (tid == 0) {
+        min_word = ~0ULL;
+    }
+    __syncthreads();
+    if (my_min != INT64_MAX) {
+        atomicMin(&min_word, static_cast<unsigned long long>(my_min + (int64_t(1) << 62)));
+    }
+    __syncthreads();
+    const bool any = min_word != ~0ULL;
+    const int64_t wg_min = any ? static_cast<int64_t>(min_word) - (int64_t(1) << 62) : 0;
+    unsigned long long tests = 0;
+    for (int64_t tb = 0; any && tb < nr; tb += kWave) {
+        if (key_at<D>(R, rstride, tb) < wg_min) {
+            break;
+        }
+        const int tn = static_cast<int>(min(static_cast<int64_t>(kWave), nr - tb));
+        if (tid < tn) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                T[tid * DP + o] = R[o * rstride + tb + tid];
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < tn; ++j) {
+            const int32_t *q = &T[j * DP];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                dead[p] |= ge_all<D, DP>(q, m[p]);
+            }
+        }
+        if (pair_tests != nullptr) {
+            tests += static_cast<unsigned long long>(P) * tn;
+        }
+        bool any_live = false;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            any_live |= !dead[p];
+        }
+        if (!__syncthreads_or(any_live)) {
+            break;
+        }
+    }
+    int kept = 0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int64_t i = base + p * kWave + tid;
+        if (i < n) {
+            alive[i] = dead[p] ? 0 : 1;
+            kept += dead[p] ? 0 : 1;
+        }
+    }
+    // wave-wide sum of kept
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        kept += __shfl_xor(kept, off);
+    }
+    if (tid == 0) {
+        slice_alive[blockIdx.x] = kept;
     }
     if (pair_tests != nullptr) {
         atomicAdd(pair_tests, tests);
