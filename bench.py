@@ -12,8 +12,12 @@ copied back to the host).
 
 Prints one JSON line.  `value` = exact Pareto points per second over the
 timed steps (sum of final frontier sizes / wall time).  `roofline` covers the
-dominant kernel (k_filter_chunks); `cpu_baseline` times the CPU restatement
-of the reference's OpenMP enumerators on a bounded sample on this host.
+kernel with the most device time; `cpu_baseline` times the CPU restatement of
+the reference's OpenMP enumerators on a bounded sample on this host.
+
+Profiling helpers (for rocprofv3 runs): --only NAME and --cases M:SD,... run a
+subset of the step; --calib-copy-mib N runs one plain device copy of N MiB
+(torch) and exits, to scale the FETCH_SIZE / WRITE_SIZE counters.
 """
 import argparse
 import json
@@ -50,10 +54,10 @@ def build_workload(tmpdir):
     return dds
 
 
-def read_traffic():
-    """HBM bytes per k_filter_chunks launch from the committed rocprofv3 PMC
-    summary (profiles/k_filter_chunks_kp_traffic.txt), or None."""
-    p = os.path.join(ROOT, "profiles", "k_filter_chunks_kp_traffic.txt")
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (profiles/<kernel>_knapsack_traffic.txt), or None."""
+    p = os.path.join(ROOT, "profiles", f"{kernel}_knapsack_traffic.txt")
     if not os.path.exists(p):
         return None
     vals = {}
@@ -99,7 +103,25 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-pair-count", action="store_true",
                     help="skip the extra run that counts executed pair tests")
+    ap.add_argument("--only", default="", help="run only this workload instance")
+    ap.add_argument("--cases", default="", help="method:state_dominance list, e.g. 3:0,1:0")
+    ap.add_argument("--calib-copy-mib", type=int, default=0)
     args = ap.parse_args()
+    if args.calib_copy_mib:
+        import torch
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": n * 4}))
+        return
+    global WORKLOAD, CASES
+    if args.only:
+        WORKLOAD = [w for w in WORKLOAD if w[0] == args.only]
+    if args.cases:
+        CASES = [tuple(int(x) for x in c.split(":")) for c in args.cases.split(",")]
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                           "(instances shard across GPUs without communication)"}))
@@ -116,7 +138,9 @@ def main():
     def step():
         pts = 0
         agg = {"kernel_filter_s": 0.0, "filter_launches": 0, "algorithmic_bytes": 0,
-               "work_frontier_survivors_total": 0, "kernel_total_s": 0.0}
+               "kernel_frontier_check_s": 0.0, "frontier_check_launches": 0,
+               "frontier_check_bytes": 0, "work_frontier_survivors_total": 0,
+               "kernel_total_s": 0.0}
         per = {}
         for name, r in res:
             ref = None
@@ -162,14 +186,24 @@ def main():
                                           count_pair_tests=True).stats["dominance_pair_tests"]
         log("pair-test count done")
 
-    f_time = tot["kernel_filter_s"]
-    launches = max(tot["filter_launches"], 1)
-    alg_bytes_per_launch = tot["algorithmic_bytes"] / launches
-    avg_launch_s = f_time / launches
+    # dominant kernel: the one with the most device time over the timed steps
+    kernels = {
+        # k_filter_chunks: compulsory bytes of the expanded layers and joins
+        "k_filter_chunks": (tot["kernel_filter_s"], tot["filter_launches"],
+                            tot["algorithmic_bytes"]),
+        # k_dense_vs_sorted: points read once, frontier range read once, flags
+        "k_dense_vs_sorted": (tot["kernel_frontier_check_s"], tot["frontier_check_launches"],
+                              tot["frontier_check_bytes"]),
+    }
+    kname = max(kernels, key=lambda k: kernels[k][0])
+    k_time, k_launches, k_bytes = kernels[kname]
+    launches = max(k_launches, 1)
+    alg_bytes_per_launch = k_bytes / launches
+    avg_launch_s = k_time / launches
     achieved_gbs = alg_bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-    traffic = read_traffic()
+    traffic = read_traffic(kname)
     roofline = {
-        "kernel": "k_filter_chunks",
+        "kernel": kname,
         "bound": "hbm",
         "achieved": achieved_gbs,
         "peak": HBM_PEAK_GBS,
@@ -177,8 +211,9 @@ def main():
         "frac": achieved_gbs / HBM_PEAK_GBS,
         "traffic": traffic,
         "avg_launch_ms": avg_launch_s * 1e3,
-        "launches_per_step": tot["filter_launches"] / max(args.steps, 1),
+        "launches_per_step": k_launches / max(args.steps, 1),
         "algorithmic_bytes_per_launch": alg_bytes_per_launch,
+        "share_of_engine_kernel_time": k_time / max(tot["kernel_total_s"], 1e-12),
     }
     if pair_tests is not None:
         d_avg = sum(d for _, _, _, d in WORKLOAD) / len(WORKLOAD)

@@ -131,6 +131,13 @@ typedef struct pmdd_stats {
     int64_t filter_launches;
     int64_t filter_candidates;
     int64_t algorithmic_bytes;
+    /* The coupling kernel that checks key-ordered products against the
+     * running frontier (k_dense_vs_sorted): device time, launches, compulsory
+     * bytes (each point read once, the scanned frontier range read once, one
+     * flag written per point). */
+    double kernel_frontier_check_s;
+    int64_t frontier_check_launches;
+    int64_t frontier_check_bytes;
     int32_t layer_coupling;
     int32_t std_capacity;
     double *std_candidates_per_layer;

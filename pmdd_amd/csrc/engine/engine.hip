@@ -111,6 +111,7 @@ struct pmdd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
+    hipEvent_t ev_e = nullptr, ev_f = nullptr, ev_g = nullptr;
     // scratch reused across layers and calls
     ChunkBufs cb_main, cb_aux;
     DevBuf run_off, which, which_a, which_b, which_c;
@@ -1289,11 +1290,21 @@ struct Run {
                 const int64_t nsl = ceil_div(n_in, kDenseSlice);
                 ctx->alive.ensure(static_cast<size_t>(n_in), "hipMalloc(alive flags)");
                 cr.nalive.ensure(static_cast<size_t>(nsl) * 4, "hipMalloc(slice counts)");
+                hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
                 K.dense_vs(dim3(static_cast<unsigned>(nsl)), s, ys, n_in, n_in,
                            ctx->alive.as<uint8_t>(), cr.nalive.as<int32_t>(),
                            R.pts.as<int32_t>() + r_lo, R.cap, r_hi - r_lo, d_tests);
                 hip_check(hipGetLastError(), "k_dense_vs_sorted");
+                hip_check(hipEventRecord(ctx->ev_f, s), "hipEventRecord");
                 d2h(nalive, cr.nalive, static_cast<size_t>(nsl), s, "slice survivor counts");
+                {
+                    float ms = 0.f;
+                    hip_check(hipEventElapsedTime(&ms, ctx->ev_e, ctx->ev_f), "hipEventElapsedTime");
+                    st->kernel_frontier_check_s += ms * 1e-3;
+                    st->frontier_check_launches += 1;
+                    st->frontier_check_bytes +=
+                        static_cast<int64_t>(d) * 4 * (n_in + (r_hi - r_lo)) + n_in + 4 * nsl;
+                }
                 std::vector<ChunkDesc> sl(nsl);
                 std::vector<int32_t> sln(nsl);
                 outpos.resize(nsl);
@@ -1338,6 +1349,7 @@ struct Run {
                 c.group_count = nb;
                 chunks.push_back(c);
             }
+            hip_check(hipEventRecord(ctx->ev_g, s), "hipEventRecord");
             if (nb > 0) {
                 upload_table(cb, chunks);
                 pmdd_k::SortedSrc ss{ys, ytotal};
@@ -1362,10 +1374,10 @@ struct Run {
                              static_cast<long long>(btotal), static_cast<long long>(r_count));
             }
             add_time(ctx->ev_a, ctx->ev_d, &st->kernel_expand_td_s);
-            add_time(ctx->ev_d, ctx->ev_c, &st->kernel_filter_s);
+            add_time(ctx->ev_g, ctx->ev_c, &st->kernel_filter_s);
             add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
             st->filter_launches += 1;
-            st->filter_candidates += xtotal;
+            st->filter_candidates += ytotal;
             st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + btotal);
             if (btotal == 0) {
                 continue;
@@ -1577,6 +1589,9 @@ int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
         hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_c), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_d), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev_e), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev_f), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev_g), "hipEventCreate");
         *out = c.release();
         return PMDD_OK;
     });
@@ -1617,8 +1632,10 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
     if (ctx->ev_c != nullptr) {
         (void)hipEventDestroy(ctx->ev_c);
     }
-    if (ctx->ev_d != nullptr) {
-        (void)hipEventDestroy(ctx->ev_d);
+    for (hipEvent_t e : {ctx->ev_d, ctx->ev_e, ctx->ev_f, ctx->ev_g}) {
+        if (e != nullptr) {
+            (void)hipEventDestroy(e);
+        }
     }
     if (ctx->stream != nullptr) {
         (void)hipStreamDestroy(ctx->stream);
