@@ -51,11 +51,3 @@ def test_cli_outputs(tmp_path, name, ptype, method, sd):
     assert rec["structure"]["is_tsp_branch"] == (ptype == "3")
     if method == "3":
         assert rec["structure"]["layer_coupling"] > 0
-
-
-def test_cli_program_name_fixes_objective_count(tmp_path):
-    link = tmp_path / "multiobj_nobjs4"
-    os.symlink(abi.CLI_BIN, link)
-    r = subprocess.run([str(link), instance_path("KP_p-3_n-10_ins-1.dat"), "1", "1", "0", "gpu"],
-                       capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1 and "expects 4" in r.stdout

@@ -244,3 +244,11 @@ def test_generator_is_deterministic_and_in_format(tmp_path):
         p = gen.write(inst, str(tmp_path / f"{k}.dat"))
         dd = DecisionDiagram.from_file(p, inst["kind"])
         assert dd.d == inst["d"]
+
+
+def test_cli_program_name_fixes_objective_count(tmp_path):
+    link = tmp_path / "multiobj_nobjs4"
+    os.symlink(abi.CLI_BIN, link)
+    r = subprocess.run([str(link), instance_path("KP_p-3_n-10_ins-1.dat"), "1", "1", "0", "gpu"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "expects 4" in r.stdout
