@@ -189,8 +189,45 @@ void pmdd_resident_free(pmdd_resident *dd);
 
 void pmdd_frontier_free(pmdd_frontier *f);
 
-/* Engine build identity, e.g. "pmdd-engine gfx950 d=2..8". */
+/* Engine build identity, e.g. "pmdd-engine gfx950 d=1..8". */
 const char *pmdd_engine_version(void);
+
+/* ---- serving: a stream of instances with continuous batching ----
+ *
+ * No counterpart in the reference (which enumerates one instance per
+ * process).  A server owns `slots` worker contexts on one device, each with
+ * its own stream and resident scratch.  Submitted requests queue in arrival
+ * order; a request is admitted when a slot is free and its memory estimate
+ * fits the budget next to the requests already running, so requests that
+ * arrive while others are on the GPU overlap with them.  A request whose
+ * estimate exceeds the whole budget completes with PMDD_E_REJECTED; any
+ * failure stays with its own request. */
+typedef struct pmdd_server_config {
+    int32_t device;
+    int32_t slots;         /* concurrent requests (>= 1); 1 = one at a time */
+    int64_t memory_budget; /* bytes of HBM for admitted requests; 0 = 80% of free memory */
+} pmdd_server_config;
+
+typedef struct pmdd_server pmdd_server;
+
+#define PMDD_PENDING 1 /* pmdd_server_poll: request not finished yet */
+
+int pmdd_server_create(const pmdd_server_config *cfg, pmdd_server **out, char *err,
+                       size_t errlen);
+/* Queue a request; the view's arrays are copied, so the caller may free
+ * them on return.  *ticket identifies the request. */
+int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_opts *opts,
+                       int64_t *ticket, char *err, size_t errlen);
+/* PMDD_PENDING while running; otherwise the request's own return code, with
+ * the frontier and statistics moved to the caller (the ticket is then
+ * retired). */
+int pmdd_server_poll(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_stats *stats,
+                     char *err, size_t errlen);
+/* As pmdd_server_poll, but blocks until the request has finished. */
+int pmdd_server_wait(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_stats *stats,
+                     char *err, size_t errlen);
+/* Finishes the running requests, drops queued ones, joins the workers. */
+void pmdd_server_destroy(pmdd_server *srv);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,17 @@ class Stats(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("slots", C.c_int32),
+        ("memory_budget", C.c_int64),
+    ]
+
+
+PMDD_PENDING = 1
+
+
 class DDStructure(C.Structure):
     _fields_ = [
         ("original_width", C.c_int64),
@@ -128,6 +139,11 @@ ENGINE_SYMBOLS = [
     "pmdd_resident_free",
     "pmdd_frontier_free",
     "pmdd_engine_version",
+    "pmdd_server_create",
+    "pmdd_server_submit",
+    "pmdd_server_poll",
+    "pmdd_server_wait",
+    "pmdd_server_destroy",
 ]
 HOST_SYMBOLS = [
     "pmdd_dd_build_from_file",
@@ -212,6 +228,16 @@ def engine_lib() -> C.CDLL:
         lib.pmdd_frontier_free.restype = None
         lib.pmdd_engine_version.argtypes = []
         lib.pmdd_engine_version.restype = C.c_char_p
+        lib.pmdd_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)] + err
+        lib.pmdd_server_create.restype = C.c_int
+        lib.pmdd_server_submit.argtypes = [
+            C.c_void_p, C.POINTER(DDView), C.POINTER(Opts), C.POINTER(C.c_int64)] + err
+        lib.pmdd_server_submit.restype = C.c_int
+        for fn in (lib.pmdd_server_poll, lib.pmdd_server_wait):
+            fn.argtypes = [C.c_void_p, C.c_int64, C.POINTER(Frontier), C.POINTER(Stats)] + err
+            fn.restype = C.c_int
+        lib.pmdd_server_destroy.argtypes = [C.c_void_p]
+        lib.pmdd_server_destroy.restype = None
         _engine = lib
     return _engine
 

@@ -15,6 +15,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1735,5 +1740,253 @@ void pmdd_frontier_free(pmdd_frontier *f) {
 }
 
 const char *pmdd_engine_version(void) { return "pmdd-engine gfx950 d=1..8"; }
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: worker slots, each a context with its own stream; FIFO admission
+// under a memory budget.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ServerJob {
+    int64_t ticket = 0;
+    std::vector<int64_t> layer_off, in_off;
+    std::vector<int32_t> arc_src, arc_val, arc_w, min_weight;
+    pmdd_dd_view view{};
+    pmdd_opts opts{};
+    int64_t estimate = 0;
+    bool done = false;
+    int rc = PMDD_OK;
+    std::string err;
+    pmdd_frontier fr{};
+    pmdd_stats st{};
+};
+
+// Device bytes a request may claim: its resident diagram (both CSR
+// directions) plus two layer frontiers and one pass survivor buffer of at
+// most min(max_cand, 256 * widest layer) points of d int32 and a flag.
+int64_t estimate_bytes(const pmdd_dd_view &v, const pmdd_opts &o) {
+    const int L = v.num_layers;
+    const int64_t N = v.layer_off[L];
+    const int64_t A = v.in_off[N];
+    int64_t widest = 1;
+    for (int l = 0; l < L; ++l) {
+        widest = std::max<int64_t>(widest, v.layer_off[l + 1] - v.layer_off[l]);
+    }
+    const int64_t arena = 2 * (8 * (N + L) + 4 * A + 4 * int64_t(v.d) * A);
+    const int64_t cap = o.max_cand > 0 ? o.max_cand : PMDD_DEFAULT_MAX_CAND;
+    const int64_t pts = std::min<int64_t>(cap, 256 * widest);
+    return arena + 3 * pts * (4 * int64_t(v.d) + 1) + (int64_t(16) << 20);
+}
+
+} // namespace
+
+struct pmdd_server {
+    int device = 0;
+    int slots = 1;
+    int64_t budget = 0;
+    int64_t in_use = 0;
+    int64_t next_ticket = 1;
+    bool stopping = false;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<ServerJob>> queue;
+    std::unordered_map<int64_t, std::shared_ptr<ServerJob>> jobs;
+    std::vector<pmdd_ctx *> ctxs;
+    std::vector<std::thread> workers;
+
+    void worker(int slot) {
+        pmdd_ctx *ctx = ctxs[slot];
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            cv_work.wait(lk, [&] {
+                return stopping ||
+                       (!queue.empty() &&
+                        (in_use == 0 || queue.front()->estimate <= budget - in_use));
+            });
+            if (stopping) {
+                return;
+            }
+            std::shared_ptr<ServerJob> job = queue.front();
+            queue.pop_front();
+            in_use += job->estimate;
+            lk.unlock();
+            cv_work.notify_all();
+            char err[1024] = {0};
+            job->st.std_capacity = 0;
+            job->rc = pmdd_enumerate(ctx, &job->view, &job->opts, &job->fr, &job->st, err,
+                                     sizeof(err));
+            job->err = err;
+            lk.lock();
+            in_use -= job->estimate;
+            job->done = true;
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int pmdd_server_create(const pmdd_server_config *cfg, pmdd_server **out, char *err,
+                       size_t errlen) {
+    return guarded(err, errlen, [&]() -> int {
+        if (cfg == nullptr || out == nullptr || cfg->slots < 1) {
+            fail(PMDD_E_INVALID, "pmdd_server_create: bad configuration");
+        }
+        *out = nullptr;
+        std::unique_ptr<pmdd_server> srv(new pmdd_server());
+        srv->device = cfg->device;
+        srv->slots = cfg->slots;
+        for (int i = 0; i < cfg->slots; ++i) {
+            pmdd_ctx *c = nullptr;
+            char e[512] = {0};
+            const int rc = pmdd_ctx_create(cfg->device, &c, e, sizeof(e));
+            if (rc != PMDD_OK) {
+                for (pmdd_ctx *x : srv->ctxs) {
+                    pmdd_ctx_destroy(x);
+                }
+                fail(rc, e);
+            }
+            srv->ctxs.push_back(c);
+        }
+        if (cfg->memory_budget > 0) {
+            srv->budget = cfg->memory_budget;
+        } else {
+            size_t fr = 0, tot = 0;
+            hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+            srv->budget = static_cast<int64_t>(fr) / 5 * 4;
+        }
+        pmdd_server *raw = srv.get();
+        for (int i = 0; i < cfg->slots; ++i) {
+            srv->workers.emplace_back([raw, i] { raw->worker(i); });
+        }
+        *out = srv.release();
+        return PMDD_OK;
+    });
+}
+
+int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_opts *opts,
+                       int64_t *ticket, char *err, size_t errlen) {
+    return guarded(err, errlen, [&]() -> int {
+        if (srv == nullptr || view == nullptr || opts == nullptr || ticket == nullptr) {
+            fail(PMDD_E_INVALID, "pmdd_server_submit: NULL argument");
+        }
+        validate_view(view);
+        auto job = std::make_shared<ServerJob>();
+        const int L = view->num_layers;
+        const int64_t N = view->layer_off[L];
+        const int64_t A = view->in_off[N];
+        job->layer_off.assign(view->layer_off, view->layer_off + L + 1);
+        job->in_off.assign(view->in_off, view->in_off + N + 1);
+        job->arc_src.assign(view->arc_src, view->arc_src + A);
+        job->arc_val.assign(view->arc_val, view->arc_val + A);
+        job->arc_w.assign(view->arc_w, view->arc_w + A * view->d);
+        if (view->min_weight != nullptr) {
+            job->min_weight.assign(view->min_weight, view->min_weight + N);
+        }
+        job->view = *view;
+        job->view.layer_off = job->layer_off.data();
+        job->view.in_off = job->in_off.data();
+        job->view.arc_src = job->arc_src.data();
+        job->view.arc_val = job->arc_val.data();
+        job->view.arc_w = job->arc_w.data();
+        job->view.min_weight = view->min_weight != nullptr ? job->min_weight.data() : nullptr;
+        job->opts = *opts;
+        job->estimate = estimate_bytes(*view, *opts);
+        std::lock_guard<std::mutex> lk(srv->mu);
+        job->ticket = srv->next_ticket++;
+        *ticket = job->ticket;
+        srv->jobs[job->ticket] = job;
+        if (job->estimate > srv->budget) {
+            job->done = true;
+            job->rc = PMDD_E_REJECTED;
+            job->err = "request needs about " + std::to_string(job->estimate) +
+                       " bytes of device memory, more than the server budget of " +
+                       std::to_string(srv->budget);
+            srv->cv_done.notify_all();
+            return PMDD_OK;
+        }
+        srv->queue.push_back(job);
+        srv->cv_work.notify_all();
+        return PMDD_OK;
+    });
+}
+
+static int server_collect(pmdd_server *srv, int64_t ticket, bool block, pmdd_frontier *out,
+                          pmdd_stats *stats, char *err, size_t errlen) {
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->jobs.find(ticket);
+    if (it == srv->jobs.end()) {
+        write_err(err, errlen, "unknown or already collected ticket");
+        return PMDD_E_INVALID;
+    }
+    std::shared_ptr<ServerJob> job = it->second;
+    if (block) {
+        srv->cv_done.wait(lk, [&] { return job->done; });
+    } else if (!job->done) {
+        return PMDD_PENDING;
+    }
+    srv->jobs.erase(it);
+    lk.unlock();
+    if (out != nullptr) {
+        *out = job->fr;
+    } else {
+        pmdd_frontier_free(&job->fr);
+    }
+    job->fr.points = nullptr;
+    if (stats != nullptr) {
+        double *a = stats->std_candidates_per_layer;
+        double *b = stats->std_frontier_survivors_per_layer;
+        const int32_t cap = stats->std_capacity;
+        *stats = job->st;
+        stats->std_candidates_per_layer = a;
+        stats->std_frontier_survivors_per_layer = b;
+        stats->std_capacity = cap;
+    }
+    write_err(err, errlen, job->err);
+    return job->rc;
+}
+
+int pmdd_server_poll(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_stats *stats,
+                     char *err, size_t errlen) {
+    if (srv == nullptr) {
+        write_err(err, errlen, "pmdd_server_poll: NULL server");
+        return PMDD_E_INVALID;
+    }
+    return server_collect(srv, ticket, false, out, stats, err, errlen);
+}
+
+int pmdd_server_wait(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_stats *stats,
+                     char *err, size_t errlen) {
+    if (srv == nullptr) {
+        write_err(err, errlen, "pmdd_server_wait: NULL server");
+        return PMDD_E_INVALID;
+    }
+    return server_collect(srv, ticket, true, out, stats, err, errlen);
+}
+
+void pmdd_server_destroy(pmdd_server *srv) {
+    if (srv == nullptr) {
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stopping = true;
+        srv->queue.clear();
+    }
+    srv->cv_work.notify_all();
+    for (std::thread &t : srv->workers) {
+        t.join();
+    }
+    for (auto &kv : srv->jobs) {
+        pmdd_frontier_free(&kv.second->fr);
+    }
+    for (pmdd_ctx *c : srv->ctxs) {
+        pmdd_ctx_destroy(c);
+    }
+    delete srv;
+}
 
 } // extern "C"

@@ -1,0 +1,182 @@
+"""Serving a stream of instances on one GPU.
+
+`Server` wraps pmdd_server_* (include/pmdd_engine.h): `slots` worker contexts,
+each with its own HIP stream, FIFO admission under a device-memory budget,
+so that requests arriving while others run overlap with them on the device.
+`slots=1` is the plain one-instance-at-a-time path.
+
+`mixed_stream` draws the request mix of the serving configuration: 60%
+knapsack (n uniform in [20,70]), 25% set packing (n in [60,150], m = n/5),
+15% TSP (n in [8,11]); objective count uniform over {3,4,5}; every instance
+seeded (SplitMix64).  `run_stream` replays a stream with Poisson arrivals and
+reports instances/s, Pareto points/s and p50/p99 latency measured from
+arrival to result (host diagram construction included).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import abi
+from .abi import PmddError
+from .dd import DecisionDiagram
+from .engine import EnumerationResult, _stats_dict, _take_frontier, sort_lexicographic_ascending
+from . import instances as gen
+
+
+class Server:
+    def __init__(self, device: int = 0, slots: int = 8, memory_budget: int = 0):
+        lib = abi.engine_lib()
+        self._lib = lib
+        cfg = abi.ServerConfig(device, slots, memory_budget)
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib.pmdd_server_create(C.byref(cfg), C.byref(h), err, len(err))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, err.value.decode())
+        self._h = h
+        self.slots = slots
+
+    def submit(self, dd, method: int = 1, state_dominance: int = 0,
+               max_cand: int = abi.DEFAULT_MAX_CAND,
+               max_prod: int = abi.DEFAULT_MAX_PROD) -> int:
+        o = abi.Opts(int(method), int(dd.problem_type), int(state_dominance), 0, int(max_cand),
+                     int(max_prod))
+        t = C.c_int64()
+        err = C.create_string_buffer(1024)
+        rc = self._lib.pmdd_server_submit(self._h, dd.view_ptr, C.byref(o), C.byref(t), err,
+                                          len(err))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, err.value.decode())
+        return int(t.value)
+
+    def _collect(self, fn, ticket):
+        fr = abi.Frontier()
+        st = abi.Stats()
+        err = C.create_string_buffer(1024)
+        rc = fn(self._h, C.c_int64(ticket), C.byref(fr), C.byref(st), err, len(err))
+        if rc == abi.PMDD_PENDING:
+            return None
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, err.value.decode())
+        pts = sort_lexicographic_ascending(_take_frontier(self._lib, fr))
+        return EnumerationResult(pts, _stats_dict(st, 0, None, None))
+
+    def poll(self, ticket: int):
+        return self._collect(self._lib.pmdd_server_poll, ticket)
+
+    def wait(self, ticket: int) -> EnumerationResult:
+        return self._collect(self._lib.pmdd_server_wait, ticket)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            self._lib.pmdd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def mixed_stream(count: int, seed: int = 2000) -> list[dict]:
+    """Request specs of the serving configuration (see module docstring)."""
+    rng = gen.SplitMix64(seed)
+    specs = []
+    for i in range(count):
+        r = rng.uniform(0, 99)
+        d = rng.uniform(3, 5)
+        inst_seed = rng.next() & 0xFFFFFFFF
+        if r < 60:
+            specs.append({"kind": 1, "n": rng.uniform(20, 70), "d": d, "seed": inst_seed})
+        elif r < 85:
+            n = rng.uniform(60, 150)
+            specs.append({"kind": 2, "n": n, "d": d, "seed": inst_seed, "m": n // 5})
+        else:
+            specs.append({"kind": 3, "n": rng.uniform(8, 11), "d": d, "seed": inst_seed})
+    return specs
+
+
+def poisson_arrivals(count: int, rate: float, seed: int = 7) -> list[float]:
+    """Arrival times (s) of a Poisson process of `rate` requests per second."""
+    rng = gen.SplitMix64(seed)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        u = ((rng.next() >> 11) + 0.5) / float(1 << 53)
+        t += -math.log(u) / rate
+        out.append(t)
+    return out
+
+
+def write_stream(specs: list[dict], directory: str) -> list[str]:
+    os.makedirs(directory, exist_ok=True)
+    paths = []
+    for i, sp in enumerate(specs):
+        extra = {"m": sp["m"]} if "m" in sp else {}
+        inst = gen.generate(sp["kind"], sp["seed"], sp["n"], sp["d"], **extra)
+        paths.append(gen.write(inst, os.path.join(directory, f"req{i:05d}.dat")))
+    return paths
+
+
+def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots: int,
+               method: int = 1, device: int = 0, builders: int = 16,
+               results: dict | None = None) -> dict:
+    """Replay the stream: at each arrival the request's diagram is built on a
+    host thread and submitted; latency runs from arrival to result."""
+    server = Server(device, slots=slots)
+    n = len(paths)
+    done_at = [0.0] * n
+    points = [0] * n
+    errors = {}
+    lock = threading.Lock()
+    t0 = time.perf_counter()
+
+    def serve(i):
+        delay = t0 + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        try:
+            dd = DecisionDiagram.from_file(paths[i], specs[i]["kind"])
+            ticket = server.submit(dd, method=method)
+            dd.close()
+            res = server.wait(ticket)
+            with lock:
+                points[i] = len(res.points)
+                if results is not None:
+                    results[i] = res.points
+        except PmddError as e:
+            with lock:
+                errors[i] = (e.code, e.message)
+        done_at[i] = time.perf_counter()
+
+    with ThreadPoolExecutor(max_workers=max(builders, slots * 2, 8)) as pool:
+        list(pool.map(serve, range(n)))
+    server.close()
+    lat = np.array([done_at[i] - (t0 + arrivals[i]) for i in range(n) if i not in errors])
+    span = max(done_at) - t0
+    return {
+        "slots": slots,
+        "requests": n,
+        "failed": len(errors),
+        "errors": {str(k): v for k, v in list(errors.items())[:5]},
+        "span_s": span,
+        "inst_per_s": (n - len(errors)) / span,
+        "pareto_pts_per_s": sum(points) / span,
+        "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+        "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+        "mean_latency_s": float(lat.mean()) if len(lat) else None,
+    }

@@ -1,0 +1,64 @@
+"""Serving: every request's frontier is the same alone and in the mix, and a
+rejected request leaves the others unaffected."""
+import numpy as np
+import pytest
+
+from pmdd_amd import DecisionDiagram, abi
+from pmdd_amd import instances as gen
+from pmdd_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b):
+    return a.shape == b.shape and bool((a == b).all())
+
+
+@pytest.mark.parametrize("method", [1, 3])
+def test_mixed_stream_alone_vs_in_the_mix(engine, tmp_path, method):
+    specs = serving.mixed_stream(24, seed=11)
+    paths = serving.write_stream(specs, str(tmp_path))
+    alone = []
+    for sp, p in zip(specs, paths):
+        alone.append(engine.enumerate(DecisionDiagram.from_file(p, sp["kind"]), method=method).points)
+    results = {}
+    arrivals = [0.0] * len(paths)   # everything arrives at once: maximal overlap
+    m = serving.run_stream(paths, specs, arrivals, slots=6, method=method, results=results)
+    assert m["failed"] == 0
+    for i in range(len(paths)):
+        assert _same(results[i], alone[i]), (i, specs[i])
+
+
+def test_rejection_is_isolated(engine, tmp_path):
+    small = gen.write(gen.tsp(3, 8, 3), str(tmp_path / "small.dat"))
+    big = gen.write(gen.setpacking(5, 150, 4, m=30), str(tmp_path / "big.dat"))
+    dd_s = DecisionDiagram.from_file(small, 3)
+    dd_b = DecisionDiagram.from_file(big, 2)
+    ref = engine.enumerate(dd_s, method=1).points
+    with serving.Server(0, slots=2, memory_budget=64 << 20) as srv:
+        t1 = srv.submit(dd_s, method=1)
+        t2 = srv.submit(dd_b, method=1)
+        t3 = srv.submit(dd_s, method=3)
+        with pytest.raises(abi.PmddError) as e:
+            srv.wait(t2)
+        assert e.value.code == abi.PMDD_E_REJECTED
+        assert _same(srv.wait(t1).points, ref)
+        assert _same(srv.wait(t3).points, ref)
+
+
+def test_poll_then_collect(engine, tmp_path):
+    p = gen.write(gen.knapsack(9, 40, 4), str(tmp_path / "k.dat"))
+    dd = DecisionDiagram.from_file(p, 1)
+    ref = engine.enumerate(dd, method=3).points
+    with serving.Server(0, slots=1) as srv:
+        t = srv.submit(dd, method=3)
+        res = None
+        for _ in range(200000):
+            res = srv.poll(t)
+            if res is not None:
+                break
+        if res is None:
+            res = srv.wait(t)
+        assert _same(res.points, ref)
+        with pytest.raises(abi.PmddError):
+            srv.poll(t)  # collected tickets are retired
