@@ -191,8 +191,9 @@ def main():
         # k_filter_chunks: compulsory bytes of the expanded layers and joins
         "k_filter_chunks": (tot["kernel_filter_s"], tot["filter_launches"],
                             tot["algorithmic_bytes"]),
-        # k_dense_vs_sorted: points read once, frontier range read once, flags
-        "k_dense_vs_sorted": (tot["kernel_frontier_check_s"], tot["frontier_check_launches"],
+        # k_refill_vs_block: products read once per block, survivors written,
+        # the block of R read once
+        "k_refill_vs_block": (tot["kernel_frontier_check_s"], tot["frontier_check_launches"],
                               tot["frontier_check_bytes"]),
     }
     kname = max(kernels, key=lambda k: kernels[k][0])

@@ -122,7 +122,7 @@ struct pmdd_ctx {
     DevBuf run_off, which, which_a, which_b, which_c;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
-    DevBuf join_idx;
+    DevBuf join_idx, join_x, join_y, join_ctr;
 };
 
 struct LayerCSR {
@@ -445,6 +445,7 @@ struct Launch {
     FilterFn filter_expand;  // ExpandSrc, presorted in LDS
     FilterFn filter_join;    // JoinSrc, presorted in LDS
     FilterFn filter_sorted;  // SortedSrc, source already in key order
+    FilterFn filter_dense;   // SortedSrc in any order, presorted in LDS
     CrossFn cross;           // chunks of a group are descending key blocks
     VsSortedFn vs_weak;
     VsSortedFn vs_strict;
@@ -458,13 +459,19 @@ struct Launch {
                         pmdd_k::ExpandSrc, int32_t *, int64_t);
     void (*merge_pass)(dim3, hipStream_t, const int32_t *, int32_t *, int64_t, const int64_t *,
                        const int32_t *, int32_t, int, int64_t);
-    void (*dense_vs)(dim3, hipStream_t, const int32_t *, int64_t, int64_t, uint8_t *, int32_t *,
-                     const int32_t *, int64_t, int64_t, unsigned long long *);
+    void (*join_products)(dim3, hipStream_t, const pmdd_k::JoinSrc &, int64_t, int32_t *,
+                          int64_t);
+    void (*refill_vs)(dim3, hipStream_t, const int32_t *, int64_t, const unsigned long long *,
+                      const int32_t *, int64_t, int64_t, int64_t, int32_t *, int64_t,
+                      unsigned long long *, unsigned long long *);
 };
 
-// points per lane of k_dense_vs_sorted; a slice is 64 * kDenseP points
-constexpr int kDenseP = 4;
-constexpr int kDenseSlice = 64 * kDenseP;
+// Coupling: blocks of R after the first 8192 points hold kRefillBlock points;
+// k_refill_vs_block runs at most kRefillWaves one-wave workgroups (16 per CU).
+constexpr int64_t kRefillBlock = 8192;
+constexpr int64_t kRefillWaves = 4096;
+// Largest coupling batch (products) once the first batches have built R.
+constexpr int64_t kJoinBatchCap = int64_t(1) << 25;
 
 template <int D> struct LaunchImpl {
     static constexpr int CH = pmdd_k::Cap<D>::CH;
@@ -518,15 +525,22 @@ template <int D> struct LaunchImpl {
         hipLaunchKernelGGL((pmdd_k::k_merge_pass<D>), g, dim3(kBlock), 0, s, in, out, stride,
                            run_start, grp_run, G, pass, total);
     }
-    static void dense_vs(dim3 g, hipStream_t s, const int32_t *X, int64_t xs, int64_t n,
-                         uint8_t *alive, int32_t *sa, const int32_t *R, int64_t rs, int64_t nr,
-                         unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_dense_vs_sorted<D, kDenseP>), g, dim3(pmdd_k::kWave), 0, s,
-                           X, xs, n, alive, sa, R, rs, nr, pt);
+    static void join_products(dim3 g, hipStream_t s, const pmdd_k::JoinSrc &src, int64_t n,
+                              int32_t *X, int64_t xs) {
+        hipLaunchKernelGGL((pmdd_k::k_join_products<D>), g, dim3(kBlock), 0, s, src, n, X, xs);
+    }
+    static void refill_vs(dim3 g, hipStream_t s, const int32_t *X, int64_t xs,
+                          const unsigned long long *n_in, const int32_t *R, int64_t rs, int64_t lo,
+                          int64_t hi, int32_t *Y, int64_t ys, unsigned long long *n_out,
+                          unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D>), g, dim3(pmdd_k::kWave), 0, s, X, xs,
+                           n_in, R, rs, lo, hi, Y, ys, n_out, pt);
     }
     static Launch table() {
         Launch L;
-        L.dense_vs = dense_vs;
+        L.join_products = join_products;
+        L.refill_vs = refill_vs;
+        L.filter_dense = filter<pmdd_k::SortedSrc, false>;
         L.chunk_cap = CH;
         L.filter_expand = filter<pmdd_k::ExpandSrc, false>;
         L.filter_join = filter<pmdd_k::JoinSrc, false>;
@@ -1183,9 +1197,11 @@ struct Run {
         ChunkBufs &cb = ctx->cb_main;
         ChunkBufs &cr = ctx->cb_aux;
         const std::vector<int32_t> none;
-        // batches start small and double up to max_prod: the first batches
-        // build R cheaply, later ones are mostly filtered by it
+        // Batches start at min(max_prod, 32K) products and double up to
+        // max(max_prod, kJoinBatchCap): the first batches build R cheaply, and
+        // the later, large ones are mostly filtered by it in few launches.
         int64_t batch_cap = std::min<int64_t>(opts.max_prod, int64_t(1) << 15);
+        const int64_t batch_limit = std::max<int64_t>(opts.max_prod, kJoinBatchCap);
         while (i < nodes.size()) {
             int64_t bprod = 0;
             size_t j = i;
@@ -1197,7 +1213,7 @@ struct Run {
                 bprod += p;
                 ++j;
             }
-            batch_cap = std::min<int64_t>(opts.max_prod, batch_cap * 2);
+            batch_cap = std::min<int64_t>(batch_limit, batch_cap * 2);
             const int32_t Kn = static_cast<int32_t>(j - i);
             std::vector<int64_t> meta(4 * static_cast<size_t>(Kn) + 1);
             std::vector<int64_t> idx;
@@ -1237,13 +1253,86 @@ struct Run {
             src.bu_pts = bu.pts.as<int32_t>();
             src.bu_stride = bu.cap;
 
-            // 1. products, chunk by chunk (chunks may straddle nodes)
+            // 1. the batch's products, densely, in product order
+            const size_t pbytes = static_cast<size_t>(bprod) * d * 4;
+            ctx->join_x.ensure(pbytes, "hipMalloc(product buffer)");
+            ctx->join_y.ensure(pbytes, "hipMalloc(product buffer)");
+            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            K.join_products(dim3(static_cast<unsigned>(
+                                std::min<int64_t>(ceil_div(bprod, pmdd_k::kBlock), 16384))),
+                            s, src, bprod, ctx->join_x.as<int32_t>(), bprod);
+            hip_check(hipGetLastError(), "k_join_products");
+            hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+            // 2. against the running frontier R (descending key order), in
+            // blocks of growing length: most products meet a dominator among
+            // R's first points, so the later blocks see few of them
+            int32_t *F = ctx->join_x.as<int32_t>();
+            int64_t ytotal = bprod;
+            std::vector<unsigned long long> ctr;
+            std::vector<int64_t> bounds;
+            if (r_count > 0) {
+                bounds.assign(1, 0);
+                for (int64_t b : {int64_t(512), int64_t(2048), int64_t(8192)}) {
+                    if (b < r_count) {
+                        bounds.push_back(b);
+                    }
+                }
+                while (bounds.back() + kRefillBlock < r_count) {
+                    bounds.push_back(bounds.back() + kRefillBlock);
+                }
+                bounds.push_back(r_count);
+                const int nbk = static_cast<int>(bounds.size()) - 1;
+                // ctr[0]: products; ctr[1 + b]: survivors of block b
+                ctr.assign(1 + static_cast<size_t>(nbk), 0);
+                ctr[0] = static_cast<unsigned long long>(bprod);
+                h2d(ctx->join_ctr, ctr, s, "hipMemcpy(join counters)");
+                unsigned long long *dc = ctx->join_ctr.as<unsigned long long>();
+                const unsigned grid = static_cast<unsigned>(
+                    std::min<int64_t>(ceil_div(bprod, pmdd_k::kWave), kRefillWaves));
+                int32_t *in = ctx->join_x.as<int32_t>();
+                int32_t *outb = ctx->join_y.as<int32_t>();
+                hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
+                for (int b = 0; b < nbk; ++b) {
+                    K.refill_vs(dim3(grid), s, in, bprod, dc + b, R.pts.as<int32_t>(), R.cap,
+                                bounds[b], bounds[b + 1], outb, bprod, dc + 1 + b, d_tests);
+                    hip_check(hipGetLastError(), "k_refill_vs_block");
+                    std::swap(in, outb);
+                }
+                hip_check(hipEventRecord(ctx->ev_f, s), "hipEventRecord");
+                d2h(ctr, ctx->join_ctr, ctr.size(), s, "join counters");
+                F = in;
+                ytotal = static_cast<int64_t>(ctr[nbk]);
+                add_time(ctx->ev_e, ctx->ev_f, &st->kernel_frontier_check_s);
+                add_time(ctx->ev_e, ctx->ev_f, &st->kernel_dominance_s);
+                st->frontier_check_launches += nbk;
+                for (int b = 0; b < nbk; ++b) {
+                    const int64_t n_in = static_cast<int64_t>(ctr[b]);
+                    const int64_t n_out = static_cast<int64_t>(ctr[b + 1]);
+                    st->frontier_check_bytes +=
+                        static_cast<int64_t>(d) * 4 * (n_in + n_out + bounds[b + 1] - bounds[b]);
+                }
+            } else {
+                hip_check(hipStreamSynchronize(s), "product sync");
+            }
+            add_time(ctx->ev_a, ctx->ev_c, &st->kernel_expand_td_s);
+            if (debug_join) {
+                std::fprintf(stderr, "join batch: nodes=%d products=%lld after_R=%lld",
+                             Kn, static_cast<long long>(bprod), static_cast<long long>(ytotal));
+            }
+            if (ytotal == 0) {
+                if (debug_join) {
+                    std::fprintf(stderr, " R=%lld\n", static_cast<long long>(r_count));
+                }
+                continue;
+            }
+            // 3. the survivors: filtered chunk by chunk (presorted in LDS),
+            // then merged into key order
             chunks.clear();
-            const int32_t cnt = static_cast<int32_t>(ceil_div(bprod, CH));
+            const int32_t cnt = static_cast<int32_t>(ceil_div(ytotal, CH));
             for (int32_t k = 0; k < cnt; ++k) {
                 ChunkDesc c;
                 c.start = static_cast<int64_t>(k) * CH;
-                c.len = static_cast<int32_t>(std::min<int64_t>(CH, bprod - c.start));
+                c.len = static_cast<int32_t>(std::min<int64_t>(CH, ytotal - c.start));
                 c.out_base = c.start;
                 c.group = 0;
                 c.group_first = 0;
@@ -1251,17 +1340,18 @@ struct Run {
                 chunks.push_back(c);
             }
             upload_table(cb, chunks);
-            ctx->surv_pts.ensure(static_cast<size_t>(bprod) * d * 4, "hipMalloc(survivor buffer)");
-            ctx->alive.ensure(static_cast<size_t>(bprod), "hipMalloc(alive flags)");
+            ctx->surv_pts.ensure(static_cast<size_t>(ytotal) * d * 4, "hipMalloc(survivor buffer)");
+            ctx->alive.ensure(static_cast<size_t>(ytotal), "hipMalloc(alive flags)");
+            pmdd_k::SortedSrc fsrc{F, bprod};
             hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            run_filter(K.filter_join, cb, ctx->which_a, none, cnt, &src,
-                       ctx->surv_pts.as<int32_t>(), bprod, ctx->alive.as<uint8_t>());
+            run_filter(K.filter_dense, cb, ctx->which_a, none, cnt, &fsrc,
+                       ctx->surv_pts.as<int32_t>(), ytotal, ctx->alive.as<uint8_t>());
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
             d2h(nalive, cb.nalive, chunks.size(), s, "product survivor counts");
             add_time(ctx->ev_a, ctx->ev_c, &st->kernel_filter_s);
             add_time(ctx->ev_a, ctx->ev_c, &st->kernel_dominance_s);
             st->filter_launches += 1;
-            st->filter_candidates += bprod;
+            st->filter_candidates += ytotal;
             outpos.resize(chunks.size());
             run_start.clear();
             int64_t xtotal = 0;
@@ -1272,74 +1362,13 @@ struct Run {
             }
             run_start.push_back(xtotal);
             grp_run.assign({0, cnt});
-            // 2. survivors to a dense buffer and into key order
             ctx->sort_a.ensure(static_cast<size_t>(xtotal) * d * 4, "hipMalloc(sort buffer)");
             ctx->sort_b.ensure(static_cast<size_t>(xtotal) * d * 4, "hipMalloc(sort buffer)");
-            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            run_compact(cb, cnt, outpos, ctx->surv_pts.as<int32_t>(), bprod,
-                        ctx->alive.as<uint8_t>(), ctx->sort_a.as<int32_t>(), xtotal);
-            int32_t *xs = merge_runs(ctx->sort_a, ctx->sort_b, xtotal, run_start, grp_run, cnt);
             hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
-            // 3. against the running frontier, on the key-ordered points; the
-            // survivors stay in key order
-            // In stages over growing ranges of R (R holds its strongest
-            // points first); the survivors of a stage are compacted before the
-            // next, so a few hard points do not keep whole slices scanning.
-            int32_t *ys = xs;
-            int64_t ytotal = xtotal;
-            const int64_t stage_end[3] = {4096, 32768, INT64_MAX};
-            int64_t r_lo = 0;
-            for (int stage = 0; stage < 3 && r_lo < r_count && ytotal > 0; ++stage) {
-                const int64_t r_hi = std::min<int64_t>(r_count, stage_end[stage]);
-                const int64_t n_in = ytotal;
-                const int64_t nsl = ceil_div(n_in, kDenseSlice);
-                ctx->alive.ensure(static_cast<size_t>(n_in), "hipMalloc(alive flags)");
-                cr.nalive.ensure(static_cast<size_t>(nsl) * 4, "hipMalloc(slice counts)");
-                hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
-                K.dense_vs(dim3(static_cast<unsigned>(nsl)), s, ys, n_in, n_in,
-                           ctx->alive.as<uint8_t>(), cr.nalive.as<int32_t>(),
-                           R.pts.as<int32_t>() + r_lo, R.cap, r_hi - r_lo, d_tests);
-                hip_check(hipGetLastError(), "k_dense_vs_sorted");
-                hip_check(hipEventRecord(ctx->ev_f, s), "hipEventRecord");
-                d2h(nalive, cr.nalive, static_cast<size_t>(nsl), s, "slice survivor counts");
-                {
-                    float ms = 0.f;
-                    hip_check(hipEventElapsedTime(&ms, ctx->ev_e, ctx->ev_f), "hipEventElapsedTime");
-                    st->kernel_frontier_check_s += ms * 1e-3;
-                    st->frontier_check_launches += 1;
-                    st->frontier_check_bytes +=
-                        static_cast<int64_t>(d) * 4 * (n_in + (r_hi - r_lo)) + n_in + 4 * nsl;
-                }
-                std::vector<ChunkDesc> sl(nsl);
-                std::vector<int32_t> sln(nsl);
-                outpos.resize(nsl);
-                int64_t kept = 0;
-                for (int64_t k = 0; k < nsl; ++k) {
-                    sl[k].start = sl[k].out_base = k * kDenseSlice;
-                    sl[k].len = static_cast<int32_t>(std::min<int64_t>(kDenseSlice, n_in - k * kDenseSlice));
-                    sl[k].group = 0;
-                    sl[k].group_first = 0;
-                    sl[k].group_count = 1;
-                    sln[k] = sl[k].len;
-                    outpos[k] = kept;
-                    kept += nalive[k];
-                }
-                if (kept > 0 && kept < n_in) {
-                    upload_table(cr, sl);
-                    h2d(cr.nsurv, sln, s, "hipMemcpy(slice sizes)");
-                    int32_t *dst = (ys == ctx->sort_a.as<int32_t>()) ? ctx->sort_b.as<int32_t>()
-                                                                      : ctx->sort_a.as<int32_t>();
-                    run_compact(cr, nsl, outpos, ys, n_in, ctx->alive.as<uint8_t>(), dst, kept);
-                    ys = dst;
-                }
-                ytotal = kept;
-                r_lo = r_hi;
-            }
-            if (debug_join) {
-                std::fprintf(stderr, "join batch: nodes=%d products=%lld local=%lld after_R=%lld",
-                             Kn, static_cast<long long>(bprod), static_cast<long long>(xtotal),
-                             static_cast<long long>(ytotal));
-            }
+            run_compact(cb, cnt, outpos, ctx->surv_pts.as<int32_t>(), ytotal,
+                        ctx->alive.as<uint8_t>(), ctx->sort_a.as<int32_t>(), xtotal);
+            int32_t *ys = merge_runs(ctx->sort_a, ctx->sort_b, xtotal, run_start, grp_run, cnt);
+            ytotal = xtotal;
             // 4. descending-key blocks of the rest: local filter, earlier blocks
             int64_t btotal = 0;
             chunks.clear();
@@ -1378,7 +1407,6 @@ struct Run {
                 std::fprintf(stderr, " batch_frontier=%lld R=%lld\n",
                              static_cast<long long>(btotal), static_cast<long long>(r_count));
             }
-            add_time(ctx->ev_a, ctx->ev_d, &st->kernel_expand_td_s);
             add_time(ctx->ev_g, ctx->ev_c, &st->kernel_filter_s);
             add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
             st->filter_launches += 1;
@@ -1777,7 +1805,12 @@ int64_t estimate_bytes(const pmdd_dd_view &v, const pmdd_opts &o) {
     const int64_t arena = 2 * (8 * (N + L) + 4 * A + 4 * int64_t(v.d) * A);
     const int64_t cap = o.max_cand > 0 ? o.max_cand : PMDD_DEFAULT_MAX_CAND;
     const int64_t pts = std::min<int64_t>(cap, 256 * widest);
-    return arena + 3 * pts * (4 * int64_t(v.d) + 1) + (int64_t(16) << 20);
+    int64_t join = 0;
+    if (o.method == 3) {
+        // the two product buffers of a coupling batch
+        join = 2 * std::min<int64_t>(kJoinBatchCap, 4 * pts) * 4 * int64_t(v.d);
+    }
+    return arena + 3 * pts * (4 * int64_t(v.d) + 1) + join + (int64_t(16) << 20);
 }
 
 } // namespace

@@ -644,69 +644,6 @@ __global__ __launch_bounds__(kBlock) void k_count_chunks(const ChunkDesc *__rest
     }
 }
 
-// Coupling merge step.  Blocks [0, nblk_b): each new batch point b dies when
-// some running-frontier point r is >= b everywhere (equal points die: r holds
-// them already).  Blocks [nblk_b, ...): each running-frontier point r dies
-// when some batch point b is >= r everywhere and differs from it.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_vs_frontier(
-    const int32_t *__restrict__ B, int64_t bstride, int64_t nb, const int32_t *__restrict__ R,
-    int64_t rstride, int64_t nr, uint8_t *__restrict__ keep_b, uint8_t *__restrict__ keep_r,
-    int64_t nblk_b, unsigned long long *__restrict__ pair_tests) {
-    constexpr int DP = Pad<D>::DP;
-    __shared__ __align__(16) int32_t T[kBlock * DP];
-    const bool b_role = static_cast<int64_t>(blockIdx.x) < nblk_b;
-    const int64_t blk = b_role ? blockIdx.x : blockIdx.x - nblk_b;
-    const int32_t *mine = b_role ? B : R;
-    const int64_t mstride = b_role ? bstride : rstride;
-    const int64_t nm = b_role ? nb : nr;
-    const int32_t *other = b_role ? R : B;
-    const int64_t ostride = b_role ? rstride : bstride;
-    const int64_t no = b_role ? nr : nb;
-    const int64_t i = blk * kBlock + threadIdx.x;
-    const bool valid = i < nm;
-    int32_t m[D];
-#pragma unroll
-    for (int o = 0; o < D; ++o) {
-        m[o] = valid ? mine[o * mstride + i] : 0;
-    }
-    bool dead = !valid;
-    unsigned long long tests = 0;
-    for (int64_t tb = 0; tb < no; tb += kBlock) {
-        const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), no - tb));
-        if (static_cast<int>(threadIdx.x) < tn) {
-#pragma unroll
-            for (int o = 0; o < D; ++o) {
-                T[threadIdx.x * DP + o] = other[o * ostride + tb + threadIdx.x];
-            }
-        }
-        __syncthreads();
-        for (int j = 0; j < tn && !dead; ++j) {
-            const int32_t *q = &T[j * DP];
-            bool ge = true;
-            bool eq = true;
-#pragma unroll
-            for (int o = 0; o < D; ++o) {
-                ge &= q[o] >= m[o];
-                eq &= q[o] == m[o];
-            }
-            dead = ge && (b_role || !eq);
-            if (pair_tests != nullptr) {
-                ++tests;
-            }
-        }
-        if (!__syncthreads_or(!dead)) {
-            break;
-        }
-    }
-    if (valid) {
-        (b_role ? keep_b : keep_r)[i] = dead ? 0 : 1;
-    }
-    if (pair_tests != nullptr) {
-        atomicAdd(pair_tests, tests);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Big groups: runs in key order merged pairwise, one pass per level.
 //
@@ -934,101 +871,210 @@ __global__ __launch_bounds__(kBlock) void k_vs_sorted(
 }
 
 // ---------------------------------------------------------------------------
-// k_dense_vs_sorted: points of a dense array in descending key order die when
-// some point of R (descending key order) is >= them in every objective (equal
-// points die).  One wave per slice of 64*P consecutive points, so the early
-// exit waits for 64*P points only; each slice stops scanning R at the first
-// tile whose largest key is below the slice's smallest key.  Writes the alive
-// flag of every point and the alive count of every slice.
+// k_join_products: the n products of a coupling batch, written densely (SoA,
+// stride xstride) in product order.
 // ---------------------------------------------------------------------------
-template <int D, int P>
-__global__ __launch_bounds__(kWave) void k_dense_vs_sorted(
-    const int32_t *__restrict__ X, int64_t xstride, int64_t n, uint8_t *__restrict__ alive,
-    int32_t *__restrict__ slice_alive, const int32_t *__restrict__ R, int64_t rstride, int64_t nr,
-    unsigned long long *__restrict__ pair_tests) {
-    constexpr int DP = Pad<D>::DP;
-    __shared__ __align__(16) int32_t T[kWave * DP];
-    __shared__ unsigned long long min_word;
-    const int tid = threadIdx.x;
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kWave * P;
-    int32_t m[P][D];
-    bool dead[P];
-    int64_t my_min = INT64_MAX;
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        const int64_t i = base + p * kWave + tid;
-        dead[p] = i >= n;
-        int64_t k = 0;
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_join_products(JoinSrc src, int64_t n,
+                                                          int32_t *__restrict__ X,
+                                                          int64_t xstride) {
+    ChunkDesc c{};
+    const int64_t step = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; q < n; q += step) {
+        int32_t p[D];
+        load_candidate<D>(src, c, q, p);
 #pragma unroll
         for (int o = 0; o < D; ++o) {
-            m[p][o] = dead[p] ? INT32_MAX : X[o * xstride + i];
-            k += m[p][o];
-        }
-        if (!dead[p] && k < my_min) {
-            my_min = k;
+            X[o * xstride + q] = p[o];
         }
     }
-    if (tid == 0) {
-        min_word = ~0ULL;
+}
+
+// ---------------------------------------------------------------------------
+// k_refill_vs_block: points X[0 .. *n_in) die when some point of the block
+// R[lo .. hi) (descending key order) is >= them in every objective (equal
+// points die).  Survivors are appended to Y (order not kept) and counted in
+// *n_out.
+//
+// One wave per workgroup.  Each lane holds one point; the wave walks the
+// block round-robin in sub-tiles of kSub rows (64-row tiles staged in LDS, so
+// every lane reads the same row, a broadcast), and after every sub-tile a lane
+// whose point has died, or has met the whole block since it was taken, takes
+// the next point of the wave's stream.  One hard point therefore never holds
+// 63 finished lanes, which is what a check in lockstep over fixed groups pays:
+// most points meet a dominator within a few rows, a few need thousands.
+// Wave w's stream is the 64-point chunks w, w + G, w + 2G, ... (G waves), so
+// taking points needs no atomics; survivors are staged in LDS and written
+// out kStage at a time with one atomic each.
+// A point whose key is above the block's largest key meets no test: no point
+// of the block can be >= it.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kWave) void k_refill_vs_block(
+    const int32_t *__restrict__ X, int64_t xstride, const unsigned long long *__restrict__ n_in,
+    const int32_t *__restrict__ R, int64_t rstride, int64_t lo, int64_t hi,
+    int32_t *__restrict__ Y, int64_t ystride, unsigned long long *__restrict__ n_out,
+    unsigned long long *__restrict__ pair_tests) {
+    constexpr int DP = Pad<D>::DP;
+    constexpr int kSub = 16;                  // rows between two refills
+    constexpr int kSubPerTile = kWave / kSub; // sub-tiles per staged tile
+    constexpr int kStage = 128;               // survivors staged per flush
+    __shared__ __align__(16) int32_t T[kWave * DP];
+    __shared__ int32_t SB[kStage * D];
+    const int lane = threadIdx.x;
+    const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (kWave - lane));
+    const int64_t n = static_cast<int64_t>(*n_in);
+    const int64_t G = gridDim.x;
+    const int64_t w = blockIdx.x;
+    if (w * kWave >= n) {
+        return;
     }
-    __syncthreads();
-    if (my_min != INT64_MAX) {
-        atomicMin(&min_word, static_cast<unsigned long long>(my_min + (int64_t(1) << 62)));
+    const int64_t len = hi - lo;
+    const int nsub = static_cast<int>((len + kSub - 1) / kSub);
+    const int ntile = static_cast<int>((len + kWave - 1) / kWave);
+    const int64_t top_key = key_at<D>(R, rstride, lo);
+    int sub = static_cast<int>(w % ntile) * kSubPerTile;
+    if (sub >= nsub) {
+        sub = 0;
     }
-    __syncthreads();
-    const bool any = min_word != ~0ULL;
-    const int64_t wg_min = any ? static_cast<int64_t>(min_word) - (int64_t(1) << 62) : 0;
+    int loaded = -1;       // tile staged in T
+    int64_t t_next = 0;    // next position in the wave's stream
+    bool exhausted = false;
+    bool holds = false;    // lane holds a point
+    int left = 0;          // sub-tiles the lane's point has still to meet
+    int32_t m[D];
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+        m[o] = 0;
+    }
+    int sb = 0;
     unsigned long long tests = 0;
-    for (int64_t tb = 0; any && tb < nr; tb += kWave) {
-        if (key_at<D>(R, rstride, tb) < wg_min) {
-            break;
+
+    auto flush = [&]() {
+        __syncthreads();
+        unsigned long long base = 0;
+        if (lane == 0) {
+            base = atomicAdd(n_out, static_cast<unsigned long long>(sb));
         }
-        const int tn = static_cast<int>(min(static_cast<int64_t>(kWave), nr - tb));
-        if (tid < tn) {
+        base = __shfl(base, 0);
+        for (int i = lane; i < sb; i += kWave) {
 #pragma unroll
             for (int o = 0; o < D; ++o) {
-                T[tid * DP + o] = R[o * rstride + tb + tid];
+                Y[o * ystride + static_cast<int64_t>(base) + i] = SB[i * D + o];
             }
         }
         __syncthreads();
-        for (int j = 0; j < tn; ++j) {
-            const int32_t *q = &T[j * DP];
+        sb = 0;
+    };
+    auto stage = [&](bool flag) {
+        const uint64_t mask = __ballot(flag);
+        if (mask == 0) {
+            return;
+        }
+        const int cnt = __popcll(mask);
+        if (sb + cnt > kStage) {
+            flush();
+        }
+        if (flag) {
+            const int pos = sb + __popcll(mask & below);
 #pragma unroll
-            for (int p = 0; p < P; ++p) {
-                dead[p] |= ge_all<D, DP>(q, m[p]);
+            for (int o = 0; o < D; ++o) {
+                SB[pos * D + o] = m[o];
             }
         }
-        if (pair_tests != nullptr) {
-            tests += static_cast<unsigned long long>(P) * tn;
-        }
-        bool any_live = false;
+        sb += cnt;
+    };
+
+    while (true) {
+        // refill lanes without a point; points above the block's top key pass
+        while (!exhausted) {
+            const bool need = !holds;
+            const uint64_t mask = __ballot(need);
+            if (mask == 0) {
+                break;
+            }
+            bool pass = false;
+            bool past = false;
+            if (need) {
+                const int64_t t = t_next + __popcll(mask & below);
+                const int64_t q = (w + (t >> 6) * G) * kWave + (t & (kWave - 1));
+                if (q < n) {
+                    int64_t k = 0;
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            any_live |= !dead[p];
+                    for (int o = 0; o < D; ++o) {
+                        m[o] = X[o * xstride + q];
+                        k += m[o];
+                    }
+                    if (k > top_key) {
+                        pass = true;
+                    } else {
+                        holds = true;
+                        left = nsub;
+                    }
+                } else {
+                    past = true;
+                }
+            }
+            t_next += __popcll(mask);
+            exhausted = __ballot(past) != 0;
+            stage(pass);
         }
-        if (!__syncthreads_or(any_live)) {
+        if (__ballot(holds) == 0) {
             break;
         }
-    }
-    int kept = 0;
+        const int tile = sub / kSubPerTile;
+        if (tile != loaded) {
+            __syncthreads();
+            const int64_t tb = lo + static_cast<int64_t>(tile) * kWave;
+            if (tb + lane < hi) {
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-        const int64_t i = base + p * kWave + tid;
-        if (i < n) {
-            alive[i] = dead[p] ? 0 : 1;
-            kept += dead[p] ? 0 : 1;
+                for (int o = 0; o < D; ++o) {
+                    T[lane * DP + o] = R[o * rstride + tb + lane];
+                }
+            }
+            __syncthreads();
+            loaded = tile;
         }
-    }
-    // wave-wide sum of kept
+        const int j0 = (sub - tile * kSubPerTile) * kSub;
+        const int je = static_cast<int>(min(static_cast<int64_t>(j0 + kSub),
+                                            hi - lo - static_cast<int64_t>(tile) * kWave));
+        bool dead = false;
+        if (holds) {
+            if (je - j0 == kSub) {
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        kept += __shfl_xor(kept, off);
+                for (int j = 0; j < kSub; ++j) {
+                    dead |= ge_all<D, DP>(&T[(j0 + j) * DP], m);
+                }
+            } else {
+                for (int j = j0; j < je; ++j) {
+                    dead |= ge_all<D, DP>(&T[j * DP], m);
+                }
+            }
+            tests += static_cast<unsigned long long>(je - j0);
+        }
+        bool survive = false;
+        if (holds) {
+            --left;
+            if (dead) {
+                holds = false;
+            } else if (left == 0) {
+                survive = true;
+                holds = false;
+            }
+        }
+        stage(survive);
+        sub = (sub + 1 == nsub) ? 0 : sub + 1;
     }
-    if (tid == 0) {
-        slice_alive[blockIdx.x] = kept;
+    if (sb > 0) {
+        flush();
     }
     if (pair_tests != nullptr) {
-        atomicAdd(pair_tests, tests);
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            tests += __shfl_xor(tests, off);
+        }
+        if (lane == 0) {
+            atomicAdd(pair_tests, tests);
+        }
     }
 }
 

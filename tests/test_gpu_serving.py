@@ -16,7 +16,10 @@ def _same(a, b):
 
 @pytest.mark.parametrize("method", [1, 3])
 def test_mixed_stream_alone_vs_in_the_mix(engine, tmp_path, method):
-    specs = serving.mixed_stream(24, seed=11)
+    # the serving mix, capped in size so that the test stays short
+    specs = [sp for sp in serving.mixed_stream(80, seed=11)
+             if (sp["kind"] == 1 and sp["n"] <= 45) or (sp["kind"] == 2 and sp["n"] <= 100)
+             or sp["kind"] == 3][:24]
     paths = serving.write_stream(specs, str(tmp_path))
     alone = []
     for sp, p in zip(specs, paths):
