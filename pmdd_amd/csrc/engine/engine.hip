@@ -122,7 +122,7 @@ struct pmdd_ctx {
     DevBuf run_off, which, which_a, which_b, which_c;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
-    DevBuf join_idx, join_x, join_y, join_ctr;
+    DevBuf join_idx, join_x, join_y, join_ctr, front_count;
 };
 
 struct LayerCSR {
@@ -434,8 +434,11 @@ using FilterFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *,
                           int32_t *, int64_t, uint8_t *, int32_t *, int32_t *, int64_t *,
                           int64_t *, unsigned long long *);
 using CrossFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
-                         int64_t, uint8_t *, const int32_t *, int32_t *, const int64_t *,
-                         const int64_t *, unsigned long long *);
+                         int64_t, uint8_t *, const int32_t *, int32_t *, const int32_t *, int64_t,
+                         const int64_t *, int, unsigned long long *);
+using AppendFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
+                          int64_t, const uint8_t *, const int32_t *, const int32_t *, int32_t *,
+                          int64_t, const int64_t *, int64_t *, int);
 using VsSortedFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *,
                             const int32_t *, int64_t, uint8_t *, const int32_t *, int32_t *,
                             const int32_t *, int64_t, int64_t, unsigned long long *);
@@ -447,6 +450,7 @@ struct Launch {
     FilterFn filter_sorted;  // SortedSrc, source already in key order
     FilterFn filter_dense;   // SortedSrc in any order, presorted in LDS
     CrossFn cross;           // chunks of a group are descending key blocks
+    AppendFn append_front;   // a wave's survivors onto their group's front
     VsSortedFn vs_weak;
     VsSortedFn vs_strict;
     void (*compact)(dim3, hipStream_t, const ChunkDesc *, const int32_t *, const int64_t *,
@@ -485,9 +489,17 @@ template <int D> struct LaunchImpl {
     }
     static void cross(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
-                      int32_t *na, const int64_t *, const int64_t *, unsigned long long *pt) {
+                      int32_t *na, const int32_t *fv, int64_t fvs, const int64_t *fc, int W,
+                      unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_cross_chunks<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
-                           stride, alive, ns, na, pt);
+                           stride, alive, ns, na, fv, fvs, fc, W, pt);
+    }
+    static void append_front(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
+                             const int32_t *pts, int64_t stride, const uint8_t *alive,
+                             const int32_t *ns, const int32_t *na, int32_t *fv, int64_t fvs,
+                             const int64_t *fin, int64_t *fout, int W) {
+        hipLaunchKernelGGL((pmdd_k::k_append_front<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
+                           stride, alive, ns, na, fv, fvs, fin, fout, W);
     }
     template <bool STRICT>
     static void vs(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
@@ -546,6 +558,7 @@ template <int D> struct LaunchImpl {
         L.filter_join = filter<pmdd_k::JoinSrc, false>;
         L.filter_sorted = filter<pmdd_k::SortedSrc, true>;
         L.cross = cross;
+        L.append_front = append_front;
         L.vs_weak = vs<false>;
         L.vs_strict = vs<true>;
         L.compact = compact;
@@ -677,11 +690,15 @@ struct Run {
     }
 
     // One-sided block filter over every group with more than one chunk (the
-    // chunks of a group are descending-key blocks), launched in waves of
-    // kWave chunks per group so that earlier chunks are final when read.
-    static constexpr int kCrossWave = 8;
+    // chunks of a group are descending-key blocks), in waves of kCrossWave
+    // chunks per group: a wave's chunks are checked against the group's front
+    // (the survivors of its earlier waves, dense in key order, kept in fv at
+    // the group's first candidate slot) and against the earlier chunks of the
+    // same wave; the wave's survivors are then appended to the front.  `fv`
+    // needs room for every candidate of the groups (stride fv_stride).
+    static constexpr int kCrossWave = 16;
     void run_cross_waves(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks, const int32_t *pts,
-                         int64_t stride, uint8_t *alive) {
+                         int64_t stride, uint8_t *alive, int32_t *fv, int64_t fv_stride) {
         int32_t max_count = 1;
         for (const ChunkDesc &c : chunks) {
             max_count = std::max(max_count, c.group_count);
@@ -690,10 +707,11 @@ struct Run {
             return;
         }
         const int waves = static_cast<int>(ceil_div(max_count, kCrossWave));
+        // per wave: chunks to check (not a group's first), then every chunk
         std::vector<int32_t> lists;
-        std::vector<size_t> start(waves + 1, 0);
+        std::vector<size_t> cstart(waves + 1, 0), astart(waves + 1, 0);
         for (int w = 0; w < waves; ++w) {
-            start[w] = lists.size();
+            cstart[w] = lists.size();
             for (size_t i = 0; i < chunks.size(); ++i) {
                 const ChunkDesc &c = chunks[i];
                 const int k = static_cast<int>(i) - c.group_first;
@@ -701,18 +719,45 @@ struct Run {
                     lists.push_back(static_cast<int32_t>(i));
                 }
             }
-        }
-        start[waves] = lists.size();
-        h2d(ctx->which_c, lists, s, "hipMemcpy(wave lists)");
-        for (int w = 0; w < waves; ++w) {
-            const size_t n = start[w + 1] - start[w];
-            if (n == 0) {
-                continue;
+            astart[w] = lists.size();
+            for (size_t i = 0; i < chunks.size(); ++i) {
+                const ChunkDesc &c = chunks[i];
+                const int k = static_cast<int>(i) - c.group_first;
+                if (c.group_count > 1 && k / kCrossWave == w) {
+                    lists.push_back(static_cast<int32_t>(i));
+                }
             }
-            K.cross(dim3(static_cast<unsigned>(n)), s, ctx->which_c.as<int32_t>() + start[w],
-                    cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(),
-                    cb.nalive.as<int32_t>(), nullptr, nullptr, d_tests);
-            hip_check(hipGetLastError(), "k_cross_chunks");
+        }
+        cstart[waves] = lists.size();
+        h2d(ctx->which_c, lists, s, "hipMemcpy(wave lists)");
+        // front counts, double-buffered by wave, indexed by a group's first chunk
+        const size_t nc = chunks.size();
+        ctx->front_count.ensure(2 * nc * sizeof(int64_t), "hipMalloc(front counts)");
+        int64_t *fc = ctx->front_count.as<int64_t>();
+        hipLaunchKernelGGL(pmdd_k::k_fill_u8,
+                           dim3(static_cast<unsigned>(ceil_div(2 * nc * sizeof(int64_t), 256))),
+                           dim3(256), 0, s, ctx->front_count.as<uint8_t>(),
+                           static_cast<int64_t>(2 * nc * sizeof(int64_t)), uint8_t(0));
+        hip_check(hipGetLastError(), "k_fill_u8");
+        const int32_t *wl = ctx->which_c.as<int32_t>();
+        for (int w = 0; w < waves; ++w) {
+            int64_t *fin = fc + (w % 2) * nc;
+            int64_t *fout = fc + ((w + 1) % 2) * nc;
+            const size_t n = astart[w] - cstart[w];
+            if (n > 0) {
+                K.cross(dim3(static_cast<unsigned>(n)), s, wl + cstart[w], cb.chunks.as<ChunkDesc>(),
+                        pts, stride, alive, cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), fv,
+                        fv_stride, fin, kCrossWave, d_tests);
+                hip_check(hipGetLastError(), "k_cross_chunks");
+            }
+            const size_t na = ((w + 1 < waves) ? cstart[w + 1] : lists.size()) - astart[w];
+            if (w + 1 < waves && na > 0) {
+                K.append_front(dim3(static_cast<unsigned>(na)), s, wl + astart[w],
+                               cb.chunks.as<ChunkDesc>(), pts, stride, alive,
+                               cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), fv, fv_stride,
+                               fin, fout, kCrossWave);
+                hip_check(hipGetLastError(), "k_append_front");
+            }
         }
     }
 
@@ -897,7 +942,7 @@ struct Run {
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
             if (!which_cross.empty()) {
                 run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), pass_c,
-                                ctx->alive.as<uint8_t>());
+                                ctx->alive.as<uint8_t>(), sorted, bigpos);
             }
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
             d2h(nalive, cb.nalive, chunks.size(), s, "chunk survivor counts");
@@ -1391,7 +1436,7 @@ struct Run {
                            ctx->surv_pts.as<int32_t>(), ytotal, ctx->alive.as<uint8_t>());
                 hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
                 run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), ytotal,
-                                ctx->alive.as<uint8_t>());
+                                ctx->alive.as<uint8_t>(), ys, ytotal);
                 d2h(nalive, cb.nalive, chunks.size(), s, "batch survivor counts");
                 outpos.resize(chunks.size());
                 for (size_t c = 0; c < chunks.size(); ++c) {

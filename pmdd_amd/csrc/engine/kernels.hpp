@@ -356,53 +356,169 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
     }
 }
 
+// Block-wide sum of one int per thread (one LDS slot per wave).
+__device__ __forceinline__ int block_sum(int v, int *wave_tot) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        wave_tot[wave] = v;
+    }
+    __syncthreads();
+    int sum = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        sum += wave_tot[w];
+    }
+    return sum;
+}
+
 // ---------------------------------------------------------------------------
 // k_cross_chunks: the chunks of a group are consecutive blocks of descending
-// keys; every alive point of a chunk dies when an alive point of an earlier
-// chunk of its group is >= it in every objective.  One workgroup per chunk
-// listed in `which`.  Launched in waves of consecutive chunks so that the
-// earlier chunks are mostly final already; tiles keep only alive points.
-// Flags of chunks in the same wave may change while they are read: either
-// value is correct (a point killed by cross checks is itself dominated by an
-// earlier point that also dominates whatever it dominates).
+// keys, taken in waves of W chunks; every alive point of a chunk dies when it
+// is weakly dominated by
+//   (a) a point of the group's front: the final survivors of the earlier
+//       waves, kept densely in key order at fv[fbase ..] (count
+//       fcount[group_first], filled by k_append_front), or
+//   (b) an alive point of an earlier chunk of the same wave.
+// One workgroup per chunk listed in `which`.  Flags of chunks in the same
+// wave may change while they are read: either value is correct (a point
+// killed by cross checks is itself dominated by an earlier point that also
+// dominates whatever it dominates).
+// The chunk's own live points are repacked into fewer rows per thread
+// whenever they fit: most points meet a dominator early, and the few that
+// scan the whole front then do so without the dead ones alongside.
 // ---------------------------------------------------------------------------
 template <int D, int CH>
 __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
     const int32_t *__restrict__ pts, int64_t stride, uint8_t *alive,
     const int32_t *__restrict__ chunk_nsurv, int32_t *__restrict__ chunk_nalive,
-    unsigned long long *__restrict__ pair_tests) {
+    const int32_t *__restrict__ fv, int64_t fv_stride, const int64_t *__restrict__ fcount,
+    int W, unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
-    constexpr int R = CH / kBlock; // points held per thread
-    __shared__ __align__(16) int32_t T[kBlock * DP];
+    constexpr int R = CH / kBlock; // rows of points per thread
+    constexpr int kU = (kBlock * DP > CH * D) ? kBlock * DP : CH * D;
+    __shared__ __align__(16) int32_t U[kU]; // tile of earlier points, or repack buffer
+    __shared__ int16_t PS[CH];              // repacked slots
     __shared__ int wave_tot[kWaves];
-    __shared__ int live_count;
 
     const int me = which[blockIdx.x];
     const ChunkDesc c = chunks[me];
     const int nS = chunk_nsurv[me];
     const int tid = threadIdx.x;
     const volatile uint8_t *valive = alive;
+    const int k_in = me - c.group_first;
+    const int wave_first = c.group_first + (k_in / W) * W;
+    const int64_t fbase = c.start - static_cast<int64_t>(k_in) * CH;
+    const int64_t fn = fcount[c.group_first];
 
     int32_t m[R][D];
     bool dead[R];
+    int slot[R]; // -1: no point
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int s = r * kBlock + tid;
+        slot[r] = s < nS ? s : -1;
         dead[r] = s >= nS || valive[c.out_base + s] == 0;
 #pragma unroll
         for (int o = 0; o < D; ++o) {
             m[r][o] = dead[r] ? INT32_MAX : pts[o * stride + c.out_base + s];
         }
     }
+    int rows = R;
     unsigned long long tests = 0;
-    bool any_live = false;
+    int live = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        any_live |= !dead[r];
+        live += dead[r] ? 0 : 1;
     }
-    bool go = __syncthreads_or(any_live);
-    for (int other = c.group_first; go && other < me; ++other) {
+    int total = block_sum(live, wave_tot);
+    bool go = total > 0;
+
+    // tn staged rows in U against the live points; then count and repack
+    auto test_tile = [&](int tn) {
+        for (int j = 0; j < tn; ++j) {
+            const int32_t *row = &U[j * DP];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r < rows) {
+                    dead[r] |= ge_all<D, DP>(row, m[r]);
+                }
+            }
+        }
+        if (pair_tests != nullptr) {
+            int held = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                held += (r < rows && slot[r] >= 0) ? 1 : 0;
+            }
+            tests += static_cast<unsigned long long>(held) * tn;
+        }
+        live = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            live += (r < rows && !dead[r]) ? 1 : 0;
+        }
+        total = block_sum(live, wave_tot); // its barriers also end the tile's reads
+        go = total > 0;
+        const int need = (total + kBlock - 1) / kBlock;
+        if (go && need < rows) {
+            // repack the live points into `need` rows; dropped points die now
+            int base = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r < rows) {
+                    if (dead[r] && slot[r] >= 0) {
+                        alive[c.out_base + slot[r]] = 0;
+                    }
+                    int cnt = 0;
+                    const int rk = block_rank(!dead[r], wave_tot, &cnt);
+                    if (!dead[r]) {
+#pragma unroll
+                        for (int o = 0; o < D; ++o) {
+                            U[(base + rk) * D + o] = m[r][o];
+                        }
+                        PS[base + rk] = static_cast<int16_t>(slot[r]);
+                    }
+                    base += cnt;
+                    __syncthreads(); // wave_tot is reused by the next rank
+                }
+            }
+            rows = need;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int s = r * kBlock + tid;
+                const bool v = r < rows && s < total;
+                slot[r] = v ? PS[s] : -1;
+                dead[r] = !v;
+#pragma unroll
+                for (int o = 0; o < D; ++o) {
+                    m[r][o] = v ? U[s * D + o] : INT32_MAX;
+                }
+            }
+            __syncthreads();
+        }
+    };
+
+    // (a) the front, dense
+    for (int64_t tb = 0; go && tb < fn; tb += kBlock) {
+        const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), fn - tb));
+        if (tid < tn) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                U[tid * DP + o] = fv[o * fv_stride + fbase + tb + tid];
+            }
+        }
+        __syncthreads();
+        test_tile(tn);
+    }
+    // (b) earlier chunks of this wave, alive points only
+    for (int other = wave_first; go && other < me; ++other) {
         const ChunkDesc oc = chunks[other];
         const int on = chunk_nsurv[other];
         for (int tb = 0; go && tb < on; tb += kBlock) {
@@ -414,61 +530,81 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
                 q[o] = a ? pts[o * stride + oc.out_base + i] : 0;
             }
             int tn = 0;
-            const int slot = block_rank(a, wave_tot, &tn);
+            const int pos = block_rank(a, wave_tot, &tn);
             if (a) {
 #pragma unroll
                 for (int o = 0; o < D; ++o) {
-                    T[slot * DP + o] = q[o];
+                    U[pos * DP + o] = q[o];
                 }
             }
             __syncthreads();
-            for (int j = 0; j < tn; ++j) {
-                const int32_t *row = &T[j * DP];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    dead[r] |= ge_all<D, DP>(row, m[r]);
-                }
-            }
-            if (pair_tests != nullptr) {
-                int live = 0;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    live += (r * kBlock + tid < nS) ? 1 : 0;
-                }
-                tests += static_cast<unsigned long long>(live) * tn;
-            }
-            any_live = false;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                any_live |= !dead[r];
-            }
-            go = __syncthreads_or(any_live);
+            test_tile(tn);
         }
     }
 
     int kept = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int s = r * kBlock + tid;
-        if (s < nS) {
+        if (r < rows && slot[r] >= 0) {
             if (dead[r]) {
-                alive[c.out_base + s] = 0;
+                alive[c.out_base + slot[r]] = 0;
             } else {
                 ++kept;
             }
         }
     }
+    kept = block_sum(kept, wave_tot);
     if (tid == 0) {
-        live_count = 0;
-    }
-    __syncthreads();
-    atomicAdd(&live_count, kept);
-    __syncthreads();
-    if (tid == 0) {
-        chunk_nalive[me] = live_count;
+        chunk_nalive[me] = kept;
     }
     if (pair_tests != nullptr) {
         atomicAdd(pair_tests, tests);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_append_front: after the cross check of a wave, the alive points of the
+// wave's chunks (listed in `which`, every chunk of the wave) are appended in
+// chunk and slot order, so in key order, to their group's front
+// fv[fbase + fcount_in[g] ..]; the wave's last chunk of each group writes the
+// new count to fcount_out[g] (g = group_first).
+// ---------------------------------------------------------------------------
+template <int D, int CH>
+__global__ __launch_bounds__(kBlock) void k_append_front(
+    const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
+    const int32_t *__restrict__ pts, int64_t stride, const uint8_t *__restrict__ alive,
+    const int32_t *__restrict__ chunk_nsurv, const int32_t *__restrict__ chunk_nalive,
+    int32_t *__restrict__ fv, int64_t fv_stride, const int64_t *__restrict__ fcount_in,
+    int64_t *__restrict__ fcount_out, int W) {
+    __shared__ int wave_tot[kWaves];
+    const int me = which[blockIdx.x];
+    const ChunkDesc c = chunks[me];
+    const int k_in = me - c.group_first;
+    const int wave_first = c.group_first + (k_in / W) * W;
+    const int wave_last = min(wave_first + W, c.group_first + c.group_count) - 1;
+    const int64_t fbase = c.start - static_cast<int64_t>(k_in) * CH;
+    int64_t at = fcount_in[c.group_first];
+    for (int other = wave_first; other < me; ++other) {
+        at += chunk_nalive[other];
+    }
+    const int nS = chunk_nsurv[me];
+    int written = 0;
+    for (int tb = 0; tb < nS; tb += kBlock) {
+        const int s = tb + threadIdx.x;
+        const bool a = s < nS && alive[c.out_base + s] != 0;
+        int cnt = 0;
+        const int rk = block_rank(a, wave_tot, &cnt);
+        if (a) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                fv[o * fv_stride + fbase + at + written + rk] = pts[o * stride + c.out_base + s];
+            }
+        }
+        written += cnt;
+        __syncthreads(); // wave_tot is reused by the next rank
+    }
+    if (me == wave_last && threadIdx.x == 0) {
+        fcount_out[c.group_first] = at + written;
     }
 }
 
