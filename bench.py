@@ -15,6 +15,16 @@ timed steps (sum of final frontier sizes / wall time).  `roofline` covers the
 kernel with the most device time; `cpu_baseline` times the CPU restatement of
 the reference's OpenMP enumerators on a bounded sample on this host.
 
+`serving` replays a seeded prefix of the mixed serving stream (knapsack,
+set packing and TSP requests of uneven size, d in {3,4,5}; --serving-requests,
+2000 for the whole stream) through the plain one-at-a-time path (one slot)
+and the batched path (--serving-slots worker slots with their own streams,
+memory-based admission): first with every request queued at t=0 (capacity),
+then with Poisson arrivals at the plain path's capacity (latency), reporting
+instances/s, Pareto points/s and p50/p99 latency from arrival to result
+(host diagram construction included).  Every request's frontier must be the
+same in the plain and the batched runs.
+
 Profiling helpers (for rocprofv3 runs): --only NAME and --cases M:SD,... run a
 subset of the step; --calib-copy-mib N runs one plain device copy of N MiB
 (torch) and exits, to scale the FETCH_SIZE / WRITE_SIZE counters.
@@ -94,6 +104,45 @@ def cpu_baseline(dds, threads):
     }
 
 
+def serving_leg(tmpdir, requests, slots, method):
+    """Plain (one slot) vs batched (`slots` slots) on the first `requests`
+    requests of the serving stream."""
+    from pmdd_amd import serving
+    specs = serving.mixed_stream(requests)
+    paths = serving.write_stream(specs, os.path.join(tmpdir, "stream"))
+    keys = ("inst_per_s", "pareto_pts_per_s", "p50_latency_s", "p99_latency_s",
+            "mean_latency_s", "span_s", "failed")
+    out = {"requests": requests, "method": method, "slots": slots,
+           "mix": "60% knapsack n~U[20,70], 25% set packing n~U[60,150] m=n/5, "
+                  "15% TSP n~U[8,11]; d~U{3,4,5}; seeded SplitMix64"}
+    res_plain, res_batched = {}, {}
+    t0 = time.perf_counter()
+    closed = [0.0] * requests
+    plain = serving.run_stream(paths, specs, closed, slots=1, method=method, results=res_plain)
+    log(f"serving: plain closed loop {plain['inst_per_s']:.2f} inst/s")
+    batched = serving.run_stream(paths, specs, closed, slots=slots, method=method,
+                                 results=res_batched)
+    log(f"serving: batched closed loop {batched['inst_per_s']:.2f} inst/s")
+    same = all(i in res_plain and i in res_batched and
+               res_plain[i].shape == res_batched[i].shape and
+               bool((res_plain[i] == res_batched[i]).all()) for i in range(requests))
+    lam = plain["inst_per_s"]
+    arrivals = serving.poisson_arrivals(requests, lam)
+    plain_o = serving.run_stream(paths, specs, arrivals, slots=1, method=method)
+    log("serving: plain open loop done")
+    batched_o = serving.run_stream(paths, specs, arrivals, slots=slots, method=method)
+    log("serving: batched open loop done")
+    out["closed_loop"] = {"plain": {k: plain[k] for k in keys},
+                          "batched": {k: batched[k] for k in keys}}
+    out["poisson"] = {"rate_per_s": lam,
+                      "plain": {k: plain_o[k] for k in keys},
+                      "batched": {k: batched_o[k] for k in keys}}
+    out["batched_over_plain_inst_per_s"] = batched["inst_per_s"] / max(plain["inst_per_s"], 1e-12)
+    out["bit_exact_batched_vs_plain"] = same
+    out["wall_s"] = time.perf_counter() - t0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +155,10 @@ def main():
     ap.add_argument("--only", default="", help="run only this workload instance")
     ap.add_argument("--cases", default="", help="method:state_dominance list, e.g. 3:0,1:0")
     ap.add_argument("--calib-copy-mib", type=int, default=0)
+    ap.add_argument("--serving-requests", type=int, default=100,
+                    help="requests of the serving stream to replay (0: skip; 2000: all)")
+    ap.add_argument("--serving-slots", type=int, default=8)
+    ap.add_argument("--serving-method", type=int, default=1)
     args = ap.parse_args()
     if args.calib_copy_mib:
         import torch
@@ -232,6 +285,12 @@ def main():
         cpu = cpu_baseline(dds, threads)
         log(f"cpu baseline done: {cpu['value']:.1f} pts/s on {threads} threads")
 
+    srv = None
+    if args.serving_requests > 0:
+        srv = serving_leg(tmp, args.serving_requests, args.serving_slots, args.serving_method)
+        if not srv["bit_exact_batched_vs_plain"]:
+            raise SystemExit("serving: batched and plain frontiers differ")
+
     line = {
         "metric": "exact Pareto pts/s, inst/s, p50/p99 latency: batched vs plain GPU",
         "value": total_pts / elapsed,
@@ -258,6 +317,7 @@ def main():
         "pareto_points_per_step": total_pts / max(args.steps, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": srv,
     }
     print(json.dumps(line), flush=True)
     for _, r in res:
