@@ -434,11 +434,11 @@ using FilterFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *,
                           int32_t *, int64_t, uint8_t *, int32_t *, int32_t *, int64_t *,
                           int64_t *, unsigned long long *);
 using CrossFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
-                         int64_t, uint8_t *, const int32_t *, int32_t *, const int32_t *, int64_t,
-                         const int64_t *, int, unsigned long long *);
+                         int64_t, uint8_t *, const int32_t *, const int32_t *, int64_t,
+                         const int64_t *, int, int, unsigned long long *);
 using AppendFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
-                          int64_t, const uint8_t *, const int32_t *, const int32_t *, int32_t *,
-                          int64_t, const int64_t *, int64_t *, int);
+                          int64_t, const uint8_t *, const int32_t *, int32_t *, int32_t *, int64_t,
+                          const int64_t *, int64_t *, int);
 using VsSortedFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *,
                             const int32_t *, int64_t, uint8_t *, const int32_t *, int32_t *,
                             const int32_t *, int64_t, int64_t, unsigned long long *);
@@ -489,14 +489,14 @@ template <int D> struct LaunchImpl {
     }
     static void cross(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
-                      int32_t *na, const int32_t *fv, int64_t fvs, const int64_t *fc, int W,
+                      const int32_t *fv, int64_t fvs, const int64_t *fc, int W, int G,
                       unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_cross_chunks<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
-                           stride, alive, ns, na, fv, fvs, fc, W, pt);
+                           stride, alive, ns, fv, fvs, fc, W, G, pt);
     }
     static void append_front(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                              const int32_t *pts, int64_t stride, const uint8_t *alive,
-                             const int32_t *ns, const int32_t *na, int32_t *fv, int64_t fvs,
+                             const int32_t *ns, int32_t *na, int32_t *fv, int64_t fvs,
                              const int64_t *fin, int64_t *fout, int W) {
         hipLaunchKernelGGL((pmdd_k::k_append_front<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
                            stride, alive, ns, na, fv, fvs, fin, fout, W);
@@ -697,6 +697,7 @@ struct Run {
     // same wave; the wave's survivors are then appended to the front.  `fv`
     // needs room for every candidate of the groups (stride fv_stride).
     static constexpr int kCrossWave = 16;
+    static constexpr int64_t kCrossSplit = 16; // most workgroups per chunk
     void run_cross_waves(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks, const int32_t *pts,
                          int64_t stride, uint8_t *alive, int32_t *fv, int64_t fv_stride) {
         int32_t max_count = 1;
@@ -745,13 +746,16 @@ struct Run {
             int64_t *fout = fc + ((w + 1) % 2) * nc;
             const size_t n = astart[w] - cstart[w];
             if (n > 0) {
-                K.cross(dim3(static_cast<unsigned>(n)), s, wl + cstart[w], cb.chunks.as<ChunkDesc>(),
-                        pts, stride, alive, cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), fv,
-                        fv_stride, fin, kCrossWave, d_tests);
+                // spread each chunk's scan over G workgroups, about 1024 in all
+                const int G = static_cast<int>(
+                    std::min<int64_t>(kCrossSplit, std::max<int64_t>(1, ceil_div(1024, n))));
+                K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + cstart[w],
+                        cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(), fv,
+                        fv_stride, fin, kCrossWave, G, d_tests);
                 hip_check(hipGetLastError(), "k_cross_chunks");
             }
             const size_t na = ((w + 1 < waves) ? cstart[w + 1] : lists.size()) - astart[w];
-            if (w + 1 < waves && na > 0) {
+            if (na > 0) {
                 K.append_front(dim3(static_cast<unsigned>(na)), s, wl + astart[w],
                                cb.chunks.as<ChunkDesc>(), pts, stride, alive,
                                cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), fv, fv_stride,

@@ -385,21 +385,23 @@ __device__ __forceinline__ int block_sum(int v, int *wave_tot) {
 //       waves, kept densely in key order at fv[fbase ..] (count
 //       fcount[group_first], filled by k_append_front), or
 //   (b) an alive point of an earlier chunk of the same wave.
-// One workgroup per chunk listed in `which`.  Flags of chunks in the same
-// wave may change while they are read: either value is correct (a point
-// killed by cross checks is itself dominated by an earlier point that also
-// dominates whatever it dominates).
-// The chunk's own live points are repacked into fewer rows per thread
-// whenever they fit: most points meet a dominator early, and the few that
-// scan the whole front then do so without the dead ones alongside.
+// G workgroups per chunk listed in `which` (workgroup g of chunk which[i] is
+// blockIdx.x = i * G + g); workgroup g takes the 256-point tiles g, g + G, ...
+// of that work, so one chunk's scan is spread over G compute units.  A point
+// that dies is flagged at once, and after every tile each workgroup drops the
+// points the others have flagged.  Flags of chunks in the same wave may change
+// while they are read: either value is correct (a point killed by cross
+// checks is itself dominated by an earlier point that also dominates whatever
+// it dominates).  The live points are repacked into fewer rows per thread
+// whenever they fit, so the few that scan everything do so without the dead.
+// Alive counts are left to k_append_front.
 // ---------------------------------------------------------------------------
 template <int D, int CH>
 __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
     const int32_t *__restrict__ pts, int64_t stride, uint8_t *alive,
-    const int32_t *__restrict__ chunk_nsurv, int32_t *__restrict__ chunk_nalive,
-    const int32_t *__restrict__ fv, int64_t fv_stride, const int64_t *__restrict__ fcount,
-    int W, unsigned long long *__restrict__ pair_tests) {
+    const int32_t *__restrict__ chunk_nsurv, const int32_t *__restrict__ fv, int64_t fv_stride,
+    const int64_t *__restrict__ fcount, int W, int G, unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
     constexpr int R = CH / kBlock; // rows of points per thread
     constexpr int kU = (kBlock * DP > CH * D) ? kBlock * DP : CH * D;
@@ -407,27 +409,27 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     __shared__ int16_t PS[CH];              // repacked slots
     __shared__ int wave_tot[kWaves];
 
-    const int me = which[blockIdx.x];
+    const int me = which[blockIdx.x / G];
+    const int g = static_cast<int>(blockIdx.x % G);
     const ChunkDesc c = chunks[me];
     const int nS = chunk_nsurv[me];
     const int tid = threadIdx.x;
-    const volatile uint8_t *valive = alive;
+    volatile uint8_t *valive = alive;
     const int k_in = me - c.group_first;
     const int wave_first = c.group_first + (k_in / W) * W;
     const int64_t fbase = c.start - static_cast<int64_t>(k_in) * CH;
     const int64_t fn = fcount[c.group_first];
 
     int32_t m[R][D];
-    bool dead[R];
-    int slot[R]; // -1: no point
+    int slot[R]; // -1: no live point
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int s = r * kBlock + tid;
-        slot[r] = s < nS ? s : -1;
-        dead[r] = s >= nS || valive[c.out_base + s] == 0;
+        const bool v = s < nS && valive[c.out_base + s] != 0;
+        slot[r] = v ? s : -1;
 #pragma unroll
         for (int o = 0; o < D; ++o) {
-            m[r][o] = dead[r] ? INT32_MAX : pts[o * stride + c.out_base + s];
+            m[r][o] = v ? pts[o * stride + c.out_base + s] : INT32_MAX;
         }
     }
     int rows = R;
@@ -435,13 +437,19 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     int live = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        live += dead[r] ? 0 : 1;
+        live += slot[r] >= 0 ? 1 : 0;
     }
     int total = block_sum(live, wave_tot);
     bool go = total > 0;
 
-    // tn staged rows in U against the live points; then count and repack
+    // tn staged rows in U against the live points; then deaths are flagged,
+    // other workgroups' kills taken, and the live points counted and repacked
     auto test_tile = [&](int tn) {
+        bool dead[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            dead[r] = false;
+        }
         for (int j = 0; j < tn; ++j) {
             const int32_t *row = &U[j * DP];
 #pragma unroll
@@ -451,34 +459,33 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
                 }
             }
         }
-        if (pair_tests != nullptr) {
-            int held = 0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                held += (r < rows && slot[r] >= 0) ? 1 : 0;
-            }
-            tests += static_cast<unsigned long long>(held) * tn;
-        }
         live = 0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            live += (r < rows && !dead[r]) ? 1 : 0;
+            if (r < rows && slot[r] >= 0) {
+                if (pair_tests != nullptr) {
+                    tests += static_cast<unsigned long long>(tn);
+                }
+                if (dead[r] || valive[c.out_base + slot[r]] == 0) {
+                    valive[c.out_base + slot[r]] = 0;
+                    slot[r] = -1;
+                } else {
+                    ++live;
+                }
+            }
         }
         total = block_sum(live, wave_tot); // its barriers also end the tile's reads
         go = total > 0;
         const int need = (total + kBlock - 1) / kBlock;
         if (go && need < rows) {
-            // repack the live points into `need` rows; dropped points die now
             int base = 0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (r < rows) {
-                    if (dead[r] && slot[r] >= 0) {
-                        alive[c.out_base + slot[r]] = 0;
-                    }
+                    const bool v = slot[r] >= 0;
                     int cnt = 0;
-                    const int rk = block_rank(!dead[r], wave_tot, &cnt);
-                    if (!dead[r]) {
+                    const int rk = block_rank(v, wave_tot, &cnt);
+                    if (v) {
 #pragma unroll
                         for (int o = 0; o < D; ++o) {
                             U[(base + rk) * D + o] = m[r][o];
@@ -495,7 +502,6 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
                 const int s = r * kBlock + tid;
                 const bool v = r < rows && s < total;
                 slot[r] = v ? PS[s] : -1;
-                dead[r] = !v;
 #pragma unroll
                 for (int o = 0; o < D; ++o) {
                     m[r][o] = v ? U[s * D + o] : INT32_MAX;
@@ -506,7 +512,8 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     };
 
     // (a) the front, dense
-    for (int64_t tb = 0; go && tb < fn; tb += kBlock) {
+    for (int64_t tb = static_cast<int64_t>(g) * kBlock; go && tb < fn;
+         tb += static_cast<int64_t>(G) * kBlock) {
         const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), fn - tb));
         if (tid < tn) {
 #pragma unroll
@@ -518,10 +525,14 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
         test_tile(tn);
     }
     // (b) earlier chunks of this wave, alive points only
+    int idx = 0;
     for (int other = wave_first; go && other < me; ++other) {
         const ChunkDesc oc = chunks[other];
         const int on = chunk_nsurv[other];
-        for (int tb = 0; go && tb < on; tb += kBlock) {
+        for (int tb = 0; go && tb < on; tb += kBlock, ++idx) {
+            if (idx % G != g) {
+                continue;
+            }
             const int i = tb + tid;
             const bool a = i < on && valive[oc.out_base + i] != 0;
             int32_t q[D];
@@ -541,39 +552,24 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
             test_tile(tn);
         }
     }
-
-    int kept = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        if (r < rows && slot[r] >= 0) {
-            if (dead[r]) {
-                alive[c.out_base + slot[r]] = 0;
-            } else {
-                ++kept;
-            }
-        }
-    }
-    kept = block_sum(kept, wave_tot);
-    if (tid == 0) {
-        chunk_nalive[me] = kept;
-    }
     if (pair_tests != nullptr) {
         atomicAdd(pair_tests, tests);
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_append_front: after the cross check of a wave, the alive points of the
-// wave's chunks (listed in `which`, every chunk of the wave) are appended in
-// chunk and slot order, so in key order, to their group's front
-// fv[fbase + fcount_in[g] ..]; the wave's last chunk of each group writes the
-// new count to fcount_out[g] (g = group_first).
+// k_append_front: after the cross check of a wave, every chunk of the wave
+// (listed in `which`) counts its alive points into chunk_nalive and appends
+// them, in slot order, to its group's front fv[fbase + at ..], where at =
+// fcount_in[g] plus the alive points of the wave's earlier chunks of the
+// group, so the front stays in key order; the wave's last chunk of each group
+// writes the new count to fcount_out[g] (g = group_first).
 // ---------------------------------------------------------------------------
 template <int D, int CH>
 __global__ __launch_bounds__(kBlock) void k_append_front(
     const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
     const int32_t *__restrict__ pts, int64_t stride, const uint8_t *__restrict__ alive,
-    const int32_t *__restrict__ chunk_nsurv, const int32_t *__restrict__ chunk_nalive,
+    const int32_t *__restrict__ chunk_nsurv, int32_t *__restrict__ chunk_nalive,
     int32_t *__restrict__ fv, int64_t fv_stride, const int64_t *__restrict__ fcount_in,
     int64_t *__restrict__ fcount_out, int W) {
     __shared__ int wave_tot[kWaves];
@@ -585,7 +581,15 @@ __global__ __launch_bounds__(kBlock) void k_append_front(
     const int64_t fbase = c.start - static_cast<int64_t>(k_in) * CH;
     int64_t at = fcount_in[c.group_first];
     for (int other = wave_first; other < me; ++other) {
-        at += chunk_nalive[other];
+        const ChunkDesc oc = chunks[other];
+        const int on = chunk_nsurv[other];
+        for (int tb = 0; tb < on; tb += kBlock) {
+            const int i = tb + threadIdx.x;
+            int cnt = 0;
+            block_rank(i < on && alive[oc.out_base + i] != 0, wave_tot, &cnt);
+            at += cnt;
+            __syncthreads();
+        }
     }
     const int nS = chunk_nsurv[me];
     int written = 0;
@@ -603,8 +607,11 @@ __global__ __launch_bounds__(kBlock) void k_append_front(
         written += cnt;
         __syncthreads(); // wave_tot is reused by the next rank
     }
-    if (me == wave_last && threadIdx.x == 0) {
-        fcount_out[c.group_first] = at + written;
+    if (threadIdx.x == 0) {
+        chunk_nalive[me] = written;
+        if (me == wave_last) {
+            fcount_out[c.group_first] = at + written;
+        }
     }
 }
 
