@@ -451,6 +451,8 @@ struct Launch {
     FilterFn filter_dense;   // SortedSrc in any order, presorted in LDS
     CrossFn cross;           // chunks of a group are descending key blocks
     AppendFn append_front;   // a wave's survivors onto their group's front
+    void (*stage)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, pmdd_k::SortedSrc,
+                  int32_t *, int64_t, uint8_t *, int32_t *, int32_t *);
     VsSortedFn vs_weak;
     VsSortedFn vs_strict;
     void (*compact)(dim3, hipStream_t, const ChunkDesc *, const int32_t *, const int64_t *,
@@ -548,8 +550,15 @@ template <int D> struct LaunchImpl {
         hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D>), g, dim3(pmdd_k::kWave), 0, s, X, xs,
                            n_in, R, rs, lo, hi, Y, ys, n_out, pt);
     }
+    static void stage(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
+                      pmdd_k::SortedSrc src, int32_t *out, int64_t stride, uint8_t *alive,
+                      int32_t *ns, int32_t *na) {
+        hipLaunchKernelGGL((pmdd_k::k_stage_chunks<D>), g, dim3(kBlock), 0, s, which, c, src, out,
+                           stride, alive, ns, na);
+    }
     static Launch table() {
         Launch L;
+        L.stage = stage;
         L.join_products = join_products;
         L.refill_vs = refill_vs;
         L.filter_dense = filter<pmdd_k::SortedSrc, false>;
@@ -689,6 +698,19 @@ struct Run {
         hip_check(hipGetLastError(), "k_filter_chunks");
     }
 
+    // Copy the listed chunks' candidates from a key-ordered source, all alive.
+    void run_stage(ChunkBufs &cb, DevBuf &wbuf, const std::vector<int32_t> &which,
+                   const pmdd_k::SortedSrc &src, int32_t *out, int64_t stride, uint8_t *alive) {
+        if (which.empty()) {
+            return;
+        }
+        h2d(wbuf, which, s, "hipMemcpy(chunk list)");
+        K.stage(dim3(static_cast<unsigned>(which.size())), s, wbuf.as<int32_t>(),
+                cb.chunks.as<ChunkDesc>(), src, out, stride, alive, cb.nsurv.as<int32_t>(),
+                cb.nalive.as<int32_t>());
+        hip_check(hipGetLastError(), "k_stage_chunks");
+    }
+
     // One-sided block filter over every group with more than one chunk (the
     // chunks of a group are descending-key blocks), in waves of kCrossWave
     // chunks per group: a wave's chunks are checked against the group's front
@@ -708,19 +730,11 @@ struct Run {
             return;
         }
         const int waves = static_cast<int>(ceil_div(max_count, kCrossWave));
-        // per wave: chunks to check (not a group's first), then every chunk
+        // per wave: every chunk of the groups with more than one chunk
         std::vector<int32_t> lists;
-        std::vector<size_t> cstart(waves + 1, 0), astart(waves + 1, 0);
+        std::vector<size_t> start(waves + 1, 0);
         for (int w = 0; w < waves; ++w) {
-            cstart[w] = lists.size();
-            for (size_t i = 0; i < chunks.size(); ++i) {
-                const ChunkDesc &c = chunks[i];
-                const int k = static_cast<int>(i) - c.group_first;
-                if (c.group_count > 1 && k > 0 && k / kCrossWave == w) {
-                    lists.push_back(static_cast<int32_t>(i));
-                }
-            }
-            astart[w] = lists.size();
+            start[w] = lists.size();
             for (size_t i = 0; i < chunks.size(); ++i) {
                 const ChunkDesc &c = chunks[i];
                 const int k = static_cast<int>(i) - c.group_first;
@@ -729,7 +743,7 @@ struct Run {
                 }
             }
         }
-        cstart[waves] = lists.size();
+        start[waves] = lists.size();
         h2d(ctx->which_c, lists, s, "hipMemcpy(wave lists)");
         // front counts, double-buffered by wave, indexed by a group's first chunk
         const size_t nc = chunks.size();
@@ -744,24 +758,21 @@ struct Run {
         for (int w = 0; w < waves; ++w) {
             int64_t *fin = fc + (w % 2) * nc;
             int64_t *fout = fc + ((w + 1) % 2) * nc;
-            const size_t n = astart[w] - cstart[w];
-            if (n > 0) {
-                // spread each chunk's scan over G workgroups, about 1024 in all
-                const int G = static_cast<int>(
-                    std::min<int64_t>(kCrossSplit, std::max<int64_t>(1, ceil_div(1024, n))));
-                K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + cstart[w],
-                        cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(), fv,
-                        fv_stride, fin, kCrossWave, G, d_tests);
-                hip_check(hipGetLastError(), "k_cross_chunks");
+            const size_t n = start[w + 1] - start[w];
+            if (n == 0) {
+                continue;
             }
-            const size_t na = ((w + 1 < waves) ? cstart[w + 1] : lists.size()) - astart[w];
-            if (na > 0) {
-                K.append_front(dim3(static_cast<unsigned>(na)), s, wl + astart[w],
-                               cb.chunks.as<ChunkDesc>(), pts, stride, alive,
-                               cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), fv, fv_stride,
-                               fin, fout, kCrossWave);
-                hip_check(hipGetLastError(), "k_append_front");
-            }
+            // spread each chunk's scan over G workgroups, about 1024 in all
+            const int G = static_cast<int>(
+                std::min<int64_t>(kCrossSplit, std::max<int64_t>(1, ceil_div(1024, n))));
+            K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + start[w], cb.chunks.as<ChunkDesc>(),
+                    pts, stride, alive, cb.nsurv.as<int32_t>(), fv, fv_stride, fin, kCrossWave, G,
+                    d_tests);
+            hip_check(hipGetLastError(), "k_cross_chunks");
+            K.append_front(dim3(static_cast<unsigned>(n)), s, wl + start[w],
+                           cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(),
+                           cb.nalive.as<int32_t>(), fv, fv_stride, fin, fout, kCrossWave);
+            hip_check(hipGetLastError(), "k_append_front");
         }
     }
 
@@ -940,9 +951,13 @@ struct Run {
             ctx->alive.ensure(static_cast<size_t>(pass_c), "hipMalloc(alive flags)");
             run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
                        ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
-            pmdd_k::SortedSrc ss{sorted, bigpos};
-            run_filter(K.filter_sorted, cb, ctx->which_b, which_big, 0, &ss,
-                       ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
+            if (!which_big.empty()) {
+                // big nodes: candidates staged as they are; the cross filter
+                // does all of their checks
+                pmdd_k::SortedSrc ss{sorted, bigpos};
+                run_stage(cb, ctx->which_b, which_big, ss, ctx->surv_pts.as<int32_t>(), pass_c,
+                          ctx->alive.as<uint8_t>());
+            }
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
             if (!which_cross.empty()) {
                 run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), pass_c,
@@ -1436,8 +1451,17 @@ struct Run {
             if (nb > 0) {
                 upload_table(cb, chunks);
                 pmdd_k::SortedSrc ss{ys, ytotal};
-                run_filter(K.filter_sorted, cb, ctx->which_a, none, nb, &ss,
-                           ctx->surv_pts.as<int32_t>(), ytotal, ctx->alive.as<uint8_t>());
+                if (nb == 1) {
+                    run_filter(K.filter_sorted, cb, ctx->which_a, none, nb, &ss,
+                               ctx->surv_pts.as<int32_t>(), ytotal, ctx->alive.as<uint8_t>());
+                } else {
+                    std::vector<int32_t> all(nb);
+                    for (int32_t k = 0; k < nb; ++k) {
+                        all[k] = k;
+                    }
+                    run_stage(cb, ctx->which_a, all, ss, ctx->surv_pts.as<int32_t>(), ytotal,
+                              ctx->alive.as<uint8_t>());
+                }
                 hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
                 run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), ytotal,
                                 ctx->alive.as<uint8_t>(), ys, ytotal);

@@ -384,7 +384,8 @@ __device__ __forceinline__ int block_sum(int v, int *wave_tot) {
 //   (a) a point of the group's front: the final survivors of the earlier
 //       waves, kept densely in key order at fv[fbase ..] (count
 //       fcount[group_first], filled by k_append_front), or
-//   (b) an alive point of an earlier chunk of the same wave.
+//   (b) an alive point of an earlier chunk of the same wave, or
+//   (c) an alive point in an earlier slot of the chunk itself.
 // G workgroups per chunk listed in `which` (workgroup g of chunk which[i] is
 // blockIdx.x = i * G + g); workgroup g takes the 256-point tiles g, g + G, ...
 // of that work, so one chunk's scan is spread over G compute units.  A point
@@ -407,6 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     constexpr int kU = (kBlock * DP > CH * D) ? kBlock * DP : CH * D;
     __shared__ __align__(16) int32_t U[kU]; // tile of earlier points, or repack buffer
     __shared__ int16_t PS[CH];              // repacked slots
+    __shared__ int16_t TS[kBlock];          // slots of a tile of the chunk itself
     __shared__ int wave_tot[kWaves];
 
     const int me = which[blockIdx.x / G];
@@ -442,20 +444,34 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     int total = block_sum(live, wave_tot);
     bool go = total > 0;
 
-    // tn staged rows in U against the live points; then deaths are flagged,
-    // other workgroups' kills taken, and the live points counted and repacked
-    auto test_tile = [&](int tn) {
+    // tn staged rows in U against the live points (rows of the chunk itself
+    // kill only points in later slots); then deaths are flagged, other
+    // workgroups' kills taken, and the live points counted and repacked
+    auto test_tile = [&](int tn, bool own) {
         bool dead[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             dead[r] = false;
         }
-        for (int j = 0; j < tn; ++j) {
-            const int32_t *row = &U[j * DP];
+        if (!own) {
+            for (int j = 0; j < tn; ++j) {
+                const int32_t *row = &U[j * DP];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (r < rows) {
-                    dead[r] |= ge_all<D, DP>(row, m[r]);
+                for (int r = 0; r < R; ++r) {
+                    if (r < rows) {
+                        dead[r] |= ge_all<D, DP>(row, m[r]);
+                    }
+                }
+            }
+        } else {
+            for (int j = 0; j < tn; ++j) {
+                const int32_t *row = &U[j * DP];
+                const int sj = TS[j];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (r < rows) {
+                        dead[r] |= (sj < slot[r]) && ge_all<D, DP>(row, m[r]);
+                    }
                 }
             }
         }
@@ -522,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
             }
         }
         __syncthreads();
-        test_tile(tn);
+        test_tile(tn, false);
     }
     // (b) earlier chunks of this wave, alive points only
     int idx = 0;
@@ -549,11 +565,62 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
                 }
             }
             __syncthreads();
-            test_tile(tn);
+            test_tile(tn, false);
         }
+    }
+    // (c) earlier points of the chunk itself, alive ones only
+    for (int tb = 0; go && tb < nS; tb += kBlock, ++idx) {
+        if (idx % G != g) {
+            continue;
+        }
+        const int i = tb + tid;
+        const bool a = i < nS && valive[c.out_base + i] != 0;
+        int32_t q[D];
+#pragma unroll
+        for (int o = 0; o < D; ++o) {
+            q[o] = a ? pts[o * stride + c.out_base + i] : 0;
+        }
+        int tn = 0;
+        const int pos = block_rank(a, wave_tot, &tn);
+        if (a) {
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                U[pos * DP + o] = q[o];
+            }
+            TS[pos] = static_cast<int16_t>(i);
+        }
+        __syncthreads();
+        test_tile(tn, true);
     }
     if (pair_tests != nullptr) {
         atomicAdd(pair_tests, tests);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_stage_chunks: the candidates of the listed chunks copied from a key-ordered
+// source to the pass buffer, all alive (no local filter: k_cross_chunks checks
+// a chunk against its own earlier points as well).
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_stage_chunks(const int32_t *__restrict__ which,
+                                                         const ChunkDesc *__restrict__ chunks,
+                                                         SortedSrc src, int32_t *__restrict__ out,
+                                                         int64_t stride, uint8_t *__restrict__ alive,
+                                                         int32_t *__restrict__ chunk_nsurv,
+                                                         int32_t *__restrict__ chunk_nalive) {
+    const int me = which[blockIdx.x];
+    const ChunkDesc c = chunks[me];
+    for (int s = threadIdx.x; s < c.len; s += kBlock) {
+#pragma unroll
+        for (int o = 0; o < D; ++o) {
+            out[o * stride + c.out_base + s] = src.pts[o * src.stride + c.start + s];
+        }
+        alive[c.out_base + s] = 1;
+    }
+    if (threadIdx.x == 0) {
+        chunk_nsurv[me] = c.len;
+        chunk_nalive[me] = c.len;
     }
 }
 
