@@ -718,7 +718,7 @@ struct Run {
     // the group's first candidate slot) and against the earlier chunks of the
     // same wave; the wave's survivors are then appended to the front.  `fv`
     // needs room for every candidate of the groups (stride fv_stride).
-    static constexpr int kCrossWave = 16;
+    static constexpr int kCrossWave = 64;
     static constexpr int64_t kCrossSplit = 16; // most workgroups per chunk
     void run_cross_waves(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks, const int32_t *pts,
                          int64_t stride, uint8_t *alive, int32_t *fv, int64_t fv_stride) {
