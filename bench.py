@@ -192,8 +192,8 @@ def main():
         pts = 0
         agg = {"kernel_filter_s": 0.0, "filter_launches": 0, "algorithmic_bytes": 0,
                "kernel_frontier_check_s": 0.0, "frontier_check_launches": 0,
-               "frontier_check_bytes": 0, "work_frontier_survivors_total": 0,
-               "kernel_total_s": 0.0}
+               "frontier_check_bytes": 0, "kernel_cross_s": 0.0, "cross_launches": 0,
+               "cross_bytes": 0, "work_frontier_survivors_total": 0, "kernel_total_s": 0.0}
         per = {}
         for name, r in res:
             ref = None
@@ -248,6 +248,9 @@ def main():
         # the block of R read once
         "k_refill_vs_block": (tot["kernel_frontier_check_s"], tot["frontier_check_launches"],
                               tot["frontier_check_bytes"]),
+        # k_cross_chunks (one launch per wave): per wave the group's front and
+        # the wave's points read once, survivors appended once, a flag per point
+        "k_cross_chunks": (tot["kernel_cross_s"], tot["cross_launches"], tot["cross_bytes"]),
     }
     kname = max(kernels, key=lambda k: kernels[k][0])
     k_time, k_launches, k_bytes = kernels[kname]

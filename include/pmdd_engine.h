@@ -131,13 +131,20 @@ typedef struct pmdd_stats {
     int64_t filter_launches;
     int64_t filter_candidates;
     int64_t algorithmic_bytes;
-    /* The coupling kernel that checks key-ordered products against the
-     * running frontier (k_dense_vs_sorted): device time, launches, compulsory
-     * bytes (each point read once, the scanned frontier range read once, one
-     * flag written per point). */
+    /* The coupling kernel that checks products against the running frontier
+     * block by block (k_refill_vs_block): device time, launches, compulsory
+     * bytes (per block: its input points read once, its survivors written
+     * once, the block of the frontier read once). */
     double kernel_frontier_check_s;
     int64_t frontier_check_launches;
     int64_t frontier_check_bytes;
+    /* The block cross filter of big nodes and coupling batches
+     * (k_cross_chunks with k_append_front): device time, launches, compulsory
+     * bytes (per wave: the group's front and the wave's points read once,
+     * the survivors appended once, one flag per point). */
+    double kernel_cross_s;
+    int64_t cross_launches;
+    int64_t cross_bytes;
     int32_t layer_coupling;
     int32_t std_capacity;
     double *std_candidates_per_layer;

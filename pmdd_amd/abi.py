@@ -93,6 +93,9 @@ class Stats(C.Structure):
         ("kernel_frontier_check_s", C.c_double),
         ("frontier_check_launches", C.c_int64),
         ("frontier_check_bytes", C.c_int64),
+        ("kernel_cross_s", C.c_double),
+        ("cross_launches", C.c_int64),
+        ("cross_bytes", C.c_int64),
         ("layer_coupling", C.c_int32),
         ("std_capacity", C.c_int32),
         ("std_candidates_per_layer", C.POINTER(C.c_double)),

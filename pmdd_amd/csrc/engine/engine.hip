@@ -123,6 +123,7 @@ struct pmdd_ctx {
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
     DevBuf join_idx, join_x, join_y, join_ctr, front_count;
+    std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
 };
 
 struct LayerCSR {
@@ -662,6 +663,28 @@ struct Run {
             std::max(st->gpu_mem_peak_used_bytes, std::max<int64_t>(0, used - mem_base));
     }
 
+    // Event i of the context's pool, created on first use.
+    hipEvent_t pool_event(size_t i) {
+        while (ctx->ev_pool.size() <= i) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreate(&e), "hipEventCreate");
+            ctx->ev_pool.push_back(e);
+        }
+        return ctx->ev_pool[i];
+    }
+    // Seconds between the pool's pairs (0,1), (2,3), ... for `pairs` pairs;
+    // the stream must have passed them.
+    double pool_seconds(int pairs) {
+        double t = 0.0;
+        for (int i = 0; i < pairs; ++i) {
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, ctx->ev_pool[2 * i], ctx->ev_pool[2 * i + 1]),
+                      "hipEventElapsedTime");
+            t += ms * 1e-3;
+        }
+        return t;
+    }
+
     float elapsed_ms() {
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b), "hipEventElapsedTime");
@@ -718,16 +741,18 @@ struct Run {
     // the group's first candidate slot) and against the earlier chunks of the
     // same wave; the wave's survivors are then appended to the front.  `fv`
     // needs room for every candidate of the groups (stride fv_stride).
+    // Returns the number of k_cross_chunks launches; launch i is bracketed by
+    // pool events 2i and 2i+1.
     static constexpr int kCrossWave = 64;
     static constexpr int64_t kCrossSplit = 16; // most workgroups per chunk
-    void run_cross_waves(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks, const int32_t *pts,
-                         int64_t stride, uint8_t *alive, int32_t *fv, int64_t fv_stride) {
+    int run_cross_waves(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks, const int32_t *pts,
+                        int64_t stride, uint8_t *alive, int32_t *fv, int64_t fv_stride) {
         int32_t max_count = 1;
         for (const ChunkDesc &c : chunks) {
             max_count = std::max(max_count, c.group_count);
         }
         if (max_count <= 1) {
-            return;
+            return 0;
         }
         const int waves = static_cast<int>(ceil_div(max_count, kCrossWave));
         // per wave: every chunk of the groups with more than one chunk
@@ -755,6 +780,7 @@ struct Run {
                            static_cast<int64_t>(2 * nc * sizeof(int64_t)), uint8_t(0));
         hip_check(hipGetLastError(), "k_fill_u8");
         const int32_t *wl = ctx->which_c.as<int32_t>();
+        int launches = 0;
         for (int w = 0; w < waves; ++w) {
             int64_t *fin = fc + (w % 2) * nc;
             int64_t *fout = fc + ((w + 1) % 2) * nc;
@@ -765,15 +791,48 @@ struct Run {
             // spread each chunk's scan over G workgroups, about 1024 in all
             const int G = static_cast<int>(
                 std::min<int64_t>(kCrossSplit, std::max<int64_t>(1, ceil_div(1024, n))));
+            hip_check(hipEventRecord(pool_event(2 * launches), s), "hipEventRecord");
             K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + start[w], cb.chunks.as<ChunkDesc>(),
                     pts, stride, alive, cb.nsurv.as<int32_t>(), fv, fv_stride, fin, kCrossWave, G,
                     d_tests);
             hip_check(hipGetLastError(), "k_cross_chunks");
+            hip_check(hipEventRecord(pool_event(2 * launches + 1), s), "hipEventRecord");
             K.append_front(dim3(static_cast<unsigned>(n)), s, wl + start[w],
                            cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(),
                            cb.nalive.as<int32_t>(), fv, fv_stride, fin, fout, kCrossWave);
             hip_check(hipGetLastError(), "k_append_front");
+            ++launches;
         }
+        return launches;
+    }
+
+    // Compulsory bytes of the cross waves over `chunks` (groups of consecutive
+    // chunks), once the final alive counts are known: per wave, the group's
+    // front and the wave's points read once, the survivors appended once, one
+    // flag per point.
+    int64_t cross_bytes(const std::vector<ChunkDesc> &chunks,
+                        const std::vector<int32_t> &nalive) const {
+        int64_t b = 0;
+        size_t i = 0;
+        while (i < chunks.size()) {
+            const ChunkDesc &c = chunks[i];
+            if (c.group_count <= 1) {
+                ++i;
+                continue;
+            }
+            int64_t front = 0;
+            for (int k = 0; k < c.group_count; k += kCrossWave) {
+                int64_t pts = 0, kept = 0;
+                for (int j = k; j < std::min(k + kCrossWave, c.group_count); ++j) {
+                    pts += chunks[i + j].len;
+                    kept += nalive[i + j];
+                }
+                b += 4 * static_cast<int64_t>(d) * (front + pts + kept) + pts;
+                front += kept;
+            }
+            i += static_cast<size_t>(c.group_count);
+        }
+        return b;
     }
 
     void run_compact(ChunkBufs &cb, int64_t nch, const std::vector<int64_t> &outpos,
@@ -959,12 +1018,18 @@ struct Run {
                           ctx->alive.as<uint8_t>());
             }
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+            int cross_waves = 0;
             if (!which_cross.empty()) {
-                run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), pass_c,
-                                ctx->alive.as<uint8_t>(), sorted, bigpos);
+                cross_waves = run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), pass_c,
+                                              ctx->alive.as<uint8_t>(), sorted, bigpos);
             }
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
             d2h(nalive, cb.nalive, chunks.size(), s, "chunk survivor counts");
+            if (cross_waves > 0) {
+                st->kernel_cross_s += pool_seconds(cross_waves);
+                st->cross_launches += cross_waves;
+                st->cross_bytes += cross_bytes(chunks, nalive);
+            }
             add_time(ctx->ev_a, ctx->ev_d, &st->kernel_expand_td_s);
             add_time(ctx->ev_d, ctx->ev_c, &st->kernel_filter_s);
             add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
@@ -1463,9 +1528,15 @@ struct Run {
                               ctx->alive.as<uint8_t>());
                 }
                 hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-                run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), ytotal,
-                                ctx->alive.as<uint8_t>(), ys, ytotal);
+                const int cross_waves = run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(),
+                                                        ytotal, ctx->alive.as<uint8_t>(), ys,
+                                                        ytotal);
                 d2h(nalive, cb.nalive, chunks.size(), s, "batch survivor counts");
+                if (cross_waves > 0) {
+                    st->kernel_cross_s += pool_seconds(cross_waves);
+                    st->cross_launches += cross_waves;
+                    st->cross_bytes += cross_bytes(chunks, nalive);
+                }
                 outpos.resize(chunks.size());
                 for (size_t c = 0; c < chunks.size(); ++c) {
                     outpos[c] = btotal;
@@ -1742,6 +1813,9 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
         if (e != nullptr) {
             (void)hipEventDestroy(e);
         }
+    }
+    for (hipEvent_t e : ctx->ev_pool) {
+        (void)hipEventDestroy(e);
     }
     if (ctx->stream != nullptr) {
         (void)hipStreamDestroy(ctx->stream);
