@@ -82,12 +82,14 @@ typedef struct pmdd_dd_view {
 /* Options of one enumeration, mirroring the reference CLI
  * (src/pmdd/util/cli_parser.hpp:10-30). */
 typedef struct pmdd_opts {
-    int32_t method;          /* 1 = top-down, 3 = dynamic layer cutset (coupled) */
+    int32_t method;          /* 1 = top-down, 2 = bottom-up (the reference runs it on the
+                              * CPU only), 3 = dynamic layer cutset (coupled) */
     int32_t problem_type;    /* 1 knapsack, 2 set packing, 3 TSP */
     int32_t state_dominance; /* 0 or 1 */
     int32_t count_pair_tests;/* 1 = count executed dominance pair tests (slower) */
     int64_t max_cand;        /* --max-cand, candidates per expansion pass (default 20M) */
-    int64_t max_prod;        /* --max-prod, products per coupling batch (default 625K) */
+    int64_t max_prod;        /* --max-prod, products of the first coupling batches
+                              * (default 625K); later batches grow up to 32M */
 } pmdd_opts;
 
 #define PMDD_DEFAULT_MAX_CAND 20000000LL
@@ -106,8 +108,8 @@ typedef struct pmdd_frontier {
 /* GPU-side statistics (subset of EnumerationStats that the GPU path fills,
  * src/pmdd/util/stats.hpp:18-85).  The engine overwrites every scalar field.
  * The per-layer arrays are caller-owned and optional: when non-NULL the engine
- * writes min(std_capacity, num_layers) entries (method 1 only; method 3 leaves
- * them at 0, as the reference clears them, enum.cu:610-613). */
+ * writes min(std_capacity, num_layers) entries (methods 1 and 2; method 3
+ * leaves them at 0, as the reference clears them, enum.cu:610-613). */
 typedef struct pmdd_stats {
     double wall_pack_transfer_s;
     double wall_join_s;

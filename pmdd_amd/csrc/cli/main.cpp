@@ -1,7 +1,7 @@
 // pmdd -- command-line entry point with the reference's argv surface
 // (src/pmdd/main.cpp:67-467, src/pmdd/util/cli_parser.cpp:181-488).
 //
-//   pmdd <input> <problem_type 1|2|3> <method 1|3> <state_dominance 0|1>
+//   pmdd <input> <problem_type 1|2|3> <method 1|2|3> <state_dominance 0|1>
 //        [--backend gpu | gpu] [--max-cand N] [--max-prod N]
 //        [--save-frontier] [--frontier-out PATH] [--save-stats] [--stats-out PATH]
 //
@@ -109,7 +109,7 @@ int main(int argc, char *argv[]) {
     // --- enumeration on the GPU (main.cpp:328-401) ---
     const auto wall_e0 = std::chrono::steady_clock::now();
     const clock_t cpu_e0 = clock();
-    const char *what = opts.method == 1 ? "top-down" : "coupled";
+    const char *what = opts.method == 1 ? "top-down" : opts.method == 2 ? "bottom-up" : "coupled";
     pmdd_ctx *ctx = nullptr;
     rc = pmdd_ctx_create(0, &ctx, err, sizeof(err));
     pmdd_frontier fr{};
@@ -178,7 +178,7 @@ int main(int argc, char *argv[]) {
     st.work_frontier_survivors_total = es.work_frontier_survivors_total;
     st.work_frontier_peak_points = es.work_frontier_peak_points;
     st.work_join_products_total = es.work_join_products_total;
-    if (opts.method == 1) {
+    if (opts.method == 1 || opts.method == 2) {
         st.std_candidates_per_layer = std_c;
         st.std_frontier_survivors_per_layer = std_s;
     }

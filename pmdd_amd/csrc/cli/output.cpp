@@ -146,7 +146,7 @@ void print_usage(const char *prog) {
     std::cout << "\t\tproblem_type = 2: set packing\n";
     std::cout << "\t\tproblem_type = 3: TSP\n\n";
     std::cout << "\t\tmethod = 1: top-down BFS\n";
-    std::cout << "\t\tmethod = 2: bottom-up BFS (CPU backend only; not built here)\n";
+    std::cout << "\t\tmethod = 2: bottom-up BFS (knapsack and set packing)\n";
     std::cout << "\t\tmethod = 3: dynamic layer cutset\n\n";
     std::cout << "\t\tstate_dominance = 0:  disable state dominance\n";
     std::cout << "\t\tstate_dominance = 1:  state dominance strategy 1\n\n";
@@ -337,7 +337,7 @@ bool parse_cli_args(int argc, char **argv, Options *out, std::string *error) {
         int t = 0;
         o.cpu_threads = (env != nullptr && parse_positive_int(env, &t)) ? t : 1;
     }
-    if (o.backend == BACKEND_GPU && o.method != 1 && o.method != 3) {
+    if (o.backend == BACKEND_GPU && o.method != 1 && o.method != 2 && o.method != 3) {
         return set_err(error, "Error - GPU backend is unsupported for method " +
                                   std::to_string(o.method) + ".");
     }

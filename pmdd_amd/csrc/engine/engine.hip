@@ -1245,6 +1245,35 @@ struct Run {
         fetch_points(cur, cur.h_off[dd->terminal], cur.size(dd->terminal), out);
     }
 
+    // Method 2 on the GPU (the reference runs it on the CPU only,
+    // bottomup_cpu_enumerate enum/cpu/enum.cpp:123-229, and rejects it for the
+    // GPU, main.cpp:356-361): bottom-up expansion from the terminal with the
+    // same kernels; the root's frontier is the result.  State dominance does
+    // not apply (the CPU path ignores it too, enum.cpp:127).
+    void bottomup(pmdd_frontier *out) {
+        const int L = dd->L;
+        DevFrontier cur, nxt;
+        seed_frontier(cur, d, dd->layer_size[L - 1], dd->terminal, s);
+        for (int l = L - 2; l >= 0; --l) {
+            int64_t cand = 0;
+            std::vector<int64_t> gamma;
+            expand(cur, dd->bu[l], dd->layer_size[l], nxt, &cand, &gamma);
+            count_layer(cand, nxt.total());
+            if (st->std_candidates_per_layer != nullptr && l < st->std_capacity) {
+                st->std_candidates_per_layer[l] = pop_std(gamma);
+            }
+            if (st->std_frontier_survivors_per_layer != nullptr && l < st->std_capacity) {
+                std::vector<int64_t> sz(nxt.n_nodes);
+                for (int64_t v = 0; v < nxt.n_nodes; ++v) {
+                    sz[v] = nxt.size(v);
+                }
+                st->std_frontier_survivors_per_layer[l] = pop_std(sz);
+            }
+            std::swap(cur, nxt);
+        }
+        fetch_points(cur, cur.h_off[dd->root], cur.size(dd->root), out);
+    }
+
     // Method 3 (coupled_bdd_cuda_enumerate enum.cu:595-756, enumerate_mdd_coupled
     // enum.cu:295-436): expand top-down while its score is not above the
     // bottom-up score, else bottom-up; the scores are sum |F(v)| * out-arcs(v)
@@ -1681,7 +1710,7 @@ int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmd
     if (opts == nullptr || out == nullptr) {
         fail(PMDD_E_INVALID, "NULL options or output");
     }
-    if (opts->method != 1 && opts->method != 3) {
+    if (opts->method < 1 || opts->method > 3) {
         fail(PMDD_E_UNSUPPORTED,
              "GPU backend is unsupported for method " + std::to_string(opts->method) + ".");
     }
@@ -1698,6 +1727,8 @@ int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmd
     Run run(ctx, dd, *opts, st);
     if (opts->method == 1) {
         run.topdown(out);
+    } else if (opts->method == 2) {
+        run.bottomup(out);
     } else {
         run.coupled(out);
     }

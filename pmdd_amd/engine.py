@@ -3,7 +3,7 @@
 ``Engine.enumerate`` plays the role of the reference's
 MultiobjEnum::pareto_frontier_{topdown,dynamic_layer_cutset}_cuda
 (src/pmdd/enum/multiobj_enum.cpp:72-124): it takes a decision diagram, a
-method (1 top-down, 3 dynamic layer cutset), the problem type and state
+method (1 top-down, 2 bottom-up, 3 dynamic layer cutset), the problem type and state
 dominance flag, plus the --max-cand / --max-prod budgets, and returns the
 frontier.  Errors raise PmddError carrying the engine's code and message
 (the reference returns NULL with a reason string instead).

@@ -19,6 +19,7 @@ pytestmark = pytest.mark.gpu
     ("KP_p-3_n-20_ins-1.dat", "1", "1", "1"),
     ("KP_p-3_n-20_ins-1.dat", "1", "3", "0"),
     ("bp-100-20-3-10-10192.dat", "2", "3", "0"),
+    ("bp-100-20-3-10-10192.dat", "2", "2", "0"),
     ("tsp-nobj3-ncities10-seed12870.dat", "3", "1", "0"),
 ])
 def test_cli_outputs(tmp_path, name, ptype, method, sd):

@@ -29,7 +29,7 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("name", SMALL_FIXTURES)
-@pytest.mark.parametrize("method", [1, 3])
+@pytest.mark.parametrize("method", [1, 2, 3])
 def test_fixture_matches_oracle(engine, name, method):
     print(f"case {name} method={method}", flush=True)
     dd = _dd(name)
@@ -94,7 +94,7 @@ def test_generated_small_vs_bruteforce(engine, tmp_path, seed):
         print(f"case seed={seed} kind={inst['kind']} n={inst['n']} d={inst['d']}", flush=True)
         bf = oracle.bruteforce(path, inst["kind"])
         dd = DecisionDiagram.from_file(path, inst["kind"])
-        for method in (1, 3):
+        for method in (1, 2, 3):
             for sd in (0, 1):
                 got = engine.enumerate(dd, method=method, state_dominance=sd).points
                 assert _same(got, bf), (seed, k, method, sd)
@@ -108,6 +108,28 @@ def test_generated_knapsack_stats_match_oracle(engine, tmp_path, d):
     dd = DecisionDiagram.from_file(path, 1)
     ref, st, std_c, std_s = oracle.enumerate_dd(dd, 1, 0, threads=4, with_stats=True)
     res = engine.enumerate(dd, method=1)
+    assert _same(res.points, ref)
+    s = res.stats
+    assert s["work_candidates_total"] == st.work_candidates_total
+    assert s["work_candidates_peak"] == st.work_candidates_peak
+    assert s["work_frontier_survivors_total"] == st.work_frontier_survivors_total
+    assert s["work_frontier_peak_points"] == st.work_frontier_peak_points
+    np.testing.assert_allclose(s["std_candidates_per_layer"], std_c, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(s["std_frontier_survivors_per_layer"], std_s, rtol=1e-12,
+                               atol=1e-9)
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_bottomup_stats_match_oracle(engine, tmp_path, kind):
+    """Method 2: frontier and order-independent work counters and per-layer
+    std against the oracle's bottom-up (bottomup_cpu_enumerate)."""
+    inst = {1: gen.knapsack(21, 28, 4), 2: gen.setpacking(22, 40, 3, m=8), 3: gen.tsp(23, 8, 3)}[kind]
+    path = gen.write(inst, str(tmp_path / "g.dat"))
+    dd = DecisionDiagram.from_file(path, kind)
+    print(f"case bottom-up kind={kind}", flush=True)
+    ref, st, std_c, std_s = oracle.enumerate_dd(dd, 2, 0, threads=4, with_stats=True)
+    assert _same(ref, oracle.enumerate_dd(dd, 1, 0, threads=4))
+    res = engine.enumerate(dd, method=2)
     assert _same(res.points, ref)
     s = res.stats
     assert s["work_candidates_total"] == st.work_candidates_total

@@ -200,7 +200,7 @@ def _cli(*args):
      "cpu thread options are not valid with backend=gpu"),
     (["f", "1", "1", "0", "--max-cand", "10"], "--max-cand is only valid with backend=gpu"),
     (["f", "1", "1", "0", "gpu", "--max-cand", "12X"], "invalid --max-cand value '12X'"),
-    (["f", "1", "2", "0", "gpu"], "GPU backend is unsupported for method 2."),
+    (["f", "1", "4", "0", "gpu"], "GPU backend is unsupported for method 4."),
     (["f", "1", "1", "0", "gpu", "--bogus"], "unrecognized optional argument '--bogus'"),
     (["f", "1", "1", "0", "gpu", "--backend", "gpu"],
      "cannot mix --backend with shorthand backend token"),
