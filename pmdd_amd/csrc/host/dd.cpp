@@ -208,26 +208,37 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
         active[v] = v;
     }
     std::vector<int64_t> counter(n, 0);
+    std::vector<int64_t> byte_hist(static_cast<size_t>(W) * 8 * 256, 0);
     WordKeyMap next_map(W);
-    std::vector<uint64_t> s0(W), s1(W);
+    std::vector<uint64_t> child_keys;
+    std::vector<int64_t> child_slot, child_ids;
 
     for (int l = 1; l <= n; ++l) {
         const int64_t width = static_cast<int64_t>(cur.size()) / W;
         // min-in-state: the active vertex contained in the fewest states,
         // first in active-list order on ties
-        for (int32_t v : active) {
-            counter[v] = 0;
-        }
+        // per byte position, how many states hold each byte value; a
+        // vertex's count is the sum over the values with its bit set
+        std::fill(byte_hist.begin(), byte_hist.end(), 0);
         for (int64_t i = 0; i < width; ++i) {
             const uint64_t *s = cur.data() + i * W;
             for (int w = 0; w < W; ++w) {
                 uint64_t bits = s[w];
-                while (bits != 0) {
-                    const int b = __builtin_ctzll(bits);
-                    counter[w * 64 + b]++;
-                    bits &= bits - 1;
+                int64_t *h = byte_hist.data() + static_cast<size_t>(w) * 8 * 256;
+                for (int b = 0; b < 8; ++b) {
+                    h[b * 256 + (bits & 0xFF)]++;
+                    bits >>= 8;
                 }
             }
+        }
+        for (int32_t v : active) {
+            const int64_t *h = byte_hist.data() + static_cast<size_t>(v / 8) * 256;
+            const int bit = v % 8;
+            int64_t c = 0;
+            for (int val = 0; val < 256; ++val) {
+                c += ((val >> bit) & 1) ? h[val] : 0;
+            }
+            counter[v] = c;
         }
         size_t sel = 0;
         for (size_t i = 1; i < active.size(); ++i) {
@@ -250,21 +261,32 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
         const int vw = vertex / 64;
         const uint64_t vb = 1ULL << (vertex % 64);
         const uint64_t *cm = compl_mask.data() + static_cast<size_t>(vertex) * W;
+        // child states in state order (0-child, then the 1-child when the
+        // vertex is eligible), inserted in that order
+        child_keys.clear();
+        child_slot.clear();
         for (int64_t i = 0; i < width; ++i) {
             const uint64_t *s = cur.data() + i * W;
             const bool was_set = (s[vw] & vb) != 0;
-            for (int w = 0; w < W; ++w) {
-                s0[w] = s[w];
-            }
-            s0[vw] &= ~vb;
-            bool ins = false;
-            bdd.child[l - 1][i][0] = static_cast<int32_t>(next_map.insert(s0.data(), &ins));
+            const size_t at = child_keys.size();
+            child_keys.insert(child_keys.end(), s, s + W);
+            child_keys[at + vw] &= ~vb;
+            child_slot.push_back(2 * i);
             if (was_set) {
+                const size_t at1 = child_keys.size();
+                child_keys.resize(at1 + W);
                 for (int w = 0; w < W; ++w) {
-                    s1[w] = s0[w] & cm[w];
+                    child_keys[at1 + w] = child_keys[at + w] & cm[w];
                 }
-                bdd.child[l - 1][i][1] = static_cast<int32_t>(next_map.insert(s1.data(), &ins));
+                child_slot.push_back(2 * i + 1);
             }
+        }
+        child_ids.resize(child_slot.size());
+        next_map.insert_many(child_keys.data(), static_cast<int64_t>(child_slot.size()),
+                             child_ids.data());
+        for (size_t c = 0; c < child_slot.size(); ++c) {
+            bdd.child[l - 1][child_slot[c] / 2][child_slot[c] % 2] =
+                static_cast<int32_t>(child_ids[c]);
         }
         const int64_t nn = next_map.size();
         cur.assign(next_map.key(0), next_map.key(0) + nn * W);
@@ -291,9 +313,16 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
     }
     const int64_t total = out->layer_off[L];
     out->in_off.assign(total + 1, 0);
-    out->arc_src.clear();
-    out->arc_val.clear();
-    out->arc_w.clear();
+    // every arc once, so the arc arrays are sized once
+    int64_t total_arcs = 0;
+    for (int l = 1; l < L; ++l) {
+        for (const std::array<int32_t, 2> &c : bdd.child[l - 1]) {
+            total_arcs += (c[0] >= 0 ? 1 : 0) + (c[1] >= 0 ? 1 : 0);
+        }
+    }
+    out->arc_src.assign(total_arcs, 0);
+    out->arc_val.assign(total_arcs, 0);
+    out->arc_w.assign(total_arcs * d, 0);
 
     // root: no in-arcs
     int64_t arc = 0;
@@ -322,9 +351,6 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
         }
         out->in_off[base_node + n_here] = cursor;
         const int64_t layer_arcs = cursor - arc;
-        out->arc_src.resize(cursor);
-        out->arc_val.resize(cursor);
-        out->arc_w.resize(cursor * d);
         // parents in ascending index order within each (head, type) group
         for (size_t p = 0; p < parents.size(); ++p) {
             for (int t = 1; t >= 0; --t) {
@@ -335,8 +361,9 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
                 const int64_t a = (t == 1) ? pos1[h]++ : pos0[h]++;
                 out->arc_src[a] = static_cast<int32_t>(p);
                 out->arc_val[a] = t;
-                for (int o = 0; o < d; ++o) {
-                    out->arc_w[a * d + o] = (t == 1) ? bdd.one_w[l - 1][o] : 0;
+                if (t == 1) {
+                    std::copy(bdd.one_w[l - 1].begin(), bdd.one_w[l - 1].end(),
+                              out->arc_w.begin() + a * d);
                 }
             }
         }

@@ -150,6 +150,49 @@ def test_bdd_in_arcs_list_one_arcs_first():
         assert list(vals) == sorted(vals, reverse=True)
 
 
+def _path_sums(dd):
+    """Objective vectors of all root->terminal paths of a small flattened DD,
+    with multiplicity."""
+    a = dd.arrays()
+    lo, io = a["layer_off"], a["in_off"]
+    L, d = dd.num_layers, dd.d
+    w = a["arc_w"].reshape(-1, d)
+    sums = {0: [tuple([0] * d)]}
+    for l in range(1, L):
+        nxt = {}
+        for v in range(lo[l + 1] - lo[l]):
+            g = lo[l] + v
+            acc = []
+            for e in range(io[g], io[g + 1]):
+                for p in sums.get(int(a["arc_src"][e]), []):
+                    acc.append(tuple(int(x) + int(y) for x, y in zip(p, w[e])))
+            nxt[v] = acc
+        sums = nxt
+    return sorted(sums[0])
+
+
+def test_setpacking_bdd_paths_are_exactly_the_independent_sets(tmp_path):
+    """Every independent set of the conflict graph is one path (and nothing
+    else is), whatever vertex order the min-in-state rule picks."""
+    inst = gen.setpacking(31, 13, 2, m=4)
+    path = gen.write(inst, str(tmp_path / "s.dat"))
+    dd = DecisionDiagram.from_file(path, 2)
+    n = inst["n"]
+    conflict = set()
+    for c in inst["constraints"]:
+        for u in c:
+            for v in c:
+                if u != v:
+                    conflict.add((u, v))
+    want = []
+    for mask in range(1 << n):
+        chosen = [v for v in range(n) if mask >> v & 1]
+        if any((u, v) in conflict for u in chosen for v in chosen):
+            continue
+        want.append(tuple(sum(inst["objs"][o][v] for v in chosen) for o in range(2)))
+    assert _path_sums(dd) == sorted(want)
+
+
 def test_setpacking_first_vertex_and_terminal(tmp_path):
     inst = gen.setpacking(3, 16, 3, m=4)
     path = gen.write(inst, str(tmp_path / "s.dat"))
