@@ -632,6 +632,7 @@ struct Run {
     hipStream_t s;
     int64_t mem_base = 0;
     unsigned long long *d_tests = nullptr;
+    DevFrontier sd_spare; // state dominance output, swapped with the layer's frontier
     // PMDD_DEBUG_JOIN=1 prints per-batch coupling counts to stderr
     const bool debug_join = std::getenv("PMDD_DEBUG_JOIN") != nullptr;
 
@@ -1156,19 +1157,23 @@ struct Run {
             return;
         }
         st->dominance_filtered_total += dropped;
-        grow_points(g, d, std::max<int64_t>(g.total(), 1), 0, s);
-        upload_offsets(g, s);
+        // the compacted labels go to the spare frontier, which then keeps the
+        // old buffers for the next layer (no allocation per layer)
+        sd_spare.n_nodes = n;
+        std::swap(sd_spare.h_off, g.h_off);
+        grow_points(sd_spare, d, std::max<int64_t>(sd_spare.total(), 1), 0, s);
+        upload_offsets(sd_spare, s);
         K.compact_nodes(dim3(static_cast<unsigned>(n)), s, f.off.as<int64_t>(),
                         f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>(),
-                        g.off.as<int64_t>(), g.pts.as<int32_t>(), g.cap);
+                        sd_spare.off.as<int64_t>(), sd_spare.pts.as<int32_t>(), sd_spare.cap);
         hip_check(hipGetLastError(), "k_compact_nodes");
         hip_check(hipStreamSynchronize(s), "state dominance sync");
-        std::swap(f.h_off, g.h_off);
-        std::swap(f.off.p, g.off.p);
-        std::swap(f.off.bytes, g.off.bytes);
-        std::swap(f.pts.p, g.pts.p);
-        std::swap(f.pts.bytes, g.pts.bytes);
-        std::swap(f.cap, g.cap);
+        std::swap(f.h_off, sd_spare.h_off);
+        std::swap(f.off.p, sd_spare.off.p);
+        std::swap(f.off.bytes, sd_spare.off.bytes);
+        std::swap(f.pts.p, sd_spare.pts.p);
+        std::swap(f.pts.bytes, sd_spare.pts.bytes);
+        std::swap(f.cap, sd_spare.cap);
     }
 
     static double pop_std(const std::vector<int64_t> &x) {
@@ -1283,6 +1288,11 @@ struct Run {
         DevFrontier td, bu, tmp;
         seed_frontier(td, d, dd->layer_size[0], dd->root, s);
         seed_frontier(bu, d, dd->layer_size[L - 1], dd->terminal, s);
+        // knapsack state dominance on the top-down sweep (the reference packs
+        // its metadata for method 3 but does not apply it, enum.cu:618): the
+        // rule is sound, so the frontier is unchanged and the join smaller
+        const bool sd = opts.state_dominance > 0 && opts.problem_type == 1 &&
+                        dd->kind == PMDD_DD_BDD && !dd->min_weight.empty();
         int lt = 0, lb = L - 1;
         int64_t td_score = 0, bu_score = 0;
         while (lt != lb) {
@@ -1291,6 +1301,9 @@ struct Run {
                 ++lt;
                 expand(td, dd->td[lt], dd->layer_size[lt], tmp, &cand, nullptr);
                 std::swap(td, tmp);
+                if (sd) {
+                    state_dominance(lt, td);
+                }
                 count_layer(cand, td.total());
                 const LayerCSR &outs = dd->bu[lt];
                 int64_t score = 0;
