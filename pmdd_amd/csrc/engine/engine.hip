@@ -112,6 +112,48 @@ struct ChunkBufs {
     DevBuf chunks, nsurv, nalive, minkey, maxkey, out;
 };
 
+// Pinned host memory for the small host<->device copies of an enumeration
+// (chunk tables, offsets, counts): copies from it are asynchronous, so the
+// host queues a layer's work without waiting.  A bump allocator: regions are
+// handed out until the next stream synchronization, which frees them all.
+struct Staging {
+    char *base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    Staging() = default;
+    Staging(const Staging &) = delete;
+    Staging &operator=(const Staging &) = delete;
+    ~Staging() {
+        if (base != nullptr) {
+            (void)hipHostFree(base);
+        }
+    }
+    // n bytes (256-aligned), or nullptr when the arena has no room left.
+    void *take(size_t n) {
+        const size_t at = (used + 255) & ~static_cast<size_t>(255);
+        if (base == nullptr || at + n > cap) {
+            return nullptr;
+        }
+        used = at + n;
+        return base + at;
+    }
+    // Only when no copy from the arena is in flight.
+    void reset() { used = 0; }
+    void grow(size_t need) {
+        if (base != nullptr) {
+            (void)hipHostFree(base);
+            base = nullptr;
+        }
+        cap = std::max<size_t>({need, 2 * cap, size_t(1) << 20});
+        if (hipHostMalloc(reinterpret_cast<void **>(&base), cap, hipHostMallocDefault) !=
+            hipSuccess) {
+            base = nullptr;
+            cap = 0;
+        }
+        used = 0;
+    }
+};
+
 struct pmdd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -124,6 +166,7 @@ struct pmdd_ctx {
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
     DevBuf join_idx, join_x, join_y, join_ctr, front_count;
     std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
+    Staging staging;
 };
 
 struct LayerCSR {
@@ -406,27 +449,7 @@ void grow_points(DevFrontier &f, int d, int64_t need, int64_t keep, hipStream_t 
     hip_check(hipStreamSynchronize(s), "frontier grow sync");
 }
 
-void upload_offsets(DevFrontier &f, hipStream_t s) {
-    f.off.ensure(f.h_off.size() * 8, "hipMalloc(frontier offsets)");
-    hip_check(hipMemcpyAsync(f.off.p, f.h_off.data(), f.h_off.size() * 8, hipMemcpyHostToDevice,
-                             s),
-              "hipMemcpy(frontier offsets)");
-}
 
-// Frontier holding {0^d} at one node of a layer, empty elsewhere.
-void seed_frontier(DevFrontier &f, int d, int64_t n_nodes, int64_t node, hipStream_t s) {
-    f.n_nodes = n_nodes;
-    f.h_off.assign(n_nodes + 1, 0);
-    for (int64_t v = node + 1; v <= n_nodes; ++v) {
-        f.h_off[v] = 1;
-    }
-    f.cap = 0;
-    f.pts.release();
-    grow_points(f, d, 1, 0, s);
-    hip_check(hipMemsetAsync(f.pts.p, 0, static_cast<size_t>(f.cap) * d * 4, s),
-              "hipMemset(seed)");
-    upload_offsets(f, s);
-}
 
 // ---------------------------------------------------------------------------
 // Kernel launchers, one instantiation per objective count.
@@ -602,21 +625,71 @@ inline int ceil_log2(int64_t x) {
     return p;
 }
 
-template <class T> void h2d(DevBuf &b, const std::vector<T> &v, hipStream_t s, const char *what) {
+// A pinned region of n bytes of the context's staging arena; when the arena
+// is full, waits for the stream (every region is then free) and grows it.
+inline void *staging_take(pmdd_ctx *ctx, size_t n) {
+    void *p = ctx->staging.take(n);
+    if (p == nullptr) {
+        hip_check(hipStreamSynchronize(ctx->stream), "staging sync");
+        if (n + 256 > ctx->staging.cap) {
+            ctx->staging.grow(n + 256);
+        } else {
+            ctx->staging.reset();
+        }
+        p = ctx->staging.take(n);
+        if (p == nullptr) {
+            fail(PMDD_E_NOMEM, "hipHostMalloc(staging) failed");
+        }
+    }
+    return p;
+}
+
+template <class T>
+void h2d(pmdd_ctx *ctx, DevBuf &b, const std::vector<T> &v, const char *what) {
     b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T), what);
     if (!v.empty()) {
-        hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s),
-                  what);
+        const size_t bytes = v.size() * sizeof(T);
+        void *p = staging_take(ctx, bytes);
+        std::memcpy(p, v.data(), bytes);
+        hip_check(hipMemcpyAsync(b.p, p, bytes, hipMemcpyHostToDevice, ctx->stream), what);
     }
 }
 
-template <class T> void d2h(std::vector<T> &v, const DevBuf &b, size_t n, hipStream_t s,
-                            const char *what) {
+// Copies n elements back and waits for the stream, which frees the staging.
+template <class T>
+void d2h(pmdd_ctx *ctx, std::vector<T> &v, const DevBuf &b, size_t n, const char *what) {
     v.resize(n);
+    void *p = nullptr;
     if (n > 0) {
-        hip_check(hipMemcpyAsync(v.data(), b.p, n * sizeof(T), hipMemcpyDeviceToHost, s), what);
+        p = staging_take(ctx, n * sizeof(T));
+        hip_check(hipMemcpyAsync(p, b.p, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
+                  what);
     }
-    hip_check(hipStreamSynchronize(s), what);
+    hip_check(hipStreamSynchronize(ctx->stream), what);
+    if (n > 0) {
+        std::memcpy(v.data(), p, n * sizeof(T));
+    }
+    ctx->staging.reset();
+}
+
+void upload_offsets(pmdd_ctx *ctx, DevFrontier &f) {
+    h2d(ctx, f.off, f.h_off, "hipMemcpy(frontier offsets)");
+}
+
+// Frontier holding {0^d} at one node of a layer, empty elsewhere.
+void seed_frontier(pmdd_ctx *ctx, DevFrontier &f, int d, int64_t n_nodes, int64_t node) {
+    hipStream_t s = ctx->stream;
+    f.n_nodes = n_nodes;
+    f.h_off.assign(n_nodes + 1, 0);
+    for (int64_t v = node + 1; v <= n_nodes; ++v) {
+        f.h_off[v] = 1;
+    }
+    f.cap = 0;
+    f.pts.release();
+    grow_points(f, d, 1, 0, s);
+    hip_check(hipMemsetAsync(f.pts.p, 0, static_cast<size_t>(f.cap) * d * 4, s),
+              "hipMemset(seed)");
+    upload_offsets(ctx, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -695,7 +768,7 @@ struct Run {
     // ---- chunk tables ----------------------------------------------------
     void upload_table(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks) {
         const size_t n = std::max<size_t>(chunks.size(), 1);
-        h2d(cb.chunks, chunks, s, "hipMemcpy(chunk table)");
+        h2d(ctx, cb.chunks, chunks, "hipMemcpy(chunk table)");
         cb.nsurv.ensure(n * 4, "hipMalloc(chunk counts)");
         cb.nalive.ensure(n * 4, "hipMalloc(chunk counts)");
         cb.minkey.ensure(n * 8, "hipMalloc(chunk keys)");
@@ -709,7 +782,7 @@ struct Run {
         const int32_t *w = nullptr;
         int64_t n = count;
         if (!which.empty()) {
-            h2d(wbuf, which, s, "hipMemcpy(chunk list)");
+            h2d(ctx, wbuf, which, "hipMemcpy(chunk list)");
             w = wbuf.as<int32_t>();
             n = static_cast<int64_t>(which.size());
         }
@@ -728,7 +801,7 @@ struct Run {
         if (which.empty()) {
             return;
         }
-        h2d(wbuf, which, s, "hipMemcpy(chunk list)");
+        h2d(ctx, wbuf, which, "hipMemcpy(chunk list)");
         K.stage(dim3(static_cast<unsigned>(which.size())), s, wbuf.as<int32_t>(),
                 cb.chunks.as<ChunkDesc>(), src, out, stride, alive, cb.nsurv.as<int32_t>(),
                 cb.nalive.as<int32_t>());
@@ -770,7 +843,7 @@ struct Run {
             }
         }
         start[waves] = lists.size();
-        h2d(ctx->which_c, lists, s, "hipMemcpy(wave lists)");
+        h2d(ctx, ctx->which_c, lists, "hipMemcpy(wave lists)");
         // front counts, double-buffered by wave, indexed by a group's first chunk
         const size_t nc = chunks.size();
         ctx->front_count.ensure(2 * nc * sizeof(int64_t), "hipMalloc(front counts)");
@@ -842,7 +915,7 @@ struct Run {
         if (nch <= 0) {
             return;
         }
-        h2d(cb.out, outpos, s, "hipMemcpy(chunk outputs)");
+        h2d(ctx, cb.out, outpos, "hipMemcpy(chunk outputs)");
         K.compact(dim3(static_cast<unsigned>(nch)), s, cb.chunks.as<ChunkDesc>(),
                   cb.nsurv.as<int32_t>(), cb.out.as<int64_t>(), src, ss, alive, dst, ds);
         hip_check(hipGetLastError(), "k_compact_chunks");
@@ -856,8 +929,8 @@ struct Run {
         if (passes == 0 || total == 0) {
             return a.as<int32_t>();
         }
-        h2d(ctx->run_start, run_start, s, "hipMemcpy(run table)");
-        h2d(ctx->grp_run, grp_run, s, "hipMemcpy(group table)");
+        h2d(ctx, ctx->run_start, run_start, "hipMemcpy(run table)");
+        h2d(ctx, ctx->grp_run, grp_run, "hipMemcpy(group table)");
         const int32_t G = static_cast<int32_t>(grp_run.size()) - 1;
         int32_t *cur = a.as<int32_t>();
         int32_t *oth = b.as<int32_t>();
@@ -896,7 +969,7 @@ struct Run {
             C += acc;
         }
         *cand_out = C;
-        h2d(ctx->run_off, run_off, s, "hipMemcpy(run offsets)");
+        h2d(ctx, ctx->run_off, run_off, "hipMemcpy(run offsets)");
 
         next.n_nodes = n_nodes;
         next.h_off.assign(n_nodes + 1, 0);
@@ -996,8 +1069,8 @@ struct Run {
                 big_base.push_back(bigpos);
                 ctx->sort_a.ensure(static_cast<size_t>(bigpos) * d * 4, "hipMalloc(sort buffer)");
                 ctx->sort_b.ensure(static_cast<size_t>(bigpos) * d * 4, "hipMalloc(sort buffer)");
-                h2d(ctx->big_base, big_base, s, "hipMemcpy(big nodes)");
-                h2d(ctx->big_node, big_node, s, "hipMemcpy(big nodes)");
+                h2d(ctx, ctx->big_base, big_base, "hipMemcpy(big nodes)");
+                h2d(ctx, ctx->big_node, big_node, "hipMemcpy(big nodes)");
                 K.gather_runs(dim3(static_cast<unsigned>(ceil_div(bigpos, kBlock))), s,
                               ctx->big_base.as<int64_t>(), ctx->big_node.as<int32_t>(),
                               static_cast<int32_t>(big_node.size()), src,
@@ -1025,7 +1098,7 @@ struct Run {
                                               ctx->alive.as<uint8_t>(), sorted, bigpos);
             }
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-            d2h(nalive, cb.nalive, chunks.size(), s, "chunk survivor counts");
+            d2h(ctx, nalive, cb.nalive, chunks.size(), "chunk survivor counts");
             if (cross_waves > 0) {
                 st->kernel_cross_s += pool_seconds(cross_waves);
                 st->cross_launches += cross_waves;
@@ -1060,7 +1133,7 @@ struct Run {
             next_total = at;
             sample_memory();
         }
-        upload_offsets(next, s);
+        upload_offsets(ctx, next);
         st->algorithmic_bytes +=
             4 * (static_cast<int64_t>(d) * (prev.total() + next_total) + csr.n_arcs * (d + 1) +
                  2 * (n_nodes + 1) + (prev.n_nodes + 1));
@@ -1130,7 +1203,7 @@ struct Run {
         hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
         hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(total, 256))),
                            dim3(256), 0, s, ctx->keep_a.as<uint8_t>(), total, uint8_t(1));
-        h2d(ctx->which, targets, s, "hipMemcpy(dominance targets)");
+        h2d(ctx, ctx->which, targets, "hipMemcpy(dominance targets)");
         K.state_dom(dim3(static_cast<unsigned>(targets.size() / 4)), s, ctx->which.as<int32_t>(),
                     f.off.as<int64_t>(), f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>());
         hip_check(hipGetLastError(), "k_state_dominance");
@@ -1142,7 +1215,7 @@ struct Run {
         hip_check(hipGetLastError(), "k_count_kept");
         hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
         std::vector<int64_t> kept;
-        d2h(kept, ctx->join_meta, n, s, "kept counts");
+        d2h(ctx, kept, ctx->join_meta, n, "kept counts");
         const double ms = elapsed_ms();
         st->kernel_dominance_s += ms * 1e-3;
         st->kernel_total_s += ms * 1e-3;
@@ -1162,7 +1235,7 @@ struct Run {
         sd_spare.n_nodes = n;
         std::swap(sd_spare.h_off, g.h_off);
         grow_points(sd_spare, d, std::max<int64_t>(sd_spare.total(), 1), 0, s);
-        upload_offsets(sd_spare, s);
+        upload_offsets(ctx, sd_spare);
         K.compact_nodes(dim3(static_cast<unsigned>(n)), s, f.off.as<int64_t>(),
                         f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>(),
                         sd_spare.off.as<int64_t>(), sd_spare.pts.as<int32_t>(), sd_spare.cap);
@@ -1224,7 +1297,7 @@ struct Run {
     void topdown(pmdd_frontier *out) {
         const int L = dd->L;
         DevFrontier cur, nxt;
-        seed_frontier(cur, d, dd->layer_size[0], dd->root, s);
+        seed_frontier(ctx, cur, d, dd->layer_size[0], dd->root);
         const bool sd = opts.state_dominance > 0 && opts.problem_type == 1 &&
                         dd->kind == PMDD_DD_BDD && !dd->min_weight.empty();
         for (int l = 1; l < L; ++l) {
@@ -1258,7 +1331,7 @@ struct Run {
     void bottomup(pmdd_frontier *out) {
         const int L = dd->L;
         DevFrontier cur, nxt;
-        seed_frontier(cur, d, dd->layer_size[L - 1], dd->terminal, s);
+        seed_frontier(ctx, cur, d, dd->layer_size[L - 1], dd->terminal);
         for (int l = L - 2; l >= 0; --l) {
             int64_t cand = 0;
             std::vector<int64_t> gamma;
@@ -1286,8 +1359,8 @@ struct Run {
     void coupled(pmdd_frontier *out) {
         const int L = dd->L;
         DevFrontier td, bu, tmp;
-        seed_frontier(td, d, dd->layer_size[0], dd->root, s);
-        seed_frontier(bu, d, dd->layer_size[L - 1], dd->terminal, s);
+        seed_frontier(ctx, td, d, dd->layer_size[0], dd->root);
+        seed_frontier(ctx, bu, d, dd->layer_size[L - 1], dd->terminal);
         // knapsack state dominance on the top-down sweep (the reference packs
         // its metadata for method 3 but does not apply it, enum.cu:618): the
         // rule is sound, so the frontier is unchanged and the join smaller
@@ -1409,8 +1482,8 @@ struct Run {
                 batch_inputs += td.size(v) + bu.size(v);
             }
             i = j;
-            h2d(ctx->join_meta, meta, s, "hipMemcpy(batch table)");
-            h2d(ctx->join_idx, idx, s, "hipMemcpy(batch rows)");
+            h2d(ctx, ctx->join_meta, meta, "hipMemcpy(batch table)");
+            h2d(ctx, ctx->join_idx, idx, "hipMemcpy(batch rows)");
             pmdd_k::JoinSrc src;
             src.prod_off = ctx->join_meta.as<int64_t>();
             src.row_base = src.prod_off + Kn + 1;
@@ -1456,7 +1529,7 @@ struct Run {
                 // ctr[0]: products; ctr[1 + b]: survivors of block b
                 ctr.assign(1 + static_cast<size_t>(nbk), 0);
                 ctr[0] = static_cast<unsigned long long>(bprod);
-                h2d(ctx->join_ctr, ctr, s, "hipMemcpy(join counters)");
+                h2d(ctx, ctx->join_ctr, ctr, "hipMemcpy(join counters)");
                 unsigned long long *dc = ctx->join_ctr.as<unsigned long long>();
                 const unsigned grid = static_cast<unsigned>(
                     std::min<int64_t>(ceil_div(bprod, pmdd_k::kWave), kRefillWaves));
@@ -1470,7 +1543,7 @@ struct Run {
                     std::swap(in, outb);
                 }
                 hip_check(hipEventRecord(ctx->ev_f, s), "hipEventRecord");
-                d2h(ctr, ctx->join_ctr, ctr.size(), s, "join counters");
+                d2h(ctx, ctr, ctx->join_ctr, ctr.size(), "join counters");
                 F = in;
                 ytotal = static_cast<int64_t>(ctr[nbk]);
                 add_time(ctx->ev_e, ctx->ev_f, &st->kernel_frontier_check_s);
@@ -1518,7 +1591,7 @@ struct Run {
             run_filter(K.filter_dense, cb, ctx->which_a, none, cnt, &fsrc,
                        ctx->surv_pts.as<int32_t>(), ytotal, ctx->alive.as<uint8_t>());
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-            d2h(nalive, cb.nalive, chunks.size(), s, "product survivor counts");
+            d2h(ctx, nalive, cb.nalive, chunks.size(), "product survivor counts");
             add_time(ctx->ev_a, ctx->ev_c, &st->kernel_filter_s);
             add_time(ctx->ev_a, ctx->ev_c, &st->kernel_dominance_s);
             st->filter_launches += 1;
@@ -1573,7 +1646,7 @@ struct Run {
                 const int cross_waves = run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(),
                                                         ytotal, ctx->alive.as<uint8_t>(), ys,
                                                         ytotal);
-                d2h(nalive, cb.nalive, chunks.size(), s, "batch survivor counts");
+                d2h(ctx, nalive, cb.nalive, chunks.size(), "batch survivor counts");
                 if (cross_waves > 0) {
                     st->kernel_cross_s += pool_seconds(cross_waves);
                     st->cross_launches += cross_waves;
@@ -1629,7 +1702,7 @@ struct Run {
                 rlen[k] = c.len;
             }
             upload_table(cr, rchunks);
-            h2d(cr.nsurv, rlen, s, "hipMemcpy(frontier chunk sizes)");
+            h2d(ctx, cr.nsurv, rlen, "hipMemcpy(frontier chunk sizes)");
             ctx->keep_b.ensure(static_cast<size_t>(r_count), "hipMalloc(frontier flags)");
             hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(r_count, 256))),
                                dim3(256), 0, s, ctx->keep_b.as<uint8_t>(), r_count, uint8_t(1));
@@ -1640,7 +1713,7 @@ struct Run {
                         B.cap, btotal, d_tests);
             hip_check(hipGetLastError(), "k_vs_sorted");
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-            d2h(rnalive, cr.nalive, rchunks.size(), s, "frontier survivor counts");
+            d2h(ctx, rnalive, cr.nalive, rchunks.size(), "frontier survivor counts");
             add_time(ctx->ev_a, ctx->ev_b, &st->kernel_dominance_s);
             routpos.resize(rn);
             int64_t ralive = 0;
