@@ -161,7 +161,7 @@ struct pmdd_ctx {
     hipEvent_t ev_e = nullptr, ev_f = nullptr, ev_g = nullptr;
     // scratch reused across layers and calls
     ChunkBufs cb_main, cb_aux;
-    DevBuf run_off, which, which_a, which_b, which_c;
+    DevBuf run_off, which, which_a, which_b, which_c, which_s;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
     DevBuf join_idx, join_x, join_y, join_ctr, front_count;
@@ -470,6 +470,7 @@ using VsSortedFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc 
 struct Launch {
     int chunk_cap;
     FilterFn filter_expand;  // ExpandSrc, presorted in LDS
+    FilterFn filter_expand_small; // the same for nodes of at most kSmallNode candidates
     FilterFn filter_join;    // JoinSrc, presorted in LDS
     FilterFn filter_sorted;  // SortedSrc, source already in key order
     FilterFn filter_dense;   // SortedSrc in any order, presorted in LDS
@@ -496,6 +497,11 @@ struct Launch {
                       unsigned long long *, unsigned long long *);
 };
 
+// Nodes of at most kSmallNode candidates are filtered by an instantiation with
+// an LDS footprint of kSmallNode points (about 9 KB instead of 40-72 KB), so
+// that many of their workgroups fit one compute unit at once.
+constexpr int kSmallNode = 256;
+
 // Coupling: blocks of R after the first 8192 points hold kRefillBlock points;
 // k_refill_vs_block runs at most kRefillWaves one-wave workgroups (16 per CU).
 constexpr int64_t kRefillBlock = 8192;
@@ -505,11 +511,11 @@ constexpr int64_t kJoinBatchCap = int64_t(1) << 25;
 
 template <int D> struct LaunchImpl {
     static constexpr int CH = pmdd_k::Cap<D>::CH;
-    template <class Src, bool PRE>
+    template <class Src, bool PRE, int C = CH>
     static void filter(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                        const void *src, int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns,
                        int32_t *na, int64_t *kmin, int64_t *kmax, unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, CH, Src, PRE>), g, dim3(kBlock), 0, s,
+        hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, C, Src, PRE>), g, dim3(kBlock), 0, s,
                            which, c, *static_cast<const Src *>(src), out, stride, alive, ns, na,
                            kmin, kmax, pt);
     }
@@ -588,6 +594,7 @@ template <int D> struct LaunchImpl {
         L.filter_dense = filter<pmdd_k::SortedSrc, false>;
         L.chunk_cap = CH;
         L.filter_expand = filter<pmdd_k::ExpandSrc, false>;
+        L.filter_expand_small = filter<pmdd_k::ExpandSrc, false, kSmallNode>;
         L.filter_join = filter<pmdd_k::JoinSrc, false>;
         L.filter_sorted = filter<pmdd_k::SortedSrc, true>;
         L.cross = cross;
@@ -992,7 +999,7 @@ struct Run {
         ChunkBufs &cb = ctx->cb_main;
         int64_t v = 0;
         std::vector<ChunkDesc> chunks;
-        std::vector<int32_t> which_small, which_big, which_cross, nalive;
+        std::vector<int32_t> which_small, which_tiny, which_big, which_cross, nalive;
         std::vector<int64_t> chunk_out, big_base, run_start;
         std::vector<int32_t> big_node, grp_run;
         while (v < n_nodes) {
@@ -1005,6 +1012,7 @@ struct Run {
             const int64_t v1 = v;
             chunks.clear();
             which_small.clear();
+            which_tiny.clear();
             which_big.clear();
             which_cross.clear();
             big_base.clear();
@@ -1026,7 +1034,7 @@ struct Run {
                     c.group = static_cast<int32_t>(u);
                     c.group_first = first;
                     c.group_count = 1;
-                    which_small.push_back(first);
+                    (g <= kSmallNode ? which_tiny : which_small).push_back(first);
                     chunks.push_back(c);
                 } else {
                     big_node.push_back(static_cast<int32_t>(u));
@@ -1082,6 +1090,8 @@ struct Run {
             upload_table(cb, chunks);
             ctx->surv_pts.ensure(static_cast<size_t>(pass_c) * d * 4, "hipMalloc(survivor buffer)");
             ctx->alive.ensure(static_cast<size_t>(pass_c), "hipMalloc(alive flags)");
+            run_filter(K.filter_expand_small, cb, ctx->which_s, which_tiny, 0, &src,
+                       ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
             run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
                        ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
             if (!which_big.empty()) {
