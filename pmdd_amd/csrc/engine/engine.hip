@@ -161,7 +161,7 @@ struct pmdd_ctx {
     hipEvent_t ev_e = nullptr, ev_f = nullptr, ev_g = nullptr;
     // scratch reused across layers and calls
     ChunkBufs cb_main, cb_aux;
-    DevBuf run_off, which, which_a, which_b, which_c, which_s;
+    DevBuf run_off, which, which_a, which_b, which_c, which_s, which_m;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
     DevBuf join_idx, join_x, join_y, join_ctr, front_count;
@@ -471,6 +471,8 @@ struct Launch {
     int chunk_cap;
     FilterFn filter_expand;  // ExpandSrc, presorted in LDS
     FilterFn filter_expand_small; // the same for nodes of at most kSmallNode candidates
+    FilterFn filter_expand_mid;   // ... and of at most mid_cap candidates (0: no such tier)
+    int mid_cap;
     FilterFn filter_join;    // JoinSrc, presorted in LDS
     FilterFn filter_sorted;  // SortedSrc, source already in key order
     FilterFn filter_dense;   // SortedSrc in any order, presorted in LDS
@@ -501,6 +503,7 @@ struct Launch {
 // an LDS footprint of kSmallNode points (about 9 KB instead of 40-72 KB), so
 // that many of their workgroups fit one compute unit at once.
 constexpr int kSmallNode = 256;
+constexpr int kMidNode = 1024; // a middle tier where a full chunk holds 2048 points
 
 // Coupling: blocks of R after the first 8192 points hold kRefillBlock points;
 // k_refill_vs_block runs at most kRefillWaves one-wave workgroups (16 per CU).
@@ -595,6 +598,8 @@ template <int D> struct LaunchImpl {
         L.chunk_cap = CH;
         L.filter_expand = filter<pmdd_k::ExpandSrc, false>;
         L.filter_expand_small = filter<pmdd_k::ExpandSrc, false, kSmallNode>;
+        L.mid_cap = CH > kMidNode ? kMidNode : 0;
+        L.filter_expand_mid = filter<pmdd_k::ExpandSrc, false, (CH > kMidNode ? kMidNode : CH)>;
         L.filter_join = filter<pmdd_k::JoinSrc, false>;
         L.filter_sorted = filter<pmdd_k::SortedSrc, true>;
         L.cross = cross;
@@ -999,7 +1004,7 @@ struct Run {
         ChunkBufs &cb = ctx->cb_main;
         int64_t v = 0;
         std::vector<ChunkDesc> chunks;
-        std::vector<int32_t> which_small, which_tiny, which_big, which_cross, nalive;
+        std::vector<int32_t> which_small, which_tiny, which_mid, which_big, which_cross, nalive;
         std::vector<int64_t> chunk_out, big_base, run_start;
         std::vector<int32_t> big_node, grp_run;
         while (v < n_nodes) {
@@ -1013,6 +1018,7 @@ struct Run {
             chunks.clear();
             which_small.clear();
             which_tiny.clear();
+            which_mid.clear();
             which_big.clear();
             which_cross.clear();
             big_base.clear();
@@ -1034,7 +1040,10 @@ struct Run {
                     c.group = static_cast<int32_t>(u);
                     c.group_first = first;
                     c.group_count = 1;
-                    (g <= kSmallNode ? which_tiny : which_small).push_back(first);
+                    (g <= kSmallNode                      ? which_tiny
+                     : (K.mid_cap > 0 && g <= K.mid_cap) ? which_mid
+                                                         : which_small)
+                        .push_back(first);
                     chunks.push_back(c);
                 } else {
                     big_node.push_back(static_cast<int32_t>(u));
@@ -1091,6 +1100,8 @@ struct Run {
             ctx->surv_pts.ensure(static_cast<size_t>(pass_c) * d * 4, "hipMalloc(survivor buffer)");
             ctx->alive.ensure(static_cast<size_t>(pass_c), "hipMalloc(alive flags)");
             run_filter(K.filter_expand_small, cb, ctx->which_s, which_tiny, 0, &src,
+                       ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
+            run_filter(K.filter_expand_mid, cb, ctx->which_m, which_mid, 0, &src,
                        ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
             run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
                        ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
