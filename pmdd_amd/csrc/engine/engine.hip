@@ -164,7 +164,7 @@ struct pmdd_ctx {
     DevBuf run_off, which, which_a, which_b, which_c, which_s, which_m;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
-    DevBuf join_idx, join_x, join_y, join_ctr, front_count;
+    DevBuf join_idx, join_x, join_y, join_ctr, front_count, merge_a, merge_b;
     std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
     Staging staging;
 };
@@ -463,9 +463,6 @@ using CrossFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, 
 using AppendFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
                           int64_t, const uint8_t *, const int32_t *, int32_t *, int32_t *, int64_t,
                           const int64_t *, int64_t *, int);
-using VsSortedFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *,
-                            const int32_t *, int64_t, uint8_t *, const int32_t *, int32_t *,
-                            const int32_t *, int64_t, int64_t, unsigned long long *);
 
 struct Launch {
     int chunk_cap;
@@ -480,8 +477,6 @@ struct Launch {
     AppendFn append_front;   // a wave's survivors onto their group's front
     void (*stage)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, pmdd_k::SortedSrc,
                   int32_t *, int64_t, uint8_t *, int32_t *, int32_t *);
-    VsSortedFn vs_weak;
-    VsSortedFn vs_strict;
     void (*compact)(dim3, hipStream_t, const ChunkDesc *, const int32_t *, const int64_t *,
                     const int32_t *, int64_t, const uint8_t *, int32_t *, int64_t);
     void (*state_dom)(dim3, hipStream_t, const int32_t *, const int64_t *, const int32_t *,
@@ -497,6 +492,9 @@ struct Launch {
     void (*refill_vs)(dim3, hipStream_t, const int32_t *, int64_t, const unsigned long long *,
                       const int32_t *, int64_t, int64_t, int64_t, int32_t *, int64_t,
                       unsigned long long *, unsigned long long *);
+    void (*refill_flags)(dim3, hipStream_t, const int32_t *, int64_t,
+                         const unsigned long long *, const int32_t *, int64_t, int64_t, int64_t,
+                         uint8_t *, unsigned long long *);
 };
 
 // Nodes of at most kSmallNode candidates are filtered by an instantiation with
@@ -509,6 +507,8 @@ constexpr int kMidNode = 1024; // a middle tier where a full chunk holds 2048 po
 // k_refill_vs_block runs at most kRefillWaves one-wave workgroups (16 per CU).
 constexpr int64_t kRefillBlock = 8192;
 constexpr int64_t kRefillWaves = 4096;
+// Blocks of a batch frontier when it removes the running frontier's points.
+constexpr int64_t kEvictBlock = 2048;
 // Largest coupling batch (products) once the first batches have built R.
 constexpr int64_t kJoinBatchCap = int64_t(1) << 25;
 
@@ -535,14 +535,6 @@ template <int D> struct LaunchImpl {
                              const int64_t *fin, int64_t *fout, int W) {
         hipLaunchKernelGGL((pmdd_k::k_append_front<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
                            stride, alive, ns, na, fv, fvs, fin, fout, W);
-    }
-    template <bool STRICT>
-    static void vs(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
-                   const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
-                   int32_t *na, const int32_t *B, int64_t bs, int64_t nb,
-                   unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_vs_sorted<D, CH, STRICT>), g, dim3(kBlock), 0, s, which, c,
-                           pts, stride, alive, ns, na, B, bs, nb, pt);
     }
     static void compact(dim3 g, hipStream_t s, const ChunkDesc *c, const int32_t *ns,
                         const int64_t *out, const int32_t *src, int64_t ss, const uint8_t *alive,
@@ -580,8 +572,14 @@ template <int D> struct LaunchImpl {
                           const unsigned long long *n_in, const int32_t *R, int64_t rs, int64_t lo,
                           int64_t hi, int32_t *Y, int64_t ys, unsigned long long *n_out,
                           unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D>), g, dim3(pmdd_k::kWave), 0, s, X, xs,
-                           n_in, R, rs, lo, hi, Y, ys, n_out, pt);
+        hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D, false>), g, dim3(pmdd_k::kWave), 0, s,
+                           X, xs, n_in, R, rs, lo, hi, Y, ys, n_out, nullptr, pt);
+    }
+    static void refill_flags(dim3 g, hipStream_t s, const int32_t *X, int64_t xs,
+                             const unsigned long long *n_in, const int32_t *R, int64_t rs,
+                             int64_t lo, int64_t hi, uint8_t *flag, unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D, true>), g, dim3(pmdd_k::kWave), 0, s, X,
+                           xs, n_in, R, rs, lo, hi, nullptr, 0, nullptr, flag, pt);
     }
     static void stage(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       pmdd_k::SortedSrc src, int32_t *out, int64_t stride, uint8_t *alive,
@@ -594,6 +592,7 @@ template <int D> struct LaunchImpl {
         L.stage = stage;
         L.join_products = join_products;
         L.refill_vs = refill_vs;
+        L.refill_flags = refill_flags;
         L.filter_dense = filter<pmdd_k::SortedSrc, false>;
         L.chunk_cap = CH;
         L.filter_expand = filter<pmdd_k::ExpandSrc, false>;
@@ -604,8 +603,6 @@ template <int D> struct LaunchImpl {
         L.filter_sorted = filter<pmdd_k::SortedSrc, true>;
         L.cross = cross;
         L.append_front = append_front;
-        L.vs_weak = vs<false>;
-        L.vs_strict = vs<true>;
         L.compact = compact;
         L.state_dom = state_dom;
         L.compact_nodes = compact_nodes;
@@ -1727,12 +1724,27 @@ struct Run {
             ctx->keep_b.ensure(static_cast<size_t>(r_count), "hipMalloc(frontier flags)");
             hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(r_count, 256))),
                                dim3(256), 0, s, ctx->keep_b.as<uint8_t>(), r_count, uint8_t(1));
+            // B passed the check against this R, so no point of B equals one of
+            // R, and "strictly dominated by B" is "weakly dominated by B":
+            // the refill check in flag mode, R in place, B in key-ordered blocks
+            h2d(ctx, ctx->join_ctr, std::vector<unsigned long long>{
+                                        static_cast<unsigned long long>(r_count)},
+                "hipMemcpy(frontier count)");
             hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            K.vs_strict(dim3(static_cast<unsigned>(rn)), s, nullptr, cr.chunks.as<ChunkDesc>(),
-                        R.pts.as<int32_t>(), R.cap, ctx->keep_b.as<uint8_t>(),
-                        cr.nsurv.as<int32_t>(), cr.nalive.as<int32_t>(), B.pts.as<int32_t>(),
-                        B.cap, btotal, d_tests);
-            hip_check(hipGetLastError(), "k_vs_sorted");
+            const unsigned rgrid = static_cast<unsigned>(
+                std::min<int64_t>(ceil_div(r_count, pmdd_k::kWave), kRefillWaves));
+            for (int64_t lo = 0; lo < btotal; lo += kEvictBlock) {
+                K.refill_flags(dim3(rgrid), s, R.pts.as<int32_t>(), R.cap,
+                               ctx->join_ctr.as<unsigned long long>(), B.pts.as<int32_t>(), B.cap,
+                               lo, std::min<int64_t>(btotal, lo + kEvictBlock),
+                               ctx->keep_b.as<uint8_t>(), d_tests);
+                hip_check(hipGetLastError(), "k_refill_vs_block");
+            }
+            hipLaunchKernelGGL(pmdd_k::k_count_chunks, dim3(static_cast<unsigned>(rn)),
+                               dim3(kBlock), 0, s, cr.chunks.as<ChunkDesc>(),
+                               cr.nsurv.as<int32_t>(), ctx->keep_b.as<uint8_t>(),
+                               cr.nalive.as<int32_t>());
+            hip_check(hipGetLastError(), "k_count_chunks");
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
             d2h(ctx, rnalive, cr.nalive, rchunks.size(), "frontier survivor counts");
             add_time(ctx->ev_a, ctx->ev_b, &st->kernel_dominance_s);
@@ -1743,33 +1755,32 @@ struct Run {
                 ralive += rnalive[k];
             }
             const int64_t total_new = ralive + btotal;
-            DevFrontier M, Rn;
-            grow_points(M, d, total_new, 0, s);
-            grow_points(Rn, d, total_new, 0, s);
+            // two reused buffers, stride total_new (the merge pass's stride)
+            DevBuf &M = ctx->merge_a;
+            DevBuf &Rn = ctx->merge_b;
+            M.ensure(static_cast<size_t>(total_new) * d * 4, "hipMalloc(frontier merge)");
+            Rn.ensure(static_cast<size_t>(total_new) * d * 4, "hipMalloc(frontier merge)");
             hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
             run_compact(cr, rn, routpos, R.pts.as<int32_t>(), R.cap, ctx->keep_b.as<uint8_t>(),
-                        M.pts.as<int32_t>(), M.cap);
+                        M.as<int32_t>(), total_new);
             for (int o = 0; o < d; ++o) {
-                hip_check(hipMemcpyAsync(M.pts.as<int32_t>() + o * M.cap + ralive,
+                hip_check(hipMemcpyAsync(M.as<int32_t>() + o * total_new + ralive,
                                          B.pts.as<int32_t>() + o * B.cap,
                                          static_cast<size_t>(btotal) * 4, hipMemcpyDeviceToDevice,
                                          s),
                           "hipMemcpy(batch frontier)");
             }
-            // both buffers have stride total_new for the merge pass
             std::vector<int64_t> rs = {0, ralive, total_new};
             std::vector<int32_t> gr = {0, 2};
-            int32_t *merged = merge_runs(M.pts, Rn.pts, total_new, rs, gr, 2);
+            int32_t *merged = merge_runs(M, Rn, total_new, rs, gr, 2);
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
             hip_check(hipStreamSynchronize(s), "frontier merge sync");
             add_time(ctx->ev_a, ctx->ev_b, &st->kernel_expand_td_s);
-            if (merged == M.pts.as<int32_t>()) {
-                std::swap(R.pts, M.pts);
-                R.cap = M.cap;
-            } else {
-                std::swap(R.pts, Rn.pts);
-                R.cap = Rn.cap;
-            }
+            // R takes the merged buffer; its old buffer is kept for the next batch
+            DevBuf &won = (merged == M.as<int32_t>()) ? M : Rn;
+            std::swap(R.pts.p, won.p);
+            std::swap(R.pts.bytes, won.bytes);
+            R.cap = total_new;
             r_count = total_new;
             sample_memory();
         }

@@ -960,125 +960,6 @@ __global__ __launch_bounds__(kBlock) void k_merge_pass(
     }
 }
 
-// ---------------------------------------------------------------------------
-// k_vs_sorted: kill chunk survivors dominated by a set B in key order.
-// STRICT=false: x dies when some b >= x everywhere (equal points die).
-// STRICT=true:  x dies when some b >= x everywhere and b != x.
-// B is scanned from its largest key down and the scan stops at the first tile
-// whose largest key is too small to dominate any live point of the chunk.
-// ---------------------------------------------------------------------------
-template <int D, int CH, bool STRICT>
-__global__ __launch_bounds__(kBlock) void k_vs_sorted(
-    const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
-    const int32_t *__restrict__ pts, int64_t stride, uint8_t *__restrict__ alive,
-    const int32_t *__restrict__ chunk_nsurv, int32_t *__restrict__ chunk_nalive,
-    const int32_t *__restrict__ B, int64_t bstride, int64_t nb,
-    unsigned long long *__restrict__ pair_tests) {
-    constexpr int DP = Pad<D>::DP;
-    constexpr int R = CH / kBlock;
-    __shared__ __align__(16) int32_t T[kBlock * DP];
-    __shared__ unsigned long long min_word;
-    __shared__ int live_count;
-    const int me = which != nullptr ? which[blockIdx.x] : static_cast<int>(blockIdx.x);
-    const ChunkDesc c = chunks[me];
-    const int nS = chunk_nsurv[me];
-    const int tid = threadIdx.x;
-    int32_t m[R][D];
-    bool dead[R];
-    int64_t my_min = INT64_MAX;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int s = r * kBlock + tid;
-        dead[r] = s >= nS || alive[c.out_base + s] == 0;
-        int64_t k = 0;
-#pragma unroll
-        for (int o = 0; o < D; ++o) {
-            m[r][o] = dead[r] ? 0 : pts[o * stride + c.out_base + s];
-            k += m[r][o];
-        }
-        if (!dead[r] && k < my_min) {
-            my_min = k;
-        }
-    }
-    if (tid == 0) {
-        min_word = ~0ULL;
-    }
-    __syncthreads();
-    if (my_min != INT64_MAX) {
-        atomicMin(&min_word, static_cast<unsigned long long>(my_min + (int64_t(1) << 62)));
-    }
-    __syncthreads();
-    const bool any = min_word != ~0ULL;
-    const int64_t wg_min = any ? static_cast<int64_t>(min_word) - (int64_t(1) << 62) : 0;
-    unsigned long long tests = 0;
-    for (int64_t tb = 0; any && tb < nb; tb += kBlock) {
-        const int64_t top = key_at<D>(B, bstride, tb);
-        if (STRICT ? (top <= wg_min) : (top < wg_min)) {
-            break;
-        }
-        const int tn = static_cast<int>(min(static_cast<int64_t>(kBlock), nb - tb));
-        if (tid < tn) {
-#pragma unroll
-            for (int o = 0; o < D; ++o) {
-                T[tid * DP + o] = B[o * bstride + tb + tid];
-            }
-        }
-        __syncthreads();
-        for (int j = 0; j < tn; ++j) {
-            const int32_t *q = &T[j * DP];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (STRICT) {
-                    bool ge = true;
-                    bool eq = true;
-#pragma unroll
-                    for (int o = 0; o < D; ++o) {
-                        ge &= q[o] >= m[r][o];
-                        eq &= q[o] == m[r][o];
-                    }
-                    dead[r] |= ge && !eq;
-                } else {
-                    dead[r] |= ge_all<D, DP>(q, m[r]);
-                }
-            }
-        }
-        if (pair_tests != nullptr) {
-            tests += static_cast<unsigned long long>(R) * tn;
-        }
-        bool any_live = false;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            any_live |= !dead[r];
-        }
-        if (!__syncthreads_or(any_live)) {
-            break;
-        }
-    }
-    int kept = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int s = r * kBlock + tid;
-        if (s < nS) {
-            if (dead[r]) {
-                alive[c.out_base + s] = 0;
-            } else {
-                ++kept;
-            }
-        }
-    }
-    if (tid == 0) {
-        live_count = 0;
-    }
-    __syncthreads();
-    atomicAdd(&live_count, kept);
-    __syncthreads();
-    if (tid == 0) {
-        chunk_nalive[me] = live_count;
-    }
-    if (pair_tests != nullptr) {
-        atomicAdd(pair_tests, tests);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // k_join_products: the n products of a coupling batch, written densely (SoA,
@@ -1119,12 +1000,15 @@ __global__ __launch_bounds__(kBlock) void k_join_products(JoinSrc src, int64_t n
 // A point whose key is above the block's largest key meets no test: no point
 // of the block can be >= it.
 // ---------------------------------------------------------------------------
-template <int D>
+//
+// FLAGS mode: X keeps its order; a point that dies gets flag[q] = 0 (points
+// whose flag is 0 already are skipped), and nothing is appended.
+template <int D, bool FLAGS>
 __global__ __launch_bounds__(kWave) void k_refill_vs_block(
     const int32_t *__restrict__ X, int64_t xstride, const unsigned long long *__restrict__ n_in,
     const int32_t *__restrict__ R, int64_t rstride, int64_t lo, int64_t hi,
     int32_t *__restrict__ Y, int64_t ystride, unsigned long long *__restrict__ n_out,
-    unsigned long long *__restrict__ pair_tests) {
+    uint8_t *__restrict__ flag, unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
     constexpr int kSub = 16;                  // rows between two refills
     constexpr int kSubPerTile = kWave / kSub; // sub-tiles per staged tile
@@ -1151,6 +1035,7 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
     int64_t t_next = 0;    // next position in the wave's stream
     bool exhausted = false;
     bool holds = false;    // lane holds a point
+    int64_t held = 0;      // its index in X
     int left = 0;          // sub-tiles the lane's point has still to meet
     int32_t m[D];
 #pragma unroll
@@ -1176,8 +1061,11 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
         __syncthreads();
         sb = 0;
     };
-    auto stage = [&](bool flag) {
-        const uint64_t mask = __ballot(flag);
+    auto stage = [&](bool keep) {
+        if (FLAGS) {
+            return;
+        }
+        const uint64_t mask = __ballot(keep);
         if (mask == 0) {
             return;
         }
@@ -1185,7 +1073,7 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
         if (sb + cnt > kStage) {
             flush();
         }
-        if (flag) {
+        if (keep) {
             const int pos = sb + __popcll(mask & below);
 #pragma unroll
             for (int o = 0; o < D; ++o) {
@@ -1208,7 +1096,9 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
             if (need) {
                 const int64_t t = t_next + __popcll(mask & below);
                 const int64_t q = (w + (t >> 6) * G) * kWave + (t & (kWave - 1));
-                if (q < n) {
+                if (q < n && FLAGS && flag[q] == 0) {
+                    // already dead: nothing to do, the lane stays empty
+                } else if (q < n) {
                     int64_t k = 0;
 #pragma unroll
                     for (int o = 0; o < D; ++o) {
@@ -1219,6 +1109,7 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
                         pass = true;
                     } else {
                         holds = true;
+                        held = q;
                         left = nsub;
                     }
                 } else {
@@ -1267,6 +1158,9 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
             --left;
             if (dead) {
                 holds = false;
+                if (FLAGS) {
+                    flag[held] = 0;
+                }
             } else if (left == 0) {
                 survive = true;
                 holds = false;
@@ -1275,7 +1169,7 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
         stage(survive);
         sub = (sub + 1 == nsub) ? 0 : sub + 1;
     }
-    if (sb > 0) {
+    if (!FLAGS && sb > 0) {
         flush();
     }
     if (pair_tests != nullptr) {
