@@ -235,6 +235,12 @@ int pmdd_server_poll(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_
 /* As pmdd_server_poll, but blocks until the request has finished. */
 int pmdd_server_wait(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_stats *stats,
                      char *err, size_t errlen);
+/* Blocks until some request has finished and collects the one that finished
+ * first among those not collected yet: *ticket names it, the return code is
+ * its own.  PMDD_E_INVALID when no request is outstanding.  Lets one thread
+ * collect results in completion order while others submit. */
+int pmdd_server_wait_any(pmdd_server *srv, int64_t *ticket, pmdd_frontier *out,
+                         pmdd_stats *stats, char *err, size_t errlen);
 /* Finishes the running requests, drops queued ones, joins the workers. */
 void pmdd_server_destroy(pmdd_server *srv);
 

@@ -146,6 +146,7 @@ ENGINE_SYMBOLS = [
     "pmdd_server_submit",
     "pmdd_server_poll",
     "pmdd_server_wait",
+    "pmdd_server_wait_any",
     "pmdd_server_destroy",
 ]
 HOST_SYMBOLS = [
@@ -239,6 +240,9 @@ def engine_lib() -> C.CDLL:
         for fn in (lib.pmdd_server_poll, lib.pmdd_server_wait):
             fn.argtypes = [C.c_void_p, C.c_int64, C.POINTER(Frontier), C.POINTER(Stats)] + err
             fn.restype = C.c_int
+        lib.pmdd_server_wait_any.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(Frontier),
+                                             C.POINTER(Stats)] + err
+        lib.pmdd_server_wait_any.restype = C.c_int
         lib.pmdd_server_destroy.argtypes = [C.c_void_p]
         lib.pmdd_server_destroy.restype = None
         _engine = lib

@@ -2122,6 +2122,7 @@ struct pmdd_server {
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<ServerJob>> queue;
     std::unordered_map<int64_t, std::shared_ptr<ServerJob>> jobs;
+    std::deque<int64_t> finished; // done, in completion order (may hold collected tickets)
     std::vector<pmdd_ctx *> ctxs;
     std::vector<std::thread> workers;
 
@@ -2150,6 +2151,7 @@ struct pmdd_server {
             lk.lock();
             in_use -= job->estimate;
             job->done = true;
+            finished.push_back(job->ticket);
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -2234,6 +2236,7 @@ int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_op
             job->err = "request needs about " + std::to_string(job->estimate) +
                        " bytes of device memory, more than the server budget of " +
                        std::to_string(srv->budget);
+            srv->finished.push_back(job->ticket);
             srv->cv_done.notify_all();
             return PMDD_OK;
         }
@@ -2243,22 +2246,9 @@ int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_op
     });
 }
 
-static int server_collect(pmdd_server *srv, int64_t ticket, bool block, pmdd_frontier *out,
-                          pmdd_stats *stats, char *err, size_t errlen) {
-    std::unique_lock<std::mutex> lk(srv->mu);
-    auto it = srv->jobs.find(ticket);
-    if (it == srv->jobs.end()) {
-        write_err(err, errlen, "unknown or already collected ticket");
-        return PMDD_E_INVALID;
-    }
-    std::shared_ptr<ServerJob> job = it->second;
-    if (block) {
-        srv->cv_done.wait(lk, [&] { return job->done; });
-    } else if (!job->done) {
-        return PMDD_PENDING;
-    }
-    srv->jobs.erase(it);
-    lk.unlock();
+// Moves a finished job's frontier and statistics to the caller.
+static int server_take(const std::shared_ptr<ServerJob> &job, pmdd_frontier *out,
+                       pmdd_stats *stats, char *err, size_t errlen) {
     if (out != nullptr) {
         *out = job->fr;
     } else {
@@ -2276,6 +2266,53 @@ static int server_collect(pmdd_server *srv, int64_t ticket, bool block, pmdd_fro
     }
     write_err(err, errlen, job->err);
     return job->rc;
+}
+
+static int server_collect(pmdd_server *srv, int64_t ticket, bool block, pmdd_frontier *out,
+                          pmdd_stats *stats, char *err, size_t errlen) {
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->jobs.find(ticket);
+    if (it == srv->jobs.end()) {
+        write_err(err, errlen, "unknown or already collected ticket");
+        return PMDD_E_INVALID;
+    }
+    std::shared_ptr<ServerJob> job = it->second;
+    if (block) {
+        srv->cv_done.wait(lk, [&] { return job->done; });
+    } else if (!job->done) {
+        return PMDD_PENDING;
+    }
+    srv->jobs.erase(ticket);
+    lk.unlock();
+    srv->cv_done.notify_all(); // a wait_any may be waiting for the last job
+    return server_take(job, out, stats, err, errlen);
+}
+
+int pmdd_server_wait_any(pmdd_server *srv, int64_t *ticket, pmdd_frontier *out,
+                         pmdd_stats *stats, char *err, size_t errlen) {
+    if (srv == nullptr || ticket == nullptr) {
+        write_err(err, errlen, "pmdd_server_wait_any: NULL argument");
+        return PMDD_E_INVALID;
+    }
+    std::unique_lock<std::mutex> lk(srv->mu);
+    std::shared_ptr<ServerJob> job;
+    while (job == nullptr) {
+        srv->cv_done.wait(lk, [&] { return !srv->finished.empty() || srv->jobs.empty(); });
+        if (srv->finished.empty()) {
+            write_err(err, errlen, "no outstanding request");
+            return PMDD_E_INVALID;
+        }
+        const int64_t t = srv->finished.front();
+        srv->finished.pop_front();
+        auto it = srv->jobs.find(t);
+        if (it != srv->jobs.end()) { // not collected by a wait or poll already
+            job = it->second;
+            srv->jobs.erase(it);
+        }
+    }
+    lk.unlock();
+    *ticket = job->ticket;
+    return server_take(job, out, stats, err, errlen);
 }
 
 int pmdd_server_poll(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_stats *stats,

@@ -17,6 +17,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import queue
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -73,6 +74,22 @@ class Server:
 
     def wait(self, ticket: int) -> EnumerationResult:
         return self._collect(self._lib.pmdd_server_wait, ticket)
+
+    def wait_any(self):
+        """Blocks until some request has finished; returns (ticket, result),
+        or (ticket, PmddError) when that request failed."""
+        fr = abi.Frontier()
+        st = abi.Stats()
+        t = C.c_int64(-1)
+        err = C.create_string_buffer(1024)
+        rc = self._lib.pmdd_server_wait_any(self._h, C.byref(t), C.byref(fr), C.byref(st), err,
+                                            len(err))
+        if t.value < 0:
+            raise PmddError(rc, err.value.decode())
+        if rc != abi.PMDD_OK:
+            return int(t.value), PmddError(rc, err.value.decode())
+        pts = sort_lexicographic_ascending(_take_frontier(self._lib, fr))
+        return int(t.value), EnumerationResult(pts, _stats_dict(st, 0, None, None))
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
@@ -136,35 +153,56 @@ def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots
                method: int = 1, device: int = 0, builders: int = 16,
                results: dict | None = None) -> dict:
     """Replay the stream: at each arrival the request's diagram is built on a
-    host thread and submitted; latency runs from arrival to result."""
+    host thread and submitted; one collector thread takes results as they
+    finish (pmdd_server_wait_any); latency runs from arrival to result."""
     server = Server(device, slots=slots)
     n = len(paths)
     done_at = [0.0] * n
     points = [0] * n
     errors = {}
+    owner = {}  # ticket -> request
     lock = threading.Lock()
+    events = queue.Queue()  # one item per request: True once submitted, False if not
     t0 = time.perf_counter()
 
-    def serve(i):
+    def submit(i):
         delay = t0 + arrivals[i] - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
         try:
             dd = DecisionDiagram.from_file(paths[i], specs[i]["kind"])
-            ticket = server.submit(dd, method=method)
-            dd.close()
-            res = server.wait(ticket)
             with lock:
-                points[i] = len(res.points)
-                if results is not None:
-                    results[i] = res.points
+                ticket = server.submit(dd, method=method)
+                owner[ticket] = i
+            dd.close()
+            events.put(True)
         except PmddError as e:
             with lock:
                 errors[i] = (e.code, e.message)
-        done_at[i] = time.perf_counter()
+            done_at[i] = time.perf_counter()
+            events.put(False)
 
-    with ThreadPoolExecutor(max_workers=max(builders, slots * 2, 8)) as pool:
-        list(pool.map(serve, range(n)))
+    def collect():
+        for _ in range(n):
+            if not events.get():
+                continue
+            ticket, res = server.wait_any()
+            now = time.perf_counter()
+            with lock:
+                i = owner[ticket]
+                if isinstance(res, PmddError):
+                    errors[i] = (res.code, res.message)
+                else:
+                    points[i] = len(res.points)
+                    if results is not None:
+                        results[i] = res.points
+            done_at[i] = now
+
+    collector = threading.Thread(target=collect)
+    collector.start()
+    with ThreadPoolExecutor(max_workers=builders) as pool:
+        list(pool.map(submit, range(n)))
+    collector.join()
     server.close()
     lat = np.array([done_at[i] - (t0 + arrivals[i]) for i in range(n) if i not in errors])
     span = max(done_at) - t0

@@ -65,3 +65,27 @@ def test_poll_then_collect(engine, tmp_path):
         assert _same(res.points, ref)
         with pytest.raises(abi.PmddError):
             srv.poll(t)  # collected tickets are retired
+
+
+def test_wait_any_collects_every_ticket_once(engine, tmp_path):
+    small = gen.write(gen.tsp(4, 8, 3), str(tmp_path / "t.dat"))
+    big = gen.write(gen.setpacking(6, 150, 4, m=30), str(tmp_path / "b.dat"))
+    dd_s = DecisionDiagram.from_file(small, 3)
+    dd_b = DecisionDiagram.from_file(big, 2)
+    ref = engine.enumerate(dd_s, method=1).points
+    with serving.Server(0, slots=2, memory_budget=64 << 20) as srv:
+        tickets = {srv.submit(dd_s, method=1): "ok", srv.submit(dd_b, method=1): "rejected",
+                   srv.submit(dd_s, method=3): "ok"}
+        seen = {}
+        for _ in range(3):
+            t, res = srv.wait_any()
+            seen[t] = res
+        assert set(seen) == set(tickets)
+        for t, kind in tickets.items():
+            if kind == "rejected":
+                assert isinstance(seen[t], abi.PmddError)
+                assert seen[t].code == abi.PMDD_E_REJECTED
+            else:
+                assert _same(seen[t].points, ref)
+        with pytest.raises(abi.PmddError):
+            srv.wait_any()  # nothing outstanding
