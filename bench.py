@@ -118,19 +118,23 @@ def serving_leg(tmpdir, requests, slots, method):
     res_plain, res_batched = {}, {}
     t0 = time.perf_counter()
     closed = [0.0] * requests
-    plain = serving.run_stream(paths, specs, closed, slots=1, method=method, results=res_plain)
+    progress = 200 if requests > 200 else 0
+    plain = serving.run_stream(paths, specs, closed, slots=1, method=method, results=res_plain,
+                               progress=progress)
     log(f"serving: plain closed loop {plain['inst_per_s']:.2f} inst/s")
     batched = serving.run_stream(paths, specs, closed, slots=slots, method=method,
-                                 results=res_batched)
+                                 results=res_batched, progress=progress)
     log(f"serving: batched closed loop {batched['inst_per_s']:.2f} inst/s")
     same = all(i in res_plain and i in res_batched and
                res_plain[i].shape == res_batched[i].shape and
                bool((res_plain[i] == res_batched[i]).all()) for i in range(requests))
     lam = plain["inst_per_s"]
     arrivals = serving.poisson_arrivals(requests, lam)
-    plain_o = serving.run_stream(paths, specs, arrivals, slots=1, method=method)
+    plain_o = serving.run_stream(paths, specs, arrivals, slots=1, method=method,
+                                 progress=progress)
     log("serving: plain open loop done")
-    batched_o = serving.run_stream(paths, specs, arrivals, slots=slots, method=method)
+    batched_o = serving.run_stream(paths, specs, arrivals, slots=slots, method=method,
+                                   progress=progress)
     log("serving: batched open loop done")
     out["closed_loop"] = {"plain": {k: plain[k] for k in keys},
                           "batched": {k: batched[k] for k in keys}}

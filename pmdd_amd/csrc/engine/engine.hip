@@ -13,10 +13,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <exception>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -292,7 +294,10 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
     r->td.resize(L);
     r->bu.resize(L);
     std::vector<std::vector<int32_t>> td_w(L), bu_w(L);
-    for (int l = 1; l < L; ++l) {
+    // layer l: its top-down CSR, and the bottom-up CSR of layer l-1 (the same
+    // arcs grouped by tail); layers are independent, so a few host threads
+    // take them in turn
+    auto pack_layer = [&](int l) {
         const int64_t n = r->layer_size[l];
         const int64_t g0 = v->layer_off[l];
         const int64_t a0 = v->in_off[g0];
@@ -304,48 +309,86 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
             c.off[i] = v->in_off[g0 + i] - a0;
         }
         c.src.assign(v->arc_src + a0, v->arc_src + a0 + E);
-        td_w[l].resize(static_cast<size_t>(d) * E);
+        std::vector<int32_t> &tw = td_w[l];
+        tw.resize(static_cast<size_t>(d) * E);
         for (int64_t e = 0; e < E; ++e) {
             for (int o = 0; o < d; ++o) {
-                td_w[l][o * E + e] = v->arc_w[(a0 + e) * d + o];
+                tw[o * E + e] = v->arc_w[(a0 + e) * d + o];
             }
         }
-        // bottom-up CSR of layer l-1: out-arcs grouped by tail, in the
-        // reference merge order (BDD: 1-arc first; MDD: by arc value)
+        // out-arcs grouped by tail in the reference merge order: BDD 1-arcs
+        // then 0-arcs, MDD by arc value, each in in-arc order
         LayerCSR &b = r->bu[l - 1];
         const int64_t np = r->layer_size[l - 1];
         b.n_arcs = E;
         b.off.assign(np + 1, 0);
-        std::vector<int64_t> head(E);
+        std::vector<int32_t> head(E);
         for (int64_t i = 0; i < n; ++i) {
             for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) {
-                head[e] = i;
-                b.off[c.src[e] + 1]++;
+                head[e] = static_cast<int32_t>(i);
             }
+        }
+        for (int64_t e = 0; e < E; ++e) {
+            b.off[c.src[e] + 1]++;
         }
         for (int64_t i = 0; i < np; ++i) {
             b.off[i + 1] += b.off[i];
         }
         std::vector<int64_t> order(E);
         std::vector<int64_t> pos(b.off.begin(), b.off.end() - 1);
-        for (int64_t e = 0; e < E; ++e) {
-            order[pos[c.src[e]]++] = e;
-        }
-        for (int64_t i = 0; i < np; ++i) {
-            std::stable_sort(order.begin() + b.off[i], order.begin() + b.off[i + 1],
-                             [&](int64_t x, int64_t y) {
-                                 const int32_t vx = v->arc_val[a0 + x], vy = v->arc_val[a0 + y];
-                                 return v->kind == PMDD_DD_BDD ? vx > vy : vx < vy;
-                             });
+        const int32_t *val = v->arc_val + a0;
+        if (v->kind == PMDD_DD_BDD) {
+            for (int t = 1; t >= 0; --t) {
+                for (int64_t e = 0; e < E; ++e) {
+                    if (val[e] == t) {
+                        order[pos[c.src[e]]++] = e;
+                    }
+                }
+            }
+        } else {
+            for (int64_t e = 0; e < E; ++e) {
+                order[pos[c.src[e]]++] = e;
+            }
+            for (int64_t i = 0; i < np; ++i) {
+                std::stable_sort(order.begin() + b.off[i], order.begin() + b.off[i + 1],
+                                 [&](int64_t x, int64_t y) { return val[x] < val[y]; });
+            }
         }
         b.src.resize(E);
-        bu_w[l - 1].resize(static_cast<size_t>(d) * E);
+        std::vector<int32_t> &bw = bu_w[l - 1];
+        bw.resize(static_cast<size_t>(d) * E);
         for (int64_t k = 0; k < E; ++k) {
             const int64_t e = order[k];
-            b.src[k] = static_cast<int32_t>(head[e]);
+            b.src[k] = head[e];
             for (int o = 0; o < d; ++o) {
-                bu_w[l - 1][o * E + k] = td_w[l][o * E + e];
+                bw[o * E + k] = tw[o * E + e];
             }
+        }
+    };
+    {
+        const int workers = static_cast<int>(std::min<unsigned>(
+            8, std::max(1u, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        std::atomic<int> next_layer(1);
+        std::exception_ptr failure;
+        std::mutex failure_mu;
+        for (int w = 0; w < workers; ++w) {
+            pool.emplace_back([&] {
+                try {
+                    for (int l = next_layer++; l < L; l = next_layer++) {
+                        pack_layer(l);
+                    }
+                } catch (...) {
+                    std::lock_guard<std::mutex> g(failure_mu);
+                    failure = std::current_exception();
+                }
+            });
+        }
+        for (std::thread &t : pool) {
+            t.join();
+        }
+        if (failure) {
+            std::rethrow_exception(failure);
         }
     }
     // knapsack metadata for state dominance (pack_bdd_layers enum.cu:496-540)

@@ -18,6 +18,7 @@ import ctypes as C
 import math
 import os
 import queue
+import sys
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -151,10 +152,11 @@ def write_stream(specs: list[dict], directory: str) -> list[str]:
 
 def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots: int,
                method: int = 1, device: int = 0, builders: int = 16,
-               results: dict | None = None) -> dict:
+               results: dict | None = None, progress: int = 0) -> dict:
     """Replay the stream: at each arrival the request's diagram is built on a
     host thread and submitted; one collector thread takes results as they
-    finish (pmdd_server_wait_any); latency runs from arrival to result."""
+    finish (pmdd_server_wait_any); latency runs from arrival to result.
+    With progress=k, a line goes to stderr after every k results."""
     server = Server(device, slots=slots)
     n = len(paths)
     done_at = [0.0] * n
@@ -183,7 +185,10 @@ def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots
             events.put(False)
 
     def collect():
-        for _ in range(n):
+        for k in range(n):
+            if progress and k % progress == 0 and k > 0:
+                print(f"served {k}/{n} after {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                      flush=True)
             if not events.get():
                 continue
             ticket, res = server.wait_any()
