@@ -77,6 +77,10 @@ typedef struct pmdd_dd_view {
     const int32_t *arc_val;   /* [num_arcs] */
     const int32_t *arc_w;     /* [num_arcs * d] */
     const int32_t *min_weight; /* [layer_off[num_layers]] or NULL */
+    /* Set-packing state dominance (no GPU counterpart in the reference; its
+     * CPU rule, dominance.cpp:88-173): per node, the layer-local index of the
+     * node whose labels may drop its labels, or -1; NULL when not built. */
+    const int32_t *sd_partner;
 } pmdd_dd_view;
 
 /* Options of one enumeration, mirroring the reference CLI

@@ -52,9 +52,13 @@ typedef struct pmdd_dd_structure {
  * diagram that main() builds for it.  problem_type: 1 knapsack, 2 set packing,
  * 3 TSP.  expect_d > 0 rejects files whose objective count differs (the
  * reference binary is compiled for one NOBJS, src/pmdd/makefile:54-58);
- * expect_d == 0 takes d from the file.  Returns PMDD_OK or PMDD_E_*. */
-int pmdd_dd_build_from_file(const char *path, int problem_type, int expect_d, pmdd_dd **out,
-                            char *err, size_t errlen);
+ * expect_d == 0 takes d from the file.  flags: PMDD_BUILD_STATE_DOMINANCE
+ * also records what state dominance needs that only the construction knows
+ * (for set packing, each node's dominance partner, view.sd_partner).
+ * Returns PMDD_OK or PMDD_E_*. */
+#define PMDD_BUILD_STATE_DOMINANCE 1
+int pmdd_dd_build_from_file(const char *path, int problem_type, int expect_d, int flags,
+                            pmdd_dd **out, char *err, size_t errlen);
 
 /* Borrowed view of the flattened diagram; valid until pmdd_dd_free. */
 const pmdd_dd_view *pmdd_dd_get_view(const pmdd_dd *dd);

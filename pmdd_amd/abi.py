@@ -47,6 +47,7 @@ class DDView(C.Structure):
         ("arc_val", _i32p),
         ("arc_w", _i32p),
         ("min_weight", _i32p),
+        ("sd_partner", _i32p),
     ]
 
 
@@ -112,6 +113,7 @@ class ServerConfig(C.Structure):
 
 
 PMDD_PENDING = 1
+PMDD_BUILD_STATE_DOMINANCE = 1
 
 
 class DDStructure(C.Structure):
@@ -174,7 +176,7 @@ def host_lib() -> C.CDLL:
         _require(HOST_LIB)
         lib = C.CDLL(HOST_LIB)
         lib.pmdd_dd_build_from_file.argtypes = [
-            C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]
+            C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]
         lib.pmdd_dd_build_from_file.restype = C.c_int
         lib.pmdd_dd_get_view.argtypes = [C.c_void_p]
         lib.pmdd_dd_get_view.restype = C.POINTER(DDView)

@@ -84,7 +84,9 @@ int main(int argc, char *argv[]) {
     pmdd_dd *dd = nullptr;
     char err[1024] = {0};
     int rc = pmdd_dd_build_from_file(opts.input_path.c_str(), opts.problem_type,
-                                     objectives_from_program_name(argv[0]), &dd, err, sizeof(err));
+                                     objectives_from_program_name(argv[0]),
+                                     opts.state_dominance > 0 ? PMDD_BUILD_STATE_DOMINANCE : 0,
+                                     &dd, err, sizeof(err));
     if (rc != PMDD_OK) {
         std::cout << err << std::endl;
         return 1;

@@ -194,6 +194,7 @@ struct pmdd_resident {
     std::vector<std::vector<int32_t>> min_weight;    // per layer, knapsack only
     std::vector<std::vector<int32_t>> single_parent; // per layer: parent index or -1
     std::vector<std::vector<int32_t>> single_arc;    // per layer: arc value of that parent
+    std::vector<std::vector<int32_t>> sd_partner;    // per layer, set packing (or empty)
     DevBuf arena;
 };
 
@@ -409,6 +410,14 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
                     r->single_arc[l][i] = v->arc_val[a0];
                 }
             }
+        }
+    }
+
+    if (v->sd_partner != nullptr) {
+        r->sd_partner.resize(L);
+        for (int l = 0; l < L; ++l) {
+            const int64_t g0 = v->layer_off[l];
+            r->sd_partner[l].assign(v->sd_partner + g0, v->sd_partner + g0 + r->layer_size[l]);
         }
     }
 
@@ -1212,6 +1221,16 @@ struct Run {
         }
     }
 
+    // State dominance runs on top-down layers when asked for and its data came
+    // with the diagram: knapsack min_weight, or set-packing partners.
+    bool state_dominance_applies() const {
+        if (opts.state_dominance <= 0 || dd->kind != PMDD_DD_BDD) {
+            return false;
+        }
+        return (opts.problem_type == 1 && !dd->min_weight.empty()) ||
+               (opts.problem_type == 2 && !dd->sd_partner.empty());
+    }
+
     // Knapsack state dominance on a top-down layer (filter_dominance_knapsack_cuda,
     // topdown.cu:363-550; rule of filter_dominance_knapsack_cpu, dominance.cpp:24-86).
     void state_dominance(int l, DevFrontier &f) {
@@ -1228,32 +1247,51 @@ struct Run {
         if (active.size() <= 1) {
             return;
         }
-        const std::vector<int32_t> &mw = dd->min_weight[l];
-        std::stable_sort(active.begin(), active.end(),
-                         [&](int64_t a, int64_t b) { return mw[a] > mw[b]; });
-        const std::vector<int32_t> &sp = dd->single_parent[l];
-        const std::vector<int32_t> &sa = dd->single_arc[l];
         std::vector<int32_t> targets;
-        for (size_t i = 0; i + 1 < active.size(); ++i) {
-            const int64_t n1 = active[i];
-            int32_t cmp[2] = {-1, -1};
-            int k = 0;
-            for (size_t j = i + 1; j < std::min(active.size(), i + 3); ++j) {
-                const int64_t n2 = active[j];
-                if (sp[n1] >= 0 && sp[n2] >= 0 && sp[n1] == sp[n2] && sa[n1] != sa[n2]) {
+        if (opts.problem_type == 2) {
+            // set packing: each node against its partner (pre-filter labels)
+            const std::vector<int32_t> &pt = dd->sd_partner[l];
+            for (int64_t n1 : active) {
+                const int32_t n2 = pt[n1];
+                if (n2 < 0 || f.size(n2) == 0) {
                     continue;
                 }
-                cmp[k++] = static_cast<int32_t>(n2);
+                for (int64_t b = 0; b < f.size(n1); b += kBlock) {
+                    targets.push_back(static_cast<int32_t>(n1));
+                    targets.push_back(n2);
+                    targets.push_back(-1);
+                    targets.push_back(static_cast<int32_t>(b));
+                }
             }
-            if (k == 0) {
-                continue;
-            }
-            // one workgroup per block of 256 labels of the target
-            for (int64_t b = 0; b < f.size(n1); b += kBlock) {
-                targets.push_back(static_cast<int32_t>(n1));
-                targets.push_back(cmp[0]);
-                targets.push_back(cmp[1]);
-                targets.push_back(static_cast<int32_t>(b));
+        } else {
+            // knapsack: nodes in descending min_weight, each against the next
+            // two (skipping same-parent, other-arc pairs)
+            const std::vector<int32_t> &mw = dd->min_weight[l];
+            std::stable_sort(active.begin(), active.end(),
+                             [&](int64_t a, int64_t b) { return mw[a] > mw[b]; });
+            const std::vector<int32_t> &sp = dd->single_parent[l];
+            const std::vector<int32_t> &sa = dd->single_arc[l];
+            for (size_t i = 0; i + 1 < active.size(); ++i) {
+                const int64_t n1 = active[i];
+                int32_t cmp[2] = {-1, -1};
+                int k = 0;
+                for (size_t j = i + 1; j < std::min(active.size(), i + 3); ++j) {
+                    const int64_t n2 = active[j];
+                    if (sp[n1] >= 0 && sp[n2] >= 0 && sp[n1] == sp[n2] && sa[n1] != sa[n2]) {
+                        continue;
+                    }
+                    cmp[k++] = static_cast<int32_t>(n2);
+                }
+                if (k == 0) {
+                    continue;
+                }
+                // one workgroup per block of 256 labels of the target
+                for (int64_t b = 0; b < f.size(n1); b += kBlock) {
+                    targets.push_back(static_cast<int32_t>(n1));
+                    targets.push_back(cmp[0]);
+                    targets.push_back(cmp[1]);
+                    targets.push_back(static_cast<int32_t>(b));
+                }
             }
         }
         if (targets.empty()) {
@@ -1359,8 +1397,7 @@ struct Run {
         const int L = dd->L;
         DevFrontier cur, nxt;
         seed_frontier(ctx, cur, d, dd->layer_size[0], dd->root);
-        const bool sd = opts.state_dominance > 0 && opts.problem_type == 1 &&
-                        dd->kind == PMDD_DD_BDD && !dd->min_weight.empty();
+        const bool sd = state_dominance_applies();
         for (int l = 1; l < L; ++l) {
             int64_t cand = 0;
             std::vector<int64_t> gamma;
@@ -1425,8 +1462,7 @@ struct Run {
         // knapsack state dominance on the top-down sweep (the reference packs
         // its metadata for method 3 but does not apply it, enum.cu:618): the
         // rule is sound, so the frontier is unchanged and the join smaller
-        const bool sd = opts.state_dominance > 0 && opts.problem_type == 1 &&
-                        dd->kind == PMDD_DD_BDD && !dd->min_weight.empty();
+        const bool sd = state_dominance_applies();
         int lt = 0, lb = L - 1;
         int64_t td_score = 0, bu_score = 0;
         while (lt != lb) {

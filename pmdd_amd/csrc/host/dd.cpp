@@ -2,6 +2,7 @@
 #include "dd.hpp"
 
 #include <algorithm>
+#include <unordered_map>
 #include <climits>
 #include <cstring>
 
@@ -23,6 +24,7 @@ pmdd_dd_view FlatDD::view() const {
     v.arc_val = arc_val.data();
     v.arc_w = arc_w.data();
     v.min_weight = min_weight.empty() ? nullptr : min_weight.data();
+    v.sd_partner = sd_partner.empty() ? nullptr : sd_partner.data();
     return v;
 }
 
@@ -161,8 +163,102 @@ std::vector<std::vector<int32_t>> knapsack_min_weights(const BinaryDD &bdd,
 // Set packing as an independent-set BDD
 // ---------------------------------------------------------------------------
 
+// Set-packing state dominance (filter_dominance_setpacking_cpu,
+// dominance.cpp:88-173): the nodes of a layer are bucketed by the size of
+// their state (eligible vertices); a node in a bucket of more than one node
+// may have its labels dropped by the labels of the first node of the next
+// non-empty bucket that has the same smallest vertex and a superset state,
+// skipping pairs of single-parent nodes fed by the same parent through
+// different arcs.  Such a partner has every completion of the node (its
+// eligible set is larger), so dropping the node's labels that a partner
+// label weakly dominates keeps the frontier.  The scan over the candidates
+// of a node stops after kPartnerScan of them (then no partner: less pruning,
+// same frontier).
+static constexpr int kPartnerScan = 64;
+
+static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &states, int W,
+                                                int n_vars,
+                                                const std::vector<std::array<int32_t, 2>> &parents) {
+    const int64_t N = static_cast<int64_t>(states.size()) / W;
+    std::vector<int32_t> partner(N, -1);
+    if (N < 2) {
+        return partner;
+    }
+    // single in-arc: its parent and arc type
+    std::vector<int32_t> indeg(N, 0), sp(N, -1), sa(N, -1);
+    for (size_t p = 0; p < parents.size(); ++p) {
+        for (int t = 0; t < 2; ++t) {
+            const int32_t c = parents[p][t];
+            if (c >= 0) {
+                indeg[c]++;
+                sp[c] = static_cast<int32_t>(p);
+                sa[c] = t;
+            }
+        }
+    }
+    std::vector<int32_t> count(N), first(N);
+    std::vector<std::vector<int32_t>> bucket(n_vars + 1);
+    for (int64_t i = 0; i < N; ++i) {
+        const uint64_t *s = states.data() + i * W;
+        int c = 0, f = -1;
+        for (int w = 0; w < W; ++w) {
+            c += __builtin_popcountll(s[w]);
+            if (f < 0 && s[w] != 0) {
+                f = w * 64 + __builtin_ctzll(s[w]);
+            }
+        }
+        count[i] = c;
+        first[i] = f;
+        bucket[c].push_back(static_cast<int32_t>(i));
+    }
+    for (int b1 = 0; b1 + 1 <= n_vars; ++b1) {
+        if (bucket[b1].size() <= 1) {
+            continue;
+        }
+        int b2 = b1 + 1;
+        while (b2 <= n_vars && bucket[b2].empty()) {
+            ++b2;
+        }
+        if (b2 > n_vars) {
+            continue;
+        }
+        // the next bucket's nodes by smallest vertex, in bucket order
+        std::unordered_map<int32_t, std::vector<int32_t>> by_first;
+        for (int32_t j : bucket[b2]) {
+            by_first[first[j]].push_back(j);
+        }
+        for (int32_t i : bucket[b1]) {
+            auto it = by_first.find(first[i]);
+            if (it == by_first.end()) {
+                continue;
+            }
+            const uint64_t *si = states.data() + static_cast<int64_t>(i) * W;
+            int scanned = 0;
+            for (int32_t j : it->second) {
+                if (++scanned > kPartnerScan) {
+                    break;
+                }
+                if (indeg[i] == 1 && indeg[j] == 1 && sp[i] == sp[j] && sa[i] != sa[j]) {
+                    continue;
+                }
+                const uint64_t *sj = states.data() + static_cast<int64_t>(j) * W;
+                bool subset = true;
+                for (int w = 0; w < W && subset; ++w) {
+                    subset = (si[w] & ~sj[w]) == 0;
+                }
+                if (subset) {
+                    partner[i] = j;
+                    break;
+                }
+            }
+        }
+    }
+    return partner;
+}
+
 BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
-                                std::vector<int32_t> *var_layer) {
+                                std::vector<int32_t> *var_layer,
+                                std::vector<std::vector<int32_t>> *partners) {
     const int n = inst.n_vars;
     const int W = (n + 63) / 64;
     // Conflict graph: variables sharing a constraint are adjacent.  compl[v]
@@ -202,6 +298,10 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
     std::vector<uint64_t> cur(W, 0);
     for (int v = 0; v < n; ++v) {
         cur[v / 64] |= (1ULL << (v % 64));
+    }
+    if (partners != nullptr) {
+        partners->assign(n + 1, std::vector<int32_t>());
+        partners->at(0).assign(1, -1);
     }
     std::vector<int32_t> active(n);
     for (int v = 0; v < n; ++v) {
@@ -290,6 +390,9 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
         }
         const int64_t nn = next_map.size();
         cur.assign(next_map.key(0), next_map.key(0) + nn * W);
+        if (partners != nullptr) {
+            (*partners)[l] = setpacking_partners(cur, W, n, bdd.child[l - 1]);
+        }
     }
     return bdd;
 }
@@ -517,8 +620,8 @@ static bool check_d(int file_d, int expect_d, std::string *error) {
     return true;
 }
 
-bool build_from_file(const std::string &path, int problem_type, int expect_d, FlatDD *out,
-                     std::string *error) {
+bool build_from_file(const std::string &path, int problem_type, int expect_d, bool state_dominance,
+                     FlatDD *out, std::string *error) {
     if (problem_type == 1) {
         KnapsackInstance inst;
         if (!read_knapsack(path, &inst, error) || !check_d(inst.n_objs, expect_d, error)) {
@@ -544,8 +647,17 @@ bool build_from_file(const std::string &path, int problem_type, int expect_d, Fl
             return false;
         }
         const int d = inst.n_objs;
-        BinaryDD bdd = build_setpacking_exact(inst, d, nullptr);
+        std::vector<std::vector<int32_t>> partners;
+        BinaryDD bdd =
+            build_setpacking_exact(inst, d, nullptr, state_dominance ? &partners : nullptr);
         flatten_bdd(bdd, nullptr, out);
+        out->sd_partner.clear();
+        if (state_dominance) {
+            out->sd_partner.reserve(static_cast<size_t>(out->layer_off[out->num_layers]));
+            for (const std::vector<int32_t> &layer : partners) {
+                out->sd_partner.insert(out->sd_partner.end(), layer.begin(), layer.end());
+            }
+        }
         out->structure.original_width = bdd.width();
         out->structure.original_num_nodes = bdd.num_nodes();
         out->structure.reduced_width = bdd.width();

@@ -26,6 +26,7 @@ struct FlatDD {
     std::vector<int32_t> arc_val;    // [A]
     std::vector<int32_t> arc_w;      // [A*d]
     std::vector<int32_t> min_weight; // [total_nodes] or empty
+    std::vector<int32_t> sd_partner; // [total_nodes] or empty: set-packing dominance partner
     pmdd_dd_structure structure{};
 
     int64_t layer_size(int l) const { return layer_off[l + 1] - layer_off[l]; }
@@ -69,8 +70,11 @@ std::vector<std::vector<int32_t>> knapsack_min_weights(const BinaryDD &bdd,
 // choose_next_vertex_min_size_next_layer, indepset_bdd.hpp:89-114) over the
 // conflict graph of a set packing instance (create_indepset_instance,
 // setpacking_instance.hpp:92-126).  var_layer receives the chosen order.
+// With `partners` non-null, also the set-packing state-dominance partner of
+// every node (layer-local index, -1 for none; see setpacking_partners).
 BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
-                                std::vector<int32_t> *var_layer);
+                                std::vector<int32_t> *var_layer,
+                                std::vector<std::vector<int32_t>> *partners = nullptr);
 
 // Flatten a binary diagram (1-arcs before 0-arcs per head, pack_bdd_layers
 // order enum.cu:444-445).
@@ -83,7 +87,10 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
 void build_tsp_exact(const TspInstance &inst, int d, FlatDD *out);
 
 // main()'s construction sequence for a problem type (main.cpp:113-199).
-bool build_from_file(const std::string &path, int problem_type, int expect_d, FlatDD *out,
+// state_dominance: also compute what the state-dominance rules need that only
+// the construction knows (the set-packing partners).
+bool build_from_file(const std::string &path, int problem_type, int expect_d, bool state_dominance,
+                     FlatDD *out,
                      std::string *error);
 
 } // namespace pmdd

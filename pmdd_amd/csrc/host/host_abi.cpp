@@ -19,7 +19,7 @@ static void write_err(char *err, size_t errlen, const std::string &msg) {
 }
 
 extern "C" int pmdd_dd_build_from_file(const char *path, int problem_type, int expect_d,
-                                       pmdd_dd **out, char *err, size_t errlen) {
+                                       int flags, pmdd_dd **out, char *err, size_t errlen) {
     if (out == nullptr || path == nullptr) {
         write_err(err, errlen, "pmdd_dd_build_from_file: NULL argument");
         return PMDD_E_INVALID;
@@ -28,7 +28,8 @@ extern "C" int pmdd_dd_build_from_file(const char *path, int problem_type, int e
     try {
         pmdd_dd *dd = new pmdd_dd();
         std::string error;
-        if (!pmdd::build_from_file(path, problem_type, expect_d, &dd->flat, &error)) {
+        if (!pmdd::build_from_file(path, problem_type, expect_d,
+                                   (flags & PMDD_BUILD_STATE_DOMINANCE) != 0, &dd->flat, &error)) {
             delete dd;
             write_err(err, errlen, error);
             return PMDD_E_INVALID;

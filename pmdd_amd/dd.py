@@ -32,12 +32,16 @@ class DecisionDiagram:
         self.kind = int(v.kind)
 
     @classmethod
-    def from_file(cls, path: str, problem_type: int, expect_d: int = 0) -> "DecisionDiagram":
+    def from_file(cls, path: str, problem_type: int, expect_d: int = 0,
+                  state_dominance: bool = False) -> "DecisionDiagram":
+        """state_dominance also records what the set-packing state-dominance
+        rule needs (node partners), which only the construction knows."""
         lib = abi.host_lib()
         h = C.c_void_p()
         err = C.create_string_buffer(512)
+        flags = abi.PMDD_BUILD_STATE_DOMINANCE if state_dominance else 0
         rc = lib.pmdd_dd_build_from_file(str(path).encode(), int(problem_type), int(expect_d),
-                                         C.byref(h), err, len(err))
+                                         flags, C.byref(h), err, len(err))
         if rc != abi.PMDD_OK:
             raise abi.PmddError(rc, err.value.decode())
         return cls(h, problem_type)
@@ -77,6 +81,8 @@ class DecisionDiagram:
         }
         if bool(v.min_weight):
             out["min_weight"] = np.ctypeslib.as_array(v.min_weight, shape=(total,)).copy()
+        if bool(v.sd_partner):
+            out["sd_partner"] = np.ctypeslib.as_array(v.sd_partner, shape=(total,)).copy()
         return out
 
     def close(self) -> None:

@@ -20,8 +20,9 @@ FIXTURES = sorted(os.listdir(os.path.join(os.path.dirname(__file__), "golden", "
 SMALL_FIXTURES = [f for f in FIXTURES if "ncities15" not in f]
 
 
-def _dd(name):
-    return DecisionDiagram.from_file(instance_path(name), problem_type_of(name))
+def _dd(name, state_dominance=False):
+    return DecisionDiagram.from_file(instance_path(name), problem_type_of(name),
+                                     state_dominance=state_dominance)
 
 
 def _same(a, b):
@@ -36,7 +37,9 @@ def test_fixture_matches_oracle(engine, name, method):
     ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
     sds = [0, 1] if problem_type_of(name) != 3 else [0]
     for sd in sds:
-        got = engine.enumerate(dd, method=method, state_dominance=sd).points
+        # state dominance data (set-packing partners) is built on request
+        ddk = _dd(name, True) if sd else dd
+        got = engine.enumerate(ddk, method=method, state_dominance=sd).points
         assert _same(got, ref), f"{name} m{method} sd{sd}: {len(got)} vs {len(ref)}"
 
 
@@ -93,10 +96,11 @@ def test_generated_small_vs_bruteforce(engine, tmp_path, seed):
         path = gen.write(inst, str(tmp_path / f"g{seed}_{k}.dat"))
         print(f"case seed={seed} kind={inst['kind']} n={inst['n']} d={inst['d']}", flush=True)
         bf = oracle.bruteforce(path, inst["kind"])
-        dd = DecisionDiagram.from_file(path, inst["kind"])
+        dds = [DecisionDiagram.from_file(path, inst["kind"]),
+               DecisionDiagram.from_file(path, inst["kind"], state_dominance=True)]
         for method in (1, 2, 3):
             for sd in (0, 1):
-                got = engine.enumerate(dd, method=method, state_dominance=sd).points
+                got = engine.enumerate(dds[sd], method=method, state_dominance=sd).points
                 assert _same(got, bf), (seed, k, method, sd)
 
 
@@ -139,6 +143,31 @@ def test_bottomup_stats_match_oracle(engine, tmp_path, kind):
     np.testing.assert_allclose(s["std_candidates_per_layer"], std_c, rtol=1e-12, atol=1e-9)
     np.testing.assert_allclose(s["std_frontier_survivors_per_layer"], std_s, rtol=1e-12,
                                atol=1e-9)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_setpacking_state_dominance_filters_and_keeps_frontier(engine, tmp_path, seed):
+    """Set-packing state dominance (partners built with the diagram) drops
+    labels on the top-down layers of methods 1 and 3 and leaves the frontier
+    equal to the oracle's without it.  The number dropped depends on the
+    partner choice, which differs from the reference's pairing, so only its
+    sign is checked."""
+    inst = gen.setpacking(500 + seed, 60 + 10 * seed, 3 + seed % 2, m=10)
+    path = gen.write(inst, str(tmp_path / "sp.dat"))
+    print(f"case sp-sd seed={seed} n={inst['n']} d={inst['d']}", flush=True)
+    dd0 = DecisionDiagram.from_file(path, 2)
+    dd1 = DecisionDiagram.from_file(path, 2, state_dominance=True)
+    ref = oracle.enumerate_dd(dd0, 1, 0, threads=8)
+    filtered = 0
+    for method in (1, 3):
+        r = engine.enumerate(dd1, method=method, state_dominance=1)
+        assert _same(r.points, ref), (seed, method, len(r.points), len(ref))
+        filtered += r.stats["dominance_filtered_total"]
+        # the flag alone, without partners, filters nothing
+        r0 = engine.enumerate(dd0, method=method, state_dominance=1)
+        assert _same(r0.points, ref)
+        assert r0.stats["dominance_filtered_total"] == 0
+    assert filtered > 0
 
 
 def test_coupled_layer_matches_oracle_gpu_rule(engine):

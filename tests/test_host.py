@@ -193,6 +193,51 @@ def test_setpacking_bdd_paths_are_exactly_the_independent_sets(tmp_path):
     assert _path_sums(dd) == sorted(want)
 
 
+def _completions(dd):
+    """For every node (global index), the set of arc-label sequences from it
+    to the terminal, for a small flattened diagram."""
+    a = dd.arrays()
+    lo, io = a["layer_off"], a["in_off"]
+    L = dd.num_layers
+    comp = {int(lo[L - 1]) + v: {()} for v in range(lo[L] - lo[L - 1])}
+    for l in range(L - 1, 0, -1):
+        parents = {}
+        for v in range(lo[l + 1] - lo[l]):
+            g = int(lo[l]) + v
+            for e in range(io[g], io[g + 1]):
+                p = int(lo[l - 1]) + int(a["arc_src"][e])
+                parents.setdefault(p, set()).update(
+                    (int(a["arc_val"][e]),) + c for c in comp[g])
+        for p in range(int(lo[l - 1]), int(lo[l])):
+            comp[p] = parents.get(p, set())
+    return comp
+
+
+def test_setpacking_partners_have_every_completion(tmp_path):
+    """A set-packing state-dominance partner can complete every path its node
+    can (its eligible set contains the node's), which is what makes dropping
+    the node's dominated labels sound."""
+    pairs = 0
+    for seed in range(41, 47):
+        inst = gen.setpacking(seed, 14, 2, m=4)
+        path = gen.write(inst, str(tmp_path / f"s{seed}.dat"))
+        dd = DecisionDiagram.from_file(path, 2, state_dominance=True)
+        a = dd.arrays()
+        lo = a["layer_off"]
+        partner = a["sd_partner"]
+        comp = _completions(dd)
+        for l in range(dd.num_layers):
+            for v in range(lo[l + 1] - lo[l]):
+                q = int(partner[lo[l] + v])
+                if q >= 0:
+                    assert q != v
+                    assert comp[int(lo[l]) + v] <= comp[int(lo[l]) + q], (seed, l, v, q)
+                    pairs += 1
+        # without the flag, no partners are recorded
+        assert "sd_partner" not in DecisionDiagram.from_file(path, 2).arrays()
+    assert pairs > 0
+
+
 def test_setpacking_first_vertex_and_terminal(tmp_path):
     inst = gen.setpacking(3, 16, 3, m=4)
     path = gen.write(inst, str(tmp_path / "s.dat"))
