@@ -56,6 +56,9 @@ void hip_check(hipError_t e, const char *where) {
 }
 
 // Device buffer; grows on demand, contents not kept unless asked.
+// Device allocations made so far (device memory use changes only with them).
+static std::atomic<uint64_t> g_device_allocs{0};
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -84,19 +87,38 @@ struct DevBuf {
         p = nullptr;
         bytes = 0;
     }
+    // Grows by half again (with at most 64 MiB of slack), so a buffer sized
+    // by a widening layer is not reallocated on every layer.
     void ensure(size_t need, const char *what) {
         if (need <= bytes && p != nullptr) {
             return;
         }
-        release();
         size_t nb = std::max<size_t>(need, 256);
+        if (bytes > 0) {
+            nb = std::max(nb, std::min(bytes + bytes / 2, need + (size_t(64) << 20)));
+        }
+        release();
         hip_check(hipMalloc(&p, nb), what);
         bytes = nb;
+        g_device_allocs.fetch_add(1, std::memory_order_relaxed);
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// Frontier of one layer on the device: offsets per node + SoA points.
+// ---------------------------------------------------------------------------
+struct DevFrontier {
+    int64_t n_nodes = 0;
+    std::vector<int64_t> h_off; // [n_nodes + 1]
+    DevBuf off;
+    DevBuf pts;
+    int64_t cap = 0; // SoA stride
+    int64_t total() const { return h_off.empty() ? 0 : h_off.back(); }
+    int64_t size(int64_t v) const { return h_off[v + 1] - h_off[v]; }
+};
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -169,6 +191,29 @@ struct pmdd_ctx {
     DevBuf join_idx, join_x, join_y, join_ctr, front_count, merge_a, merge_b;
     std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
     Staging staging;
+    // frontier buffers handed from one enumeration to the next (FrontierLease)
+    std::vector<DevFrontier> frontier_pool;
+};
+
+// A frontier borrowed from the context's pool for one enumeration and handed
+// back at its end, so a context's frontier buffers grow once, not per call.
+// The SoA stride is recomputed for this enumeration's objective count.
+struct FrontierLease {
+    pmdd_ctx *ctx;
+    DevFrontier f;
+    FrontierLease(pmdd_ctx *c, int d) : ctx(c) {
+        if (!c->frontier_pool.empty()) {
+            f = std::move(c->frontier_pool.back());
+            c->frontier_pool.pop_back();
+        }
+        f.n_nodes = 0;
+        f.h_off.clear();
+        f.cap = f.pts.p != nullptr ? static_cast<int64_t>(f.pts.bytes / (4 * static_cast<size_t>(d)))
+                                   : 0;
+    }
+    ~FrontierLease() { ctx->frontier_pool.push_back(std::move(f)); }
+    FrontierLease(const FrontierLease &) = delete;
+    FrontierLease &operator=(const FrontierLease &) = delete;
 };
 
 struct LayerCSR {
@@ -467,18 +512,6 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
     return r.release();
 }
 
-// ---------------------------------------------------------------------------
-// Frontier of one layer on the device: offsets per node + SoA points.
-// ---------------------------------------------------------------------------
-struct DevFrontier {
-    int64_t n_nodes = 0;
-    std::vector<int64_t> h_off; // [n_nodes + 1]
-    DevBuf off;
-    DevBuf pts;
-    int64_t cap = 0; // SoA stride
-    int64_t total() const { return h_off.empty() ? 0 : h_off.back(); }
-    int64_t size(int64_t v) const { return h_off[v + 1] - h_off[v]; }
-};
 
 // Grow a SoA point buffer to hold `need` points, keeping the first `keep`.
 void grow_points(DevFrontier &f, int d, int64_t need, int64_t keep, hipStream_t s) {
@@ -745,11 +778,10 @@ void seed_frontier(pmdd_ctx *ctx, DevFrontier &f, int d, int64_t n_nodes, int64_
     for (int64_t v = node + 1; v <= n_nodes; ++v) {
         f.h_off[v] = 1;
     }
-    f.cap = 0;
-    f.pts.release();
-    grow_points(f, d, 1, 0, s);
-    hip_check(hipMemsetAsync(f.pts.p, 0, static_cast<size_t>(f.cap) * d * 4, s),
-              "hipMemset(seed)");
+    grow_points(f, d, 1, 0, s); // keeps a leased buffer
+    for (int o = 0; o < d; ++o) {
+        hip_check(hipMemsetAsync(f.pts.as<int32_t>() + o * f.cap, 0, 4, s), "hipMemset(seed)");
+    }
     upload_offsets(ctx, f);
 }
 
@@ -766,12 +798,14 @@ struct Run {
     hipStream_t s;
     int64_t mem_base = 0;
     unsigned long long *d_tests = nullptr;
-    DevFrontier sd_spare; // state dominance output, swapped with the layer's frontier
+    FrontierLease sd_lease;
+    DevFrontier &sd_spare; // state dominance output, swapped with the layer's frontier
     // PMDD_DEBUG_JOIN=1 prints per-batch coupling counts to stderr
     const bool debug_join = std::getenv("PMDD_DEBUG_JOIN") != nullptr;
 
     Run(pmdd_ctx *c, pmdd_resident *r, const pmdd_opts &o, pmdd_stats *stats)
-        : ctx(c), dd(r), opts(o), st(stats), K(launch_for(r->d)), d(r->d), s(c->stream) {
+        : ctx(c), dd(r), opts(o), st(stats), K(launch_for(r->d)), d(r->d), s(c->stream),
+          sd_lease(c, r->d), sd_spare(sd_lease.f) {
         if (opts.max_cand <= 0) {
             opts.max_cand = PMDD_DEFAULT_MAX_CAND;
         }
@@ -789,7 +823,15 @@ struct Run {
         }
     }
 
+    // Peak device memory in use, sampled after allocations (it changes only
+    // with them).
+    uint64_t allocs_seen = ~uint64_t(0);
     void sample_memory() {
+        const uint64_t a = g_device_allocs.load(std::memory_order_relaxed);
+        if (a == allocs_seen) {
+            return;
+        }
+        allocs_seen = a;
         size_t fr = 0, tot = 0;
         hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
         const int64_t used = static_cast<int64_t>(tot) - static_cast<int64_t>(fr);
@@ -1395,7 +1437,8 @@ struct Run {
     // Method 1 (enumerate_bdd_topdown / enumerate_mdd_topdown).
     void topdown(pmdd_frontier *out) {
         const int L = dd->L;
-        DevFrontier cur, nxt;
+        FrontierLease lc(ctx, d), ln(ctx, d);
+        DevFrontier &cur = lc.f, &nxt = ln.f;
         seed_frontier(ctx, cur, d, dd->layer_size[0], dd->root);
         const bool sd = state_dominance_applies();
         for (int l = 1; l < L; ++l) {
@@ -1428,7 +1471,8 @@ struct Run {
     // not apply (the CPU path ignores it too, enum.cpp:127).
     void bottomup(pmdd_frontier *out) {
         const int L = dd->L;
-        DevFrontier cur, nxt;
+        FrontierLease lc(ctx, d), ln(ctx, d);
+        DevFrontier &cur = lc.f, &nxt = ln.f;
         seed_frontier(ctx, cur, d, dd->layer_size[L - 1], dd->terminal);
         for (int l = L - 2; l >= 0; --l) {
             int64_t cand = 0;
@@ -1456,7 +1500,8 @@ struct Run {
     // and trunc(1.5 * sum |F(v)| * in-arcs(v)); stop where the sweeps meet.
     void coupled(pmdd_frontier *out) {
         const int L = dd->L;
-        DevFrontier td, bu, tmp;
+        FrontierLease lt_(ctx, d), lb_(ctx, d), lm_(ctx, d);
+        DevFrontier &td = lt_.f, &bu = lb_.f, &tmp = lm_.f;
         seed_frontier(ctx, td, d, dd->layer_size[0], dd->root);
         seed_frontier(ctx, bu, d, dd->layer_size[L - 1], dd->terminal);
         // knapsack state dominance on the top-down sweep (the reference packs
@@ -1528,7 +1573,8 @@ struct Run {
             return td.size(a) + bu.size(a) > td.size(b) + bu.size(b);
         });
 
-        DevFrontier R, B;
+        FrontierLease lr(ctx, d), lbt(ctx, d);
+        DevFrontier &R = lr.f, &B = lbt.f;
         int64_t r_count = 0;
         size_t i = 0;
         std::vector<ChunkDesc> chunks, rchunks;
@@ -1771,8 +1817,6 @@ struct Run {
             if (btotal == 0) {
                 continue;
             }
-            B.cap = 0;
-            B.pts.release();
             grow_points(B, d, btotal, 0, s);
             run_compact(cb, nb, outpos, ctx->surv_pts.as<int32_t>(), ytotal,
                         ctx->alive.as<uint8_t>(), B.pts.as<int32_t>(), B.cap);
@@ -2027,6 +2071,7 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
                           &c->run_start, &c->grp_run, &c->join_idx}) {
             b->release();
         }
+        c->frontier_pool.clear();
     }
     if (ctx->ev_a != nullptr) {
         (void)hipEventDestroy(ctx->ev_a);
