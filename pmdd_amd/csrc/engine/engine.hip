@@ -190,6 +190,7 @@ struct pmdd_ctx {
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
     DevBuf join_idx, join_x, join_y, join_ctr, front_count, merge_a, merge_b;
     std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
+    std::vector<hipEvent_t> ev_late; // start/end pairs read after a later wait
     Staging staging;
     // frontier buffers handed from one enumeration to the next (FrontierLease)
     std::vector<DevFrontier> frontier_pool;
@@ -862,6 +863,27 @@ struct Run {
         return t;
     }
 
+    // Timings read after a later synchronisation, so the host need not wait
+    // for the work they bracket: pair i of ctx->ev_late adds to late_acc[i].
+    std::vector<double *> late_acc;
+    std::pair<hipEvent_t, hipEvent_t> late_pair(double *acc) {
+        const size_t i = late_acc.size();
+        while (ctx->ev_late.size() < 2 * (i + 1)) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreate(&e), "hipEventCreate");
+            ctx->ev_late.push_back(e);
+        }
+        late_acc.push_back(acc);
+        return {ctx->ev_late[2 * i], ctx->ev_late[2 * i + 1]};
+    }
+    // Only once the stream has passed every recorded pair.
+    void flush_late() {
+        for (size_t i = 0; i < late_acc.size(); ++i) {
+            add_time(ctx->ev_late[2 * i], ctx->ev_late[2 * i + 1], late_acc[i]);
+        }
+        late_acc.clear();
+    }
+
     float elapsed_ms() {
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b), "hipEventElapsedTime");
@@ -1211,6 +1233,7 @@ struct Run {
             }
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
             d2h(ctx, nalive, cb.nalive, chunks.size(), "chunk survivor counts");
+            flush_late();
             if (cross_waves > 0) {
                 st->kernel_cross_s += pool_seconds(cross_waves);
                 st->cross_launches += cross_waves;
@@ -1235,13 +1258,13 @@ struct Run {
                 next.h_off[u + 1] = at;
             }
             grow_points(next, d, at, next_total, s);
-            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            // no wait: the next pass or layer is planned while this runs
+            const auto tc = late_pair(&st->kernel_expand_td_s);
+            hip_check(hipEventRecord(tc.first, s), "hipEventRecord");
             run_compact(cb, static_cast<int64_t>(chunks.size()), chunk_out,
                         ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>(),
                         next.pts.as<int32_t>(), next.cap);
-            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-            hip_check(hipStreamSynchronize(s), "compaction sync");
-            add_time(ctx->ev_a, ctx->ev_b, &st->kernel_expand_td_s);
+            hip_check(hipEventRecord(tc.second, s), "hipEventRecord");
             next_total = at;
             sample_memory();
         }
@@ -1381,7 +1404,6 @@ struct Run {
                         f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>(),
                         sd_spare.off.as<int64_t>(), sd_spare.pts.as<int32_t>(), sd_spare.cap);
         hip_check(hipGetLastError(), "k_compact_nodes");
-        hip_check(hipStreamSynchronize(s), "state dominance sync");
         std::swap(f.h_off, sd_spare.h_off);
         std::swap(f.off.p, sd_spare.off.p);
         std::swap(f.off.bytes, sd_spare.off.bytes);
@@ -1912,6 +1934,8 @@ struct Run {
     }
 
     void finish() {
+        hip_check(hipStreamSynchronize(s), "finish sync");
+        flush_late();
         if (d_tests != nullptr) {
             unsigned long long t = 0;
             hip_check(hipMemcpyAsync(&t, d_tests, 8, hipMemcpyDeviceToHost, s), "hipMemcpy(counter)");
@@ -2088,6 +2112,9 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
         }
     }
     for (hipEvent_t e : ctx->ev_pool) {
+        (void)hipEventDestroy(e);
+    }
+    for (hipEvent_t e : ctx->ev_late) {
         (void)hipEventDestroy(e);
     }
     if (ctx->stream != nullptr) {
