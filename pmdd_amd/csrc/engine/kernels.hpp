@@ -203,6 +203,34 @@ __device__ __forceinline__ int64_t key_at(const int32_t *pts, int64_t stride, in
     return k;
 }
 
+// A row of DP ints (16-byte aligned) read into registers.
+template <int D, int DP>
+__device__ __forceinline__ void load_row(const int32_t *row, int32_t (&q)[D]) {
+    const int4 a = *reinterpret_cast<const int4 *>(row);
+    q[0] = a.x;
+    if (D > 1) q[D > 1 ? 1 : 0] = a.y;
+    if (D > 2) q[D > 2 ? 2 : 0] = a.z;
+    if (D > 3) q[D > 3 ? 3 : 0] = a.w;
+    if (D > 4) {
+        const int4 b = *reinterpret_cast<const int4 *>(row + 4);
+        q[D > 4 ? 4 : 0] = b.x;
+        if (D > 5) q[D > 5 ? 5 : 0] = b.y;
+        if (D > 6) q[D > 6 ? 6 : 0] = b.z;
+        if (D > 7) q[D > 7 ? 7 : 0] = b.w;
+    }
+}
+
+// q >= m in every objective.
+template <int D>
+__device__ __forceinline__ bool ge_vals(const int32_t (&q)[D], const int32_t (&m)[D]) {
+    bool r = q[0] >= m[0];
+#pragma unroll
+    for (int o = 1; o < D; ++o) {
+        r &= q[o] >= m[o];
+    }
+    return r;
+}
+
 // row >= m in every objective; rows are 16-byte aligned, padded to DP ints.
 template <int D, int DP>
 __device__ __forceinline__ bool ge_all(const int32_t *row, const int32_t (&m)[D]) {
@@ -377,6 +405,42 @@ __device__ __forceinline__ int block_sum(int v, int *wave_tot) {
     return sum;
 }
 
+// The first NR of a thread's R rows against tn staged rows (DP ints each);
+// with OWN, staged row j (slot TS[j]) only kills rows in later slots.
+template <int D, int DP, int NR, int R, bool OWN>
+__device__ __forceinline__ void scan_rows(const int32_t *U, const int16_t *TS, int tn,
+                                          const int32_t (&m)[R][D], const int (&slot)[R],
+                                          bool (&dead)[R]) {
+#pragma unroll 2
+    for (int j = 0; j < tn; ++j) {
+        int32_t q[D];
+        load_row<D, DP>(&U[j * DP], q);
+        const int sj = OWN ? TS[j] : 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            bool t = ge_vals<D>(q, m[r]);
+            if (OWN) {
+                t &= sj < slot[r];
+            }
+            dead[r] |= t;
+        }
+    }
+}
+
+// scan_rows for a row count known only at run time (the same in every lane).
+template <int D, int DP, int R, bool OWN, int NR = R>
+__device__ __forceinline__ void scan_rows_upto(int rows, const int32_t *U, const int16_t *TS,
+                                               int tn, const int32_t (&m)[R][D],
+                                               const int (&slot)[R], bool (&dead)[R]) {
+    if constexpr (NR >= 1) {
+        if (rows == NR) {
+            scan_rows<D, DP, NR, R, OWN>(U, TS, tn, m, slot, dead);
+        } else {
+            scan_rows_upto<D, DP, R, OWN, NR - 1>(rows, U, TS, tn, m, slot, dead);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_cross_chunks: the chunks of a group are consecutive blocks of descending
 // keys, taken in waves of W chunks; every alive point of a chunk dies when it
@@ -453,27 +517,12 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
         for (int r = 0; r < R; ++r) {
             dead[r] = false;
         }
-        if (!own) {
-            for (int j = 0; j < tn; ++j) {
-                const int32_t *row = &U[j * DP];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if (r < rows) {
-                        dead[r] |= ge_all<D, DP>(row, m[r]);
-                    }
-                }
-            }
+        // rows is the same in every lane: one branch-free scan per row count
+        const int urows = __builtin_amdgcn_readfirstlane(rows);
+        if (own) {
+            scan_rows_upto<D, DP, R, true>(urows, U, TS, tn, m, slot, dead);
         } else {
-            for (int j = 0; j < tn; ++j) {
-                const int32_t *row = &U[j * DP];
-                const int sj = TS[j];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if (r < rows) {
-                        dead[r] |= (sj < slot[r]) && ge_all<D, DP>(row, m[r]);
-                    }
-                }
-            }
+            scan_rows_upto<D, DP, R, false>(urows, U, TS, tn, m, slot, dead);
         }
         live = 0;
 #pragma unroll
