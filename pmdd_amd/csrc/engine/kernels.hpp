@@ -249,6 +249,39 @@ __device__ __forceinline__ bool ge_all(const int32_t *row, const int32_t (&m)[D]
     return r;
 }
 
+// Rows [0, n) of a staged table (DP ints per row, every lane reading the
+// same row), of which a lane only meets rows below its own bound lim <= n,
+// against the lane's point m: returns dead, or'ed with whether some such row
+// is >= m.  Rows go eight at a time so that their LDS reads overlap, and the
+// wave stops once none of its lanes is alive; n is the same in every lane.
+// tests counts the rows a lane met while alive.
+template <int D, int DP>
+__device__ __forceinline__ bool scan_until_dead(const int32_t *S, int n, int lim,
+                                                const int32_t (&m)[D], bool dead,
+                                                unsigned long long &tests) {
+    constexpr int kB = 8;
+    int s = 0;
+    for (; s + kB <= n; s += kB) {
+        if (!__any(!dead && s < lim)) {
+            return dead;
+        }
+        if (!dead) {
+            tests += static_cast<unsigned long long>(max(0, min(kB, lim - s)));
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            int32_t q[D];
+            load_row<D, DP>(&S[(s + k) * DP], q);
+            dead |= (s + k < lim) & ge_vals<D>(q, m);
+        }
+    }
+    for (; s < n && s < lim && !dead; ++s) {
+        ++tests;
+        dead = ge_all<D, DP>(&S[s * DP], m);
+    }
+    return dead;
+}
+
 // ---------------------------------------------------------------------------
 // k_filter_chunks: per-chunk load, presort, sort-filter skyline.
 // ---------------------------------------------------------------------------
@@ -333,19 +366,11 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
         __syncthreads();
         bool dead = !valid;
         // against survivors of earlier steps
-        for (int s = 0; s < nS && !dead; ++s) {
-            dead = ge_all<D, DP>(&S[s * DP], m);
-            if (pair_tests != nullptr) {
-                ++tests;
-            }
-        }
-        // against earlier candidates of this step
-        for (int j = 0; j < tid && j < tcount && !dead; ++j) {
-            dead = ge_all<D, DP>(&S[(nS + j) * DP], m);
-            if (pair_tests != nullptr) {
-                ++tests;
-            }
-        }
+        dead = scan_until_dead<D, DP>(S, nS, nS, m, dead, tests);
+        // against earlier candidates of this step (slots below the lane's own)
+        const int wave_end = __builtin_amdgcn_readfirstlane(
+            min(tcount, static_cast<int>((tid | (kWave - 1)))));
+        dead = scan_until_dead<D, DP>(S + nS * DP, wave_end, min(tid, tcount), m, dead, tests);
         __syncthreads();
         int total = 0;
         const int rank = block_rank(!dead, wave_tot, &total);
@@ -812,9 +837,8 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
                 }
             }
             __syncthreads();
-            for (int j = 0; j < tn && !dead; ++j) {
-                dead = ge_all<D, DP>(&T[j * DP], m);
-            }
+            unsigned long long tests = 0;
+            dead = scan_until_dead<D, DP>(T, tn, tn, m, dead, tests);
         }
     }
     if (valid && dead) {
