@@ -1067,6 +1067,8 @@ __global__ __launch_bounds__(kBlock) void k_join_products(JoinSrc src, int64_t n
 // the next point of the wave's stream.  One hard point therefore never holds
 // 63 finished lanes, which is what a check in lockstep over fixed groups pays:
 // most points meet a dominator within a few rows, a few need thousands.
+// kSub = 64 (a whole tile between refills) measured faster than 16 or 32:
+// the refill's bookkeeping costs more than the tests a dead lane idles over.
 // Wave w's stream is the 64-point chunks w, w + G, w + 2G, ... (G waves), so
 // taking points needs no atomics; survivors are staged in LDS and written
 // out kStage at a time with one atomic each.
@@ -1083,7 +1085,7 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
     int32_t *__restrict__ Y, int64_t ystride, unsigned long long *__restrict__ n_out,
     uint8_t *__restrict__ flag, unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
-    constexpr int kSub = 16;                  // rows between two refills
+    constexpr int kSub = 64;                  // rows between two refills
     constexpr int kSubPerTile = kWave / kSub; // sub-tiles per staged tile
     constexpr int kStage = 128;               // survivors staged per flush
     __shared__ __align__(16) int32_t T[kWave * DP];
