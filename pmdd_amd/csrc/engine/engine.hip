@@ -238,6 +238,8 @@ struct pmdd_resident {
     std::vector<LayerCSR> td; // td[l]: arcs into layer l (l >= 1)
     std::vector<LayerCSR> bu; // bu[l]: arcs out of layer l (l <= L-2), "src" = child in l+1
     std::vector<std::vector<int32_t>> min_weight;    // per layer, knapsack only
+    std::vector<std::vector<int32_t>> mw_order;      // per layer: nodes by descending
+                                                     // min_weight (stable)
     std::vector<std::vector<int32_t>> single_parent; // per layer: parent index or -1
     std::vector<std::vector<int32_t>> single_arc;    // per layer: arc value of that parent
     std::vector<std::vector<int32_t>> sd_partner;    // per layer, set packing (or empty)
@@ -441,12 +443,21 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
     // knapsack metadata for state dominance (pack_bdd_layers enum.cu:496-540)
     if (v->min_weight != nullptr) {
         r->min_weight.resize(L);
+        r->mw_order.resize(L);
         r->single_parent.resize(L);
         r->single_arc.resize(L);
         for (int l = 0; l < L; ++l) {
             const int64_t n = r->layer_size[l];
             const int64_t g0 = v->layer_off[l];
             r->min_weight[l].assign(v->min_weight + g0, v->min_weight + g0 + n);
+            const std::vector<int32_t> &mw = r->min_weight[l];
+            std::vector<int32_t> &ord = r->mw_order[l];
+            ord.resize(n);
+            for (int64_t i = 0; i < n; ++i) {
+                ord[i] = static_cast<int32_t>(i);
+            }
+            std::stable_sort(ord.begin(), ord.end(),
+                             [&](int32_t a, int32_t b) { return mw[a] > mw[b]; });
             r->single_parent[l].assign(n, -1);
             r->single_arc[l].assign(n, -1);
             for (int64_t i = 0; i < n; ++i) {
@@ -863,6 +874,15 @@ struct Run {
         return t;
     }
 
+    // Host tables of a layer, kept between layers so that planning a layer
+    // neither allocates nor touches fresh pages.
+    struct HostTables {
+        std::vector<int64_t> run_off, gamma, chunk_out, big_base, run_start, active;
+        std::vector<ChunkDesc> chunks;
+        std::vector<int32_t> which_small, which_tiny, which_mid, which_big, which_cross, nalive;
+        std::vector<int32_t> big_node, grp_run, targets;
+    } xh;
+
     // Timings read after a later synchronisation, so the host need not wait
     // for the work they bracket: pair i of ctx->ev_late adds to late_acc[i].
     std::vector<double *> late_acc;
@@ -1081,8 +1101,10 @@ struct Run {
     void expand(const DevFrontier &prev, const LayerCSR &csr, int64_t n_nodes, DevFrontier &next,
                 int64_t *cand_out, std::vector<int64_t> *gamma_out) {
         const int CH = K.chunk_cap;
-        std::vector<int64_t> run_off(std::max<int64_t>(csr.n_arcs, 1));
-        std::vector<int64_t> gamma(n_nodes, 0);
+        std::vector<int64_t> &run_off = xh.run_off;
+        run_off.resize(std::max<int64_t>(csr.n_arcs, 1));
+        std::vector<int64_t> &gamma = xh.gamma;
+        gamma.assign(n_nodes, 0);
         int64_t C = 0;
         for (int64_t v = 0; v < n_nodes; ++v) {
             int64_t acc = 0;
@@ -1116,10 +1138,13 @@ struct Run {
 
         ChunkBufs &cb = ctx->cb_main;
         int64_t v = 0;
-        std::vector<ChunkDesc> chunks;
-        std::vector<int32_t> which_small, which_tiny, which_mid, which_big, which_cross, nalive;
-        std::vector<int64_t> chunk_out, big_base, run_start;
-        std::vector<int32_t> big_node, grp_run;
+        std::vector<ChunkDesc> &chunks = xh.chunks;
+        std::vector<int32_t> &which_small = xh.which_small, &which_tiny = xh.which_tiny,
+                             &which_mid = xh.which_mid, &which_big = xh.which_big,
+                             &which_cross = xh.which_cross, &nalive = xh.nalive;
+        std::vector<int64_t> &chunk_out = xh.chunk_out, &big_base = xh.big_base,
+                             &run_start = xh.run_start;
+        std::vector<int32_t> &big_node = xh.big_node, &grp_run = xh.grp_run;
         while (v < n_nodes) {
             const int64_t v0 = v;
             int64_t pass_c = 0;
@@ -1303,16 +1328,28 @@ struct Run {
         if (n <= 1) {
             return;
         }
-        std::vector<int64_t> active;
-        for (int64_t v = 0; v < n; ++v) {
-            if (f.size(v) > 0) {
-                active.push_back(v);
+        // nodes with labels: index order, or (knapsack) descending min_weight,
+        // the stable order of the whole layer filtered
+        std::vector<int64_t> &active = xh.active;
+        active.clear();
+        if (opts.problem_type == 2) {
+            for (int64_t v = 0; v < n; ++v) {
+                if (f.size(v) > 0) {
+                    active.push_back(v);
+                }
+            }
+        } else {
+            for (int32_t v : dd->mw_order[l]) {
+                if (f.size(v) > 0) {
+                    active.push_back(v);
+                }
             }
         }
         if (active.size() <= 1) {
             return;
         }
-        std::vector<int32_t> targets;
+        std::vector<int32_t> &targets = xh.targets;
+        targets.clear();
         if (opts.problem_type == 2) {
             // set packing: each node against its partner (pre-filter labels)
             const std::vector<int32_t> &pt = dd->sd_partner[l];
@@ -1331,9 +1368,6 @@ struct Run {
         } else {
             // knapsack: nodes in descending min_weight, each against the next
             // two (skipping same-parent, other-arc pairs)
-            const std::vector<int32_t> &mw = dd->min_weight[l];
-            std::stable_sort(active.begin(), active.end(),
-                             [&](int64_t a, int64_t b) { return mw[a] > mw[b]; });
             const std::vector<int32_t> &sp = dd->single_parent[l];
             const std::vector<int32_t> &sa = dd->single_arc[l];
             for (size_t i = 0; i + 1 < active.size(); ++i) {
@@ -1463,9 +1497,9 @@ struct Run {
         DevFrontier &cur = lc.f, &nxt = ln.f;
         seed_frontier(ctx, cur, d, dd->layer_size[0], dd->root);
         const bool sd = state_dominance_applies();
+        std::vector<int64_t> gamma;
         for (int l = 1; l < L; ++l) {
             int64_t cand = 0;
-            std::vector<int64_t> gamma;
             expand(cur, dd->td[l], dd->layer_size[l], nxt, &cand, &gamma);
             if (sd) {
                 state_dominance(l, nxt);
@@ -1496,9 +1530,9 @@ struct Run {
         FrontierLease lc(ctx, d), ln(ctx, d);
         DevFrontier &cur = lc.f, &nxt = ln.f;
         seed_frontier(ctx, cur, d, dd->layer_size[L - 1], dd->terminal);
+        std::vector<int64_t> gamma;
         for (int l = L - 2; l >= 0; --l) {
             int64_t cand = 0;
-            std::vector<int64_t> gamma;
             expand(cur, dd->bu[l], dd->layer_size[l], nxt, &cand, &gamma);
             count_layer(cand, nxt.total());
             if (st->std_candidates_per_layer != nullptr && l < st->std_capacity) {
