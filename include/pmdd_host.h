@@ -71,6 +71,13 @@ int64_t pmdd_dd_layer_size(const pmdd_dd *dd, int layer);
 
 void pmdd_dd_free(pmdd_dd *dd);
 
+/* Sort a frontier's points lexicographically ascending, in place: the order
+ * main() writes (ParetoFrontier::sort_lexicographic_ascending,
+ * src/pmdd/enum/pareto_frontier.hpp:418-433, called at src/pmdd/main.cpp:403).
+ * The engine returns frontiers unsorted, as the reference's GPU entry points
+ * do. */
+void pmdd_frontier_sort(pmdd_frontier *fr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -157,6 +157,7 @@ HOST_SYMBOLS = [
     "pmdd_dd_get_structure",
     "pmdd_dd_layer_size",
     "pmdd_dd_free",
+    "pmdd_frontier_sort",
 ]
 
 _host = None
@@ -186,6 +187,8 @@ def host_lib() -> C.CDLL:
         lib.pmdd_dd_layer_size.restype = C.c_int64
         lib.pmdd_dd_free.argtypes = [C.c_void_p]
         lib.pmdd_dd_free.restype = None
+        lib.pmdd_frontier_sort.argtypes = [C.POINTER(Frontier)]
+        lib.pmdd_frontier_sort.restype = None
         _host = lib
     return _host
 

@@ -143,16 +143,8 @@ int main(int argc, char *argv[]) {
 
     // lexicographically ascending, as written by the reference (main.cpp:403)
     const int d = fr.d;
-    std::vector<int64_t> order(fr.count);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-        return std::lexicographical_compare(fr.points + a * d, fr.points + a * d + d,
-                                            fr.points + b * d, fr.points + b * d + d);
-    });
-    std::vector<int32_t> sorted(static_cast<size_t>(fr.count) * d);
-    for (int64_t i = 0; i < fr.count; ++i) {
-        std::copy(fr.points + order[i] * d, fr.points + order[i] * d + d, sorted.begin() + i * d);
-    }
+    pmdd_frontier_sort(&fr);
+    std::vector<int32_t> sorted(fr.points, fr.points + static_cast<size_t>(fr.count) * d);
     if (opts.save_frontier) {
         std::string save_error;
         if (!write_frontier_gzip_csv(sorted, d, opts.frontier_out_path, &save_error)) {

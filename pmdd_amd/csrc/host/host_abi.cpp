@@ -1,8 +1,12 @@
 // C ABI of the host surface (declared in include/pmdd_host.h).
+#include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <string>
+#include <vector>
 
 #include "../../../include/pmdd_host.h"
 #include "dd.hpp"
@@ -65,3 +69,51 @@ extern "C" int64_t pmdd_dd_layer_size(const pmdd_dd *dd, int layer) {
 }
 
 extern "C" void pmdd_dd_free(pmdd_dd *dd) { delete dd; }
+
+namespace {
+
+// Rows of D ints sorted as fixed-size arrays (lexicographic operator<).
+template <int D> void sort_rows(int32_t *p, int64_t n) {
+    std::vector<std::array<int32_t, D>> rows(static_cast<size_t>(n));
+    std::memcpy(rows.data(), p, sizeof(int32_t) * D * static_cast<size_t>(n));
+    std::sort(rows.begin(), rows.end());
+    std::memcpy(p, rows.data(), sizeof(int32_t) * D * static_cast<size_t>(n));
+}
+
+} // namespace
+
+// ParetoFrontier::sort_lexicographic_ascending (src/pmdd/enum/pareto_frontier.hpp:418-433),
+// the order main() writes (src/pmdd/main.cpp:403).
+extern "C" void pmdd_frontier_sort(pmdd_frontier *fr) {
+    if (fr == nullptr || fr->points == nullptr || fr->count < 2) {
+        return;
+    }
+    const int64_t n = fr->count;
+    switch (fr->d) {
+    case 1: sort_rows<1>(fr->points, n); return;
+    case 2: sort_rows<2>(fr->points, n); return;
+    case 3: sort_rows<3>(fr->points, n); return;
+    case 4: sort_rows<4>(fr->points, n); return;
+    case 5: sort_rows<5>(fr->points, n); return;
+    case 6: sort_rows<6>(fr->points, n); return;
+    case 7: sort_rows<7>(fr->points, n); return;
+    case 8: sort_rows<8>(fr->points, n); return;
+    default: break;
+    }
+    const int d = fr->d;
+    if (d <= 0) {
+        return;
+    }
+    std::vector<int64_t> order(static_cast<size_t>(n));
+    std::iota(order.begin(), order.end(), 0);
+    const int32_t *pts = fr->points;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return std::lexicographical_compare(pts + a * d, pts + a * d + d, pts + b * d,
+                                            pts + b * d + d);
+    });
+    std::vector<int32_t> sorted(static_cast<size_t>(n) * d);
+    for (int64_t i = 0; i < n; ++i) {
+        std::copy(pts + order[i] * d, pts + order[i] * d + d, sorted.begin() + i * d);
+    }
+    std::copy(sorted.begin(), sorted.end(), fr->points);
+}

@@ -25,14 +25,6 @@ class EnumerationResult:
     stats: dict = field(default_factory=dict)
 
 
-def sort_lexicographic_ascending(points: np.ndarray) -> np.ndarray:
-    """The order main() writes (ParetoFrontier::sort_lexicographic_ascending,
-    src/pmdd/enum/pareto_frontier.hpp:418-433)."""
-    if len(points) == 0:
-        return points
-    return points[np.lexsort(points.T[::-1])]
-
-
 def _stats_dict(st: abi.Stats, nl: int, std_c, std_s) -> dict:
     out = {name: getattr(st, name) for name, _ in abi.Stats._fields_
            if name not in ("std_candidates_per_layer", "std_frontier_survivors_per_layer",
@@ -43,8 +35,12 @@ def _stats_dict(st: abi.Stats, nl: int, std_c, std_s) -> dict:
     return out
 
 
-def _take_frontier(lib, fr: abi.Frontier) -> np.ndarray:
+def _take_frontier(lib, fr: abi.Frontier, sort: bool = False) -> np.ndarray:
+    """The points of a returned frontier as an (n, d) array (freeing it),
+    lexicographically ascending when sort is set (pmdd_frontier_sort)."""
     n, d = int(fr.count), int(fr.d)
+    if sort and n > 1:
+        abi.host_lib().pmdd_frontier_sort(C.byref(fr))
     if n > 0:
         arr = np.ctypeslib.as_array(fr.points, shape=(n * d,)).copy().reshape(n, d)
     else:
@@ -103,9 +99,7 @@ class Engine:
                                       C.byref(st), err, len(err))
         if rc != abi.PMDD_OK:
             raise PmddError(rc, err.value.decode())
-        pts = _take_frontier(self._lib, fr)
-        if sort:
-            pts = sort_lexicographic_ascending(pts)
+        pts = _take_frontier(self._lib, fr, sort)
         return EnumerationResult(pts, _stats_dict(st, nl, std_c, std_s))
 
     def upload(self, dd) -> "ResidentDD":
@@ -158,9 +152,7 @@ class ResidentDD:
                                               C.byref(st), err, len(err))
         if rc != abi.PMDD_OK:
             raise PmddError(rc, err.value.decode())
-        pts = _take_frontier(eng._lib, fr)
-        if sort:
-            pts = sort_lexicographic_ascending(pts)
+        pts = _take_frontier(eng._lib, fr, sort)
         return EnumerationResult(pts, _stats_dict(st, self.num_layers, None, None))
 
     def close(self) -> None:

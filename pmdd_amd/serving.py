@@ -28,7 +28,7 @@ import numpy as np
 from . import abi
 from .abi import PmddError
 from .dd import DecisionDiagram
-from .engine import EnumerationResult, _stats_dict, _take_frontier, sort_lexicographic_ascending
+from .engine import EnumerationResult, _stats_dict, _take_frontier
 from . import instances as gen
 
 
@@ -67,7 +67,7 @@ class Server:
             return None
         if rc != abi.PMDD_OK:
             raise PmddError(rc, err.value.decode())
-        pts = sort_lexicographic_ascending(_take_frontier(self._lib, fr))
+        pts = _take_frontier(self._lib, fr, True)
         return EnumerationResult(pts, _stats_dict(st, 0, None, None))
 
     def poll(self, ticket: int):
@@ -89,7 +89,7 @@ class Server:
             raise PmddError(rc, err.value.decode())
         if rc != abi.PMDD_OK:
             return int(t.value), PmddError(rc, err.value.decode())
-        pts = sort_lexicographic_ascending(_take_frontier(self._lib, fr))
+        pts = _take_frontier(self._lib, fr, True)
         return int(t.value), EnumerationResult(pts, _stats_dict(st, 0, None, None))
 
     def close(self) -> None:

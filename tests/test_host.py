@@ -340,3 +340,21 @@ def test_cli_program_name_fixes_objective_count(tmp_path):
     r = subprocess.run([str(link), instance_path("KP_p-3_n-10_ins-1.dat"), "1", "1", "0", "gpu"],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "expects 4" in r.stdout
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 5, 8, 9])
+def test_frontier_sort_matches_lexsort(d):
+    """pmdd_frontier_sort orders rows as ParetoFrontier::sort_lexicographic_ascending
+    (pareto_frontier.hpp:418-433): lexicographically ascending, duplicates kept."""
+    import ctypes as C
+    from pmdd_amd import abi
+    rng = np.random.default_rng(100 + d)
+    pts = rng.integers(-5, 6, size=(3000, d)).astype(np.int32)  # many ties
+    want = pts[np.lexsort(pts.T[::-1])]
+    buf = np.ascontiguousarray(pts.copy())
+    fr = abi.Frontier()
+    fr.d = d
+    fr.count = len(buf)
+    fr.points = buf.ctypes.data_as(C.POINTER(C.c_int32))
+    abi.host_lib().pmdd_frontier_sort(C.byref(fr))
+    assert (buf == want).all()
