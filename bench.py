@@ -248,8 +248,9 @@ def main():
         # k_filter_chunks: compulsory bytes of the expanded layers and joins
         "k_filter_chunks": (tot["kernel_filter_s"], tot["filter_launches"],
                             tot["algorithmic_bytes"]),
-        # k_refill_vs_block: products read once per block, survivors written,
-        # the block of R read once
+        # k_refill_vs_block: products read once per block of R, survivors
+        # written, the block read once; and R flagged against blocks of the
+        # batch frontier (R and the block read once, R's flags read and written)
         "k_refill_vs_block": (tot["kernel_frontier_check_s"], tot["frontier_check_launches"],
                               tot["frontier_check_bytes"]),
         # k_cross_chunks (one launch per wave): per wave the group's front and
@@ -275,6 +276,13 @@ def main():
         "launches_per_step": k_launches / max(args.steps, 1),
         "algorithmic_bytes_per_launch": alg_bytes_per_launch,
         "share_of_engine_kernel_time": k_time / max(tot["kernel_total_s"], 1e-12),
+        # every family's live per-launch average, for comparison with rocprofv3
+        "families": {
+            k: {"avg_launch_ms": v[0] / max(v[1], 1) * 1e3,
+                "launches_per_step": v[1] / max(args.steps, 1),
+                "share_of_engine_kernel_time": v[0] / max(tot["kernel_total_s"], 1e-12)}
+            for k, v in kernels.items()
+        },
     }
     if pair_tests is not None:
         d_avg = sum(d for _, _, _, d in WORKLOAD) / len(WORKLOAD)

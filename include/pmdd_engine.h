@@ -137,10 +137,13 @@ typedef struct pmdd_stats {
     int64_t filter_launches;
     int64_t filter_candidates;
     int64_t algorithmic_bytes;
-    /* The coupling kernel that checks products against the running frontier
-     * block by block (k_refill_vs_block): device time, launches, compulsory
-     * bytes (per block: its input points read once, its survivors written
-     * once, the block of the frontier read once). */
+    /* The coupling kernel k_refill_vs_block, both of its uses: products
+     * checked against the running frontier block by block (per launch: its
+     * input points read once, its survivors written once, the block of the
+     * frontier read once), and the running frontier's points flagged when a
+     * block of the batch frontier dominates them (per launch: the frontier and
+     * the block read once, the flags read and written).  Device time,
+     * launches, compulsory bytes.  Part of kernel_dominance_s. */
     double kernel_frontier_check_s;
     int64_t frontier_check_launches;
     int64_t frontier_check_bytes;

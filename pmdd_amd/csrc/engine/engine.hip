@@ -920,24 +920,32 @@ struct Run {
         cb.maxkey.ensure(n * 8, "hipMalloc(chunk keys)");
     }
 
-    // Launch a filter over the listed chunks (all chunks when `which` is empty
-    // and `count` > 0).
-    void run_filter(FilterFn f, ChunkBufs &cb, DevBuf &wbuf, const std::vector<int32_t> &which,
-                    int64_t count, const void *src, int32_t *out, int64_t stride, uint8_t *alive) {
+    // A chunk list for run_filter, copied to the device ahead of the launch.
+    void upload_list(DevBuf &wbuf, const std::vector<int32_t> &which) {
+        if (!which.empty()) {
+            h2d(ctx, wbuf, which, "hipMemcpy(chunk list)");
+        }
+    }
+
+    // Launch a filter over the listed chunks (the list already uploaded to
+    // wbuf), or over chunks 0 .. count-1 when `which` is empty; returns the
+    // number of launches (0 or 1).
+    int run_filter(FilterFn f, ChunkBufs &cb, const DevBuf &wbuf, const std::vector<int32_t> &which,
+                   int64_t count, const void *src, int32_t *out, int64_t stride, uint8_t *alive) {
         const int32_t *w = nullptr;
         int64_t n = count;
         if (!which.empty()) {
-            h2d(ctx, wbuf, which, "hipMemcpy(chunk list)");
             w = wbuf.as<int32_t>();
             n = static_cast<int64_t>(which.size());
         }
         if (n <= 0) {
-            return;
+            return 0;
         }
         f(dim3(static_cast<unsigned>(n)), s, w, cb.chunks.as<ChunkDesc>(), src, out, stride, alive,
           cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), cb.minkey.as<int64_t>(),
           cb.maxkey.as<int64_t>(), d_tests);
         hip_check(hipGetLastError(), "k_filter_chunks");
+        return 1;
     }
 
     // Copy the listed chunks' candidates from a key-ordered source, all alive.
@@ -1233,16 +1241,12 @@ struct Run {
                 hip_check(hipGetLastError(), "k_gather_runs");
                 sorted = merge_runs(ctx->sort_a, ctx->sort_b, bigpos, run_start, grp_run, max_runs);
             }
-            hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
             upload_table(cb, chunks);
             ctx->surv_pts.ensure(static_cast<size_t>(pass_c) * d * 4, "hipMalloc(survivor buffer)");
             ctx->alive.ensure(static_cast<size_t>(pass_c), "hipMalloc(alive flags)");
-            run_filter(K.filter_expand_small, cb, ctx->which_s, which_tiny, 0, &src,
-                       ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
-            run_filter(K.filter_expand_mid, cb, ctx->which_m, which_mid, 0, &src,
-                       ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
-            run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
-                       ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
+            upload_list(ctx->which_s, which_tiny);
+            upload_list(ctx->which_m, which_mid);
+            upload_list(ctx->which_a, which_small);
             if (!which_big.empty()) {
                 // big nodes: candidates staged as they are; the cross filter
                 // does all of their checks
@@ -1250,6 +1254,14 @@ struct Run {
                 run_stage(cb, ctx->which_b, which_big, ss, ctx->surv_pts.as<int32_t>(), pass_c,
                           ctx->alive.as<uint8_t>());
             }
+            // only k_filter_chunks launches between ev_d and ev_c
+            hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
+            int filters = run_filter(K.filter_expand_small, cb, ctx->which_s, which_tiny, 0, &src,
+                                     ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
+            filters += run_filter(K.filter_expand_mid, cb, ctx->which_m, which_mid, 0, &src,
+                                  ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
+            filters += run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
+                                  ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
             int cross_waves = 0;
             if (!which_cross.empty()) {
@@ -1267,7 +1279,7 @@ struct Run {
             add_time(ctx->ev_a, ctx->ev_d, &st->kernel_expand_td_s);
             add_time(ctx->ev_d, ctx->ev_c, &st->kernel_filter_s);
             add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
-            st->filter_launches += 1;
+            st->filter_launches += filters;
             st->filter_candidates += pass_c;
 
             // dense placement of the survivors, node by node
@@ -1302,11 +1314,14 @@ struct Run {
         }
     }
 
+    // Adds the seconds between two passed events to *acc; expansion and
+    // dominance time also count in kernel_total_s (the filter, frontier-check
+    // and cross figures are parts of the dominance time).
     void add_time(hipEvent_t a, hipEvent_t b, double *acc) {
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
         *acc += ms * 1e-3;
-        if (acc != &st->kernel_filter_s) {
+        if (acc == &st->kernel_expand_td_s || acc == &st->kernel_dominance_s) {
             st->kernel_total_s += ms * 1e-3;
         }
     }
@@ -1787,13 +1802,14 @@ struct Run {
             ctx->alive.ensure(static_cast<size_t>(ytotal), "hipMalloc(alive flags)");
             pmdd_k::SortedSrc fsrc{F, bprod};
             hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            run_filter(K.filter_dense, cb, ctx->which_a, none, cnt, &fsrc,
-                       ctx->surv_pts.as<int32_t>(), ytotal, ctx->alive.as<uint8_t>());
+            const int dense = run_filter(K.filter_dense, cb, ctx->which_a, none, cnt, &fsrc,
+                                         ctx->surv_pts.as<int32_t>(), ytotal,
+                                         ctx->alive.as<uint8_t>());
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
             d2h(ctx, nalive, cb.nalive, chunks.size(), "product survivor counts");
             add_time(ctx->ev_a, ctx->ev_c, &st->kernel_filter_s);
             add_time(ctx->ev_a, ctx->ev_c, &st->kernel_dominance_s);
-            st->filter_launches += 1;
+            st->filter_launches += dense;
             st->filter_candidates += ytotal;
             outpos.resize(chunks.size());
             run_start.clear();
@@ -1826,13 +1842,15 @@ struct Run {
                 c.group_count = nb;
                 chunks.push_back(c);
             }
-            hip_check(hipEventRecord(ctx->ev_g, s), "hipEventRecord");
+            int sorted_filters = 0;
             if (nb > 0) {
                 upload_table(cb, chunks);
+                hip_check(hipEventRecord(ctx->ev_g, s), "hipEventRecord");
                 pmdd_k::SortedSrc ss{ys, ytotal};
                 if (nb == 1) {
-                    run_filter(K.filter_sorted, cb, ctx->which_a, none, nb, &ss,
-                               ctx->surv_pts.as<int32_t>(), ytotal, ctx->alive.as<uint8_t>());
+                    sorted_filters = run_filter(K.filter_sorted, cb, ctx->which_a, none, nb, &ss,
+                                                ctx->surv_pts.as<int32_t>(), ytotal,
+                                                ctx->alive.as<uint8_t>());
                 } else {
                     std::vector<int32_t> all(nb);
                     for (int32_t k = 0; k < nb; ++k) {
@@ -1857,6 +1875,7 @@ struct Run {
                     btotal += nalive[c];
                 }
             } else {
+                hip_check(hipEventRecord(ctx->ev_g, s), "hipEventRecord");
                 hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
             }
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
@@ -1865,9 +1884,12 @@ struct Run {
                 std::fprintf(stderr, " batch_frontier=%lld R=%lld\n",
                              static_cast<long long>(btotal), static_cast<long long>(r_count));
             }
-            add_time(ctx->ev_g, ctx->ev_c, &st->kernel_filter_s);
+            // ev_g .. ev_c: one k_filter_chunks launch, or the staging of a
+            // batch of several chunks
+            add_time(ctx->ev_g, ctx->ev_c,
+                     sorted_filters > 0 ? &st->kernel_filter_s : &st->kernel_expand_td_s);
             add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
-            st->filter_launches += 1;
+            st->filter_launches += sorted_filters;
             st->filter_candidates += ytotal;
             st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + btotal);
             if (btotal == 0) {
@@ -1912,13 +1934,19 @@ struct Run {
             hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
             const unsigned rgrid = static_cast<unsigned>(
                 std::min<int64_t>(ceil_div(r_count, pmdd_k::kWave), kRefillWaves));
+            hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
             for (int64_t lo = 0; lo < btotal; lo += kEvictBlock) {
+                const int64_t hi = std::min<int64_t>(btotal, lo + kEvictBlock);
                 K.refill_flags(dim3(rgrid), s, R.pts.as<int32_t>(), R.cap,
                                ctx->join_ctr.as<unsigned long long>(), B.pts.as<int32_t>(), B.cap,
-                               lo, std::min<int64_t>(btotal, lo + kEvictBlock),
-                               ctx->keep_b.as<uint8_t>(), d_tests);
+                               lo, hi, ctx->keep_b.as<uint8_t>(), d_tests);
                 hip_check(hipGetLastError(), "k_refill_vs_block");
+                // R and the block of B read once, R's flags read and written
+                st->frontier_check_launches += 1;
+                st->frontier_check_bytes += static_cast<int64_t>(d) * 4 * (r_count + hi - lo) +
+                                            2 * r_count;
             }
+            hip_check(hipEventRecord(ctx->ev_f, s), "hipEventRecord");
             hipLaunchKernelGGL(pmdd_k::k_count_chunks, dim3(static_cast<unsigned>(rn)),
                                dim3(kBlock), 0, s, cr.chunks.as<ChunkDesc>(),
                                cr.nsurv.as<int32_t>(), ctx->keep_b.as<uint8_t>(),
@@ -1927,6 +1955,7 @@ struct Run {
             hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
             d2h(ctx, rnalive, cr.nalive, rchunks.size(), "frontier survivor counts");
             add_time(ctx->ev_a, ctx->ev_b, &st->kernel_dominance_s);
+            add_time(ctx->ev_e, ctx->ev_f, &st->kernel_frontier_check_s);
             routpos.resize(rn);
             int64_t ralive = 0;
             for (int32_t k = 0; k < rn; ++k) {
