@@ -1884,10 +1884,11 @@ struct Run {
                 std::fprintf(stderr, " batch_frontier=%lld R=%lld\n",
                              static_cast<long long>(btotal), static_cast<long long>(r_count));
             }
-            // ev_g .. ev_c: one k_filter_chunks launch, or the staging of a
-            // batch of several chunks
-            add_time(ctx->ev_g, ctx->ev_c,
-                     sorted_filters > 0 ? &st->kernel_filter_s : &st->kernel_expand_td_s);
+            // ev_g .. ev_c (inside the dominance time): one k_filter_chunks
+            // launch, or the staging of a batch of several chunks
+            if (sorted_filters > 0) {
+                add_time(ctx->ev_g, ctx->ev_c, &st->kernel_filter_s);
+            }
             add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
             st->filter_launches += sorted_filters;
             st->filter_candidates += ytotal;

@@ -193,3 +193,30 @@ def test_pair_test_counter_is_deterministic(engine):
     a = engine.enumerate(dd, method=1, count_pair_tests=True).stats["dominance_pair_tests"]
     b = engine.enumerate(dd, method=1, count_pair_tests=True).stats["dominance_pair_tests"]
     assert a == b and a > 0
+
+
+def test_engine_reuse_across_objective_counts(engine, tmp_path):
+    """One context runs instances of different d in turn: frontier buffers are
+    reused between calls with the SoA stride recomputed for each d."""
+    cases = [gen.knapsack(61, 24, 5), gen.knapsack(62, 30, 3), gen.setpacking(63, 40, 4, m=8),
+             gen.knapsack(64, 22, 5)]
+    for k, inst in enumerate(cases):
+        path = gen.write(inst, str(tmp_path / f"r{k}.dat"))
+        dd = DecisionDiagram.from_file(path, inst["kind"])
+        ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
+        for method in (1, 3, 2):
+            print(f"case reuse k={k} d={inst['d']} method={method}", flush=True)
+            assert _same(engine.enumerate(dd, method=method).points, ref), (k, method)
+
+
+def test_kernel_time_breakdown_is_consistent(engine):
+    """kernel_total_s is expansion plus dominance time; the filter, frontier
+    check and cross figures are parts of the dominance time."""
+    dd = _dd("KP_p-3_n-30_ins-2.dat")
+    for method in (1, 3):
+        s = engine.enumerate(dd, method=method, state_dominance=1).stats
+        assert s["kernel_total_s"] == pytest.approx(
+            s["kernel_expand_td_s"] + s["kernel_dominance_s"], rel=1e-9, abs=1e-12)
+        parts = s["kernel_filter_s"] + s["kernel_frontier_check_s"] + s["kernel_cross_s"]
+        assert parts <= s["kernel_dominance_s"] * (1 + 1e-6) + 1e-9
+        assert s["filter_launches"] > 0
