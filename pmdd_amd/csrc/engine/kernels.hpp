@@ -171,6 +171,7 @@ __device__ __forceinline__ int64_t unpack_key(uint64_t w) {
 
 // Block-wide exclusive prefix of a 0/1 flag; returns the rank and writes the
 // block total to *total.  Uses wave ballots and one LDS slot per wave.
+template <int NW = kWaves>
 __device__ __forceinline__ int block_rank(bool flag, int *wave_tot, int *total) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -183,7 +184,7 @@ __device__ __forceinline__ int block_rank(bool flag, int *wave_tot, int *total) 
     int before = 0;
     int sum = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
+    for (int w = 0; w < NW; ++w) {
         const int t = wave_tot[w];
         before += (w < wave) ? t : 0;
         sum += t;
@@ -285,8 +286,8 @@ __device__ __forceinline__ bool scan_until_dead(const int32_t *S, int n, int lim
 // ---------------------------------------------------------------------------
 // k_filter_chunks: per-chunk load, presort, sort-filter skyline.
 // ---------------------------------------------------------------------------
-template <int D, int CH, class Src, bool PRESORTED>
-__global__ __launch_bounds__(kBlock) void k_filter_chunks(
+template <int D, int CH, class Src, bool PRESORTED, int NT = kBlock>
+__global__ __launch_bounds__(NT) void k_filter_chunks(
     const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks, Src src,
     int32_t *__restrict__ out_pts,
     int64_t out_stride, uint8_t *__restrict__ alive, int32_t *__restrict__ chunk_nsurv,
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
     __shared__ int32_t cs[D][CH];
     __shared__ uint64_t sk[CH];
     __shared__ __align__(16) int32_t S[CH * DP];
-    __shared__ int wave_tot[kWaves];
+    __shared__ int wave_tot[NT / kWave];
 
     const int ci = which != nullptr ? which[blockIdx.x] : static_cast<int>(blockIdx.x);
     const ChunkDesc c = chunks[ci];
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
     while (npad < n) {
         npad <<= 1;
     }
-    for (int t = tid; t < npad; t += kBlock) {
+    for (int t = tid; t < npad; t += NT) {
         if (t < n) {
             int32_t p[D];
             load_candidate<D>(src, c, t, p);
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
     // bitonic sort, descending (skipped when the source is in key order)
     for (int k = 2; !PRESORTED && k <= npad; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = tid; t < (npad >> 1); t += kBlock) {
+            for (int t = tid; t < (npad >> 1); t += NT) {
                 const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
                 const int l = i + j;
                 const uint64_t a = sk[i];
@@ -342,13 +343,13 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
         }
     }
 
-    // sort-filter skyline, 256 candidates per step
+    // sort-filter skyline, NT candidates per step
     int nS = 0;
     unsigned long long tests = 0;
-    for (int base = 0; base < n; base += kBlock) {
+    for (int base = 0; base < n; base += NT) {
         const int i = base + tid;
         const bool valid = i < n;
-        const int tcount = min(kBlock, n - base);
+        const int tcount = min(NT, n - base);
         int32_t m[D];
         if (valid) {
             const int slot = unpack_slot(sk[i]);
@@ -373,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
         dead = scan_until_dead<D, DP>(S + nS * DP, wave_end, min(tid, tcount), m, dead, tests);
         __syncthreads();
         int total = 0;
-        const int rank = block_rank(!dead, wave_tot, &total);
+        const int rank = block_rank<NT / kWave>(!dead, wave_tot, &total);
         if (!dead) {
 #pragma unroll
             for (int o = 0; o < D; ++o) {
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_chunks(
         __syncthreads();
     }
 
-    for (int s = tid; s < nS; s += kBlock) {
+    for (int s = tid; s < nS; s += NT) {
         const int64_t pos = c.out_base + s;
 #pragma unroll
         for (int o = 0; o < D; ++o) {
