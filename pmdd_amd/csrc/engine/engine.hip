@@ -1023,9 +1023,10 @@ struct Run {
             if (n == 0) {
                 continue;
             }
-            // spread each chunk's scan over G workgroups, about 1024 in all
+            // spread each chunk's scan over G workgroups, about 2048 in all
+            // (1536 fit the chip at this kernel's LDS use)
             const int G = static_cast<int>(
-                std::min<int64_t>(kCrossSplit, std::max<int64_t>(1, ceil_div(1024, n))));
+                std::min<int64_t>(kCrossSplit, std::max<int64_t>(1, ceil_div(2048, n))));
             hip_check(hipEventRecord(pool_event(2 * launches), s), "hipEventRecord");
             K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + start[w], cb.chunks.as<ChunkDesc>(),
                     pts, stride, alive, cb.nsurv.as<int32_t>(), fv, fv_stride, fin, kCrossWave, G,
