@@ -28,6 +28,12 @@ same in the plain and the batched runs.
 Profiling helpers (for rocprofv3 runs): --only NAME and --cases M:SD,... run a
 subset of the step; --calib-copy-mib N runs one plain device copy of N MiB
 (torch) and exits, to scale the FETCH_SIZE / WRITE_SIZE counters.
+
+--paper runs the reference paper's benchmark instance families instead
+(tests/golden/paper/, the instances of its experiments, 10 per family): each
+instance one-shot (upload, enumeration, result copy) with method 3 and
+method 1, printing one JSON line per instance and per family with the
+paper's mean H100 times beside ours.
 """
 import argparse
 import json
@@ -147,6 +153,71 @@ def serving_leg(tmpdir, requests, slots, method):
     return out
 
 
+# The reference paper's instance families (reports/aaai/aaai2026-article.tex,
+# tables at lines 546-596 and 865-938): (family, problem type, state dominance,
+# mean seconds on an H100 for PmddG (method 3) and PmddG-TD (method 1)).
+PAPER_FAMILIES = [
+    ("mokp_n40_k6", 1, 1, 2.0, 3.0),
+    ("mokp_n40_k7", 1, 1, 4.0, 5.0),
+    ("mokp_n50_k5", 1, 1, 4.0, 3.0),
+    ("mokp_n70_k3", 1, 1, 2.0, 2.0),
+    ("mospp_n150_k3", 2, 0, 5.0, 7.0),
+    ("mospp_n150_k4", 2, 0, 6.0, 6.0),
+    ("mospp_n150_k7", 2, 0, 17.0, 9.0),
+    ("motsp_n15_k4", 3, 0, 3.0, 3.0),
+    ("motsp_n15_k5", 3, 0, 48.0, 36.0),
+    # the paper's hardest settings: method 1 solved 6/10 (4 out of memory)
+    # and 0/10, method 3 10/10 and 6/10 of them
+    ("mokp_n70_k6", 1, 1, 1010.0, 335.0),
+    ("motsp_n15_k6", 3, 0, 369.0, None),
+    ("motsp_n15_k7", 3, 0, 1828.0, None),
+]
+
+
+def paper_sets(guard_s, only, limit):
+    """One-shot method 3 and method 1 on every instance of the paper's
+    families; a family stops after an instance slower than guard_s."""
+    from pmdd_amd import DecisionDiagram
+    from pmdd_amd.engine import Engine
+    eng = Engine(0)
+    base = os.path.join(ROOT, "tests", "golden", "paper")
+    warm = DecisionDiagram.from_file(os.path.join(base, "mokp_n70_k3",
+                                                  sorted(os.listdir(os.path.join(
+                                                      base, "mokp_n70_k3")))[0]), 1)
+    eng.enumerate(warm, method=1)  # device and context set-up outside the timings
+    for fam, pt, sd, paper_m3, paper_m1 in PAPER_FAMILIES:
+        if only and fam not in only:
+            continue
+        files = sorted(os.listdir(os.path.join(base, fam)))
+        if limit > 0:
+            files = files[:limit]
+        times = {1: [], 3: []}
+        for f in files:
+            t = time.perf_counter()
+            dd = DecisionDiagram.from_file(os.path.join(base, fam, f), pt)
+            build_s = time.perf_counter() - t
+            row = {"family": fam, "instance": f, "build_s": build_s}
+            pts = {}
+            for method in (3, 1):
+                t = time.perf_counter()
+                out = eng.enumerate(dd, method=method, state_dominance=sd)
+                dt = time.perf_counter() - t
+                times[method].append(dt)
+                pts[method] = out.points
+                row[f"m{method}_s"] = dt
+            row["points"] = len(pts[3])
+            row["m1_points"] = len(pts[1])
+            row["m1_equals_m3"] = bool(pts[1].shape == pts[3].shape and (pts[1] == pts[3]).all())
+            print(json.dumps(row), flush=True)
+            if max(row["m3_s"], row["m1_s"]) > guard_s:
+                log(f"{fam}: {f} took more than {guard_s} s; rest of the family skipped")
+                break
+        mean = {m: sum(v) / len(v) for m, v in times.items() if v}
+        print(json.dumps({"family": fam, "solved": len(times[3]), "of": len(files),
+                          "mean_m3_s": mean.get(3), "mean_m1_s": mean.get(1),
+                          "paper_h100_m3_s": paper_m3, "paper_h100_m1_s": paper_m1}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,7 +234,16 @@ def main():
                     help="requests of the serving stream to replay (0: skip; 2000: all)")
     ap.add_argument("--serving-slots", type=int, default=8)
     ap.add_argument("--serving-method", type=int, default=1)
+    ap.add_argument("--paper", action="store_true",
+                    help="run the reference paper's instance families instead of the step")
+    ap.add_argument("--paper-guard-s", type=float, default=120.0)
+    ap.add_argument("--paper-families", default="", help="comma list (default: all)")
+    ap.add_argument("--paper-limit", type=int, default=0, help="first N instances per family")
     args = ap.parse_args()
+    if args.paper:
+        paper_sets(args.paper_guard_s, [f for f in args.paper_families.split(",") if f],
+                   args.paper_limit)
+        return
     if args.calib_copy_mib:
         import torch
         n = args.calib_copy_mib * (1 << 20) // 4

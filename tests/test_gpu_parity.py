@@ -220,3 +220,14 @@ def test_kernel_time_breakdown_is_consistent(engine):
         parts = s["kernel_filter_s"] + s["kernel_frontier_check_s"] + s["kernel_cross_s"]
         assert parts <= s["kernel_dominance_s"] * (1 + 1e-6) + 1e-9
         assert s["filter_launches"] > 0
+
+
+def test_tsp15_5obj_seed13095_known_count(engine):
+    """117171 solutions: the reference's CPU run of this instance,
+    kb/autoresearch/TSP_STATUS.md:50 (the reference GPU path ran out of
+    memory on it before chunking, kb/autoresearch/TSP_STATUS.md:28-37)."""
+    dd = _dd("tsp-nobj5-ncities15-seed13095.dat")
+    a = engine.enumerate(dd, method=3).points
+    assert len(a) == 117171
+    b = engine.enumerate(dd, method=1).points
+    assert _same(a, b)
