@@ -133,7 +133,7 @@ double now_s() {
 // Device tables of one chunk list: descriptors, per-chunk survivor counts
 // (local and after cross checks), key range of the survivors, output slots.
 struct ChunkBufs {
-    DevBuf chunks, nsurv, nalive, minkey, maxkey, out;
+    DevBuf chunks, nsurv, nalive, out;
 };
 
 // Pinned host memory for the small host<->device copies of an enumeration
@@ -552,8 +552,8 @@ void grow_points(DevFrontier &f, int d, int64_t need, int64_t keep, hipStream_t 
 // Kernel launchers, one instantiation per objective count.
 // ---------------------------------------------------------------------------
 using FilterFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const void *,
-                          int32_t *, int64_t, uint8_t *, int32_t *, int32_t *, int64_t *,
-                          int64_t *, unsigned long long *);
+                          int32_t *, int64_t, uint8_t *, int32_t *, int32_t *,
+                          unsigned long long *);
 using CrossFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
                          int64_t, uint8_t *, const int32_t *, const int32_t *, int64_t,
                          const int64_t *, int, int, unsigned long long *);
@@ -621,11 +621,11 @@ template <int D> struct LaunchImpl {
     template <class Src, bool PRE, int C = CH>
     static void filter(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                        const void *src, int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns,
-                       int32_t *na, int64_t *kmin, int64_t *kmax, unsigned long long *pt) {
+                       int32_t *na, unsigned long long *pt) {
         constexpr int NT = C >= 1024 ? kFilterThreadsBig : pmdd_k::kBlock;
         hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, C, Src, PRE, NT>), g, dim3(NT), 0, s,
                            which, c, *static_cast<const Src *>(src), out, stride, alive, ns, na,
-                           kmin, kmax, pt);
+                           pt);
     }
     static void cross(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
@@ -924,8 +924,6 @@ struct Run {
         h2d(ctx, cb.chunks, chunks, "hipMemcpy(chunk table)");
         cb.nsurv.ensure(n * 4, "hipMalloc(chunk counts)");
         cb.nalive.ensure(n * 4, "hipMalloc(chunk counts)");
-        cb.minkey.ensure(n * 8, "hipMalloc(chunk keys)");
-        cb.maxkey.ensure(n * 8, "hipMalloc(chunk keys)");
     }
 
     // A chunk list for run_filter, copied to the device ahead of the launch.
@@ -950,8 +948,7 @@ struct Run {
             return 0;
         }
         f(dim3(static_cast<unsigned>(n)), s, w, cb.chunks.as<ChunkDesc>(), src, out, stride, alive,
-          cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), cb.minkey.as<int64_t>(),
-          cb.maxkey.as<int64_t>(), d_tests);
+          cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), d_tests);
         hip_check(hipGetLastError(), "k_filter_chunks");
         return 1;
     }
@@ -2158,8 +2155,6 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
             cb->chunks.release();
             cb->nsurv.release();
             cb->nalive.release();
-            cb->minkey.release();
-            cb->maxkey.release();
             cb->out.release();
         }
         for (DevBuf *b : {&c->run_off, &c->which, &c->which_a, &c->which_b, &c->which_c,

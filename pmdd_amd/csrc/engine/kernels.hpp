@@ -291,8 +291,7 @@ __global__ __launch_bounds__(NT) void k_filter_chunks(
     const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks, Src src,
     int32_t *__restrict__ out_pts,
     int64_t out_stride, uint8_t *__restrict__ alive, int32_t *__restrict__ chunk_nsurv,
-    int32_t *__restrict__ chunk_nalive, int64_t *__restrict__ chunk_minkey,
-    int64_t *__restrict__ chunk_maxkey, unsigned long long *__restrict__ pair_tests) {
+    int32_t *__restrict__ chunk_nalive, unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
     __shared__ int32_t cs[D][CH];
     __shared__ uint64_t sk[CH];
@@ -396,14 +395,6 @@ __global__ __launch_bounds__(NT) void k_filter_chunks(
     if (tid == 0) {
         chunk_nsurv[ci] = nS;
         chunk_nalive[ci] = nS;
-        int64_t kmax = 0, kmin = 0;
-#pragma unroll
-        for (int o = 0; o < D; ++o) {
-            kmax += S[o];
-            kmin += S[(nS - 1) * DP + o];
-        }
-        chunk_maxkey[ci] = kmax;
-        chunk_minkey[ci] = kmin;
     }
     if (pair_tests != nullptr) {
         atomicAdd(pair_tests, tests);
