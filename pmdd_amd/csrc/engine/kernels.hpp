@@ -1,34 +1,46 @@
 // Device kernels of the frontier engine (gfx950, wave64).
 //
 // Points are int32 vectors of D objectives stored structure-of-arrays in HBM
-// (objective o of point i at pts[o * stride + i]).  Larger is better.
+// (objective o of point i at pts[o * stride + i]).  Larger is better.  The key
+// of a point is the sum of its objectives; every node frontier is kept in
+// descending key order.
 //
 // The segmented Pareto filter works on "groups" (the candidates of one node
 // during a layer expansion; the products of one coupling batch) cut into
 // chunks of at most CH candidates:
 //
-//   k_filter_chunks   one workgroup per chunk.  Loads its candidates straight
-//                     from the parent frontiers (parent point + arc weight) or
-//                     the cutset frontiers (TD point + BU point) into LDS,
-//                     sorts them by key = sum of objectives, descending, with a
-//                     bitonic network, then runs a sort-filter skyline: a
-//                     candidate survives iff no earlier (higher-key) candidate
-//                     is >= it in every objective.  Ties (equal points) keep
-//                     the first copy.  Survivors go to HBM in key order.
-//   k_cross_chunks    groups of several chunks: every local survivor is
-//                     checked against the local survivors of the group's other
-//                     chunks; it dies when one is >= it everywhere and either
-//                     differs or sits in an earlier chunk (keeps one copy of
-//                     equal points).  Chunks whose largest key is below this
-//                     chunk's smallest key are skipped: a point can only be
-//                     weakly dominated by a point of larger or equal key.
+//   k_filter_chunks   one workgroup per chunk (a node of at most CH
+//                     candidates).  Loads its candidates straight from the
+//                     parent frontiers (parent point + arc weight) or a dense
+//                     key-ordered source into LDS, sorts them by key,
+//                     descending, with a bitonic network (skipped when the
+//                     source is in key order), then runs a sort-filter
+//                     skyline: a candidate survives iff no earlier
+//                     (higher-key) candidate is >= it in every objective.  Ties
+//                     (equal points) keep the first copy.  Survivors go to HBM
+//                     in key order.
+//   k_gather_runs,    big nodes: the parent runs of each node are copied out
+//   k_merge_pass      and merged pairwise into one key-ordered run.
+//   k_stage_chunks    cuts a key-ordered run into chunks, all points alive.
+//   k_cross_chunks    the chunks of a group in waves of W: a point dies when
+//                     a point of the group's front (survivors of the earlier
+//                     waves), an alive point of an earlier chunk of its wave,
+//                     or an earlier point of its own chunk is >= it.
+//   k_append_front    appends a wave's survivors to the group's front.
 //   k_compact_chunks  writes the alive survivors of each chunk densely.
+//
+// Coupling (method 3): k_join_products writes a batch's products; the refill
+// check k_refill_vs_block drops products that a block of the running frontier
+// R dominates and, in flag mode, the points of R that a block of the batch
+// frontier dominates.  State dominance: k_state_dominance drops the labels of
+// a node that a label of its comparison nodes dominates; k_count_kept and
+// k_compact_nodes rebuild the layer.
 //
 // Why it is exact: q >= p everywhere and q != p implies key(q) > key(p), so
 // after sorting by key descending only earlier points can dominate a point;
 // and if an earlier point dominates p, so does some earlier survivor
-// (transitivity), hence checking survivors only is enough.  Across chunks the
-// check is symmetric, so chunk order does not matter.
+// (transitivity), hence checking survivors only is enough, and checking
+// against a point that has itself died is harmless.
 #pragma once
 
 #include <hip/hip_runtime.h>
