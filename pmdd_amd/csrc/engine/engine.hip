@@ -9,7 +9,11 @@
 //   expand_layer_frontiers     bottomup.cu:242-533      -> Run::expand
 //   apply_knapsack_state_dominance topdown.cu:363-521   -> Run::state_dominance
 //   couple_cutsets_cuda        couple.cu:340-498        -> Run::couple
-// with kernels of its own design (kernels.hpp).
+// with kernels of its own design (kernels.hpp).  Also method 2 on the GPU
+// (bottomup_cpu_enumerate enum/cpu/enum.cpp:123-229, CPU only in the
+// reference) -> Run::bottomup, and a serving loop with no reference
+// counterpart (pmdd_server_*: worker slots with their own streams, memory
+// admission).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -55,9 +59,10 @@ void hip_check(hipError_t e, const char *where) {
     fail(code, std::string(where) + ": " + hipGetErrorString(e));
 }
 
-// Device buffer; grows on demand, contents not kept unless asked.
 // Device allocations made so far (device memory use changes only with them).
 static std::atomic<uint64_t> g_device_allocs{0};
+
+// Device buffer; grows on demand, contents not kept unless asked.
 
 struct DevBuf {
     void *p = nullptr;
