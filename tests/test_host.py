@@ -358,3 +358,22 @@ def test_frontier_sort_matches_lexsort(d):
     fr.points = buf.ctypes.data_as(C.POINTER(C.c_int32))
     abi.host_lib().pmdd_frontier_sort(C.byref(fr))
     assert (buf == want).all()
+
+
+def test_paper_instance_families_build():
+    """Every instance of the reference paper's families under
+    tests/golden/paper/ (the data of `bench.py --paper`) parses and builds
+    with its family's objective count."""
+    import os
+    base = os.path.join(os.path.dirname(__file__), "golden", "paper")
+    kinds = {"mokp": 1, "mospp": 2, "motsp": 3}
+    fams = sorted(os.listdir(base))
+    assert len(fams) >= 9
+    for fam in fams:
+        kind = kinds[fam.split("_")[0]]
+        k = int(fam.split("_k")[-1])
+        files = sorted(os.listdir(os.path.join(base, fam)))
+        assert len(files) == 10, fam
+        for f in files:
+            dd = DecisionDiagram.from_file(os.path.join(base, fam, f), kind, expect_d=k)
+            assert dd.d == k and dd.num_layers > 1, (fam, f)
