@@ -160,7 +160,11 @@ typedef struct pmdd_stats {
     double *std_frontier_survivors_per_layer;
 } pmdd_stats;
 
-/* Per-device engine context (device memory pools, streams, tuning table). */
+/* Per-device engine context: one HIP stream (default priority) and the device
+ * buffers its enumerations reuse.  Those buffers (pass scratch and frontier
+ * buffers) stay at the largest size any call needed until pmdd_ctx_destroy,
+ * so repeated enumerations allocate nothing.  Calls on one context must not
+ * overlap; use one context per thread (or the server below). */
 typedef struct pmdd_ctx pmdd_ctx;
 
 /* A diagram uploaded to device memory, reusable across enumerations. */
