@@ -20,7 +20,8 @@ set packing and TSP requests of uneven size, d in {3,4,5}; --serving-requests,
 2000 for the whole stream) through the plain one-at-a-time path (one slot)
 and the batched path (--serving-slots worker slots with their own streams,
 memory-based admission): first with every request queued at t=0 (capacity),
-then with Poisson arrivals at the plain path's capacity (latency), reporting
+then with Poisson arrivals at 0.5x, 1x and 1.5x the plain path's capacity
+(latency; --serving-rates), reporting
 instances/s, Pareto points/s and p50/p99 latency from arrival to result
 (host diagram construction included).  Every request's frontier must be the
 same in the plain and the batched runs.
@@ -95,11 +96,16 @@ def cpu_baseline(dds, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     name, path, dd = dds[0]
-    pts = 0
-    t0 = time.perf_counter()
-    for method in (1, 3):
-        pts += len(oracle.enumerate_dd(dd, method, 0, threads=threads))
-    dt = time.perf_counter() - t0
+
+    def run(thr):
+        pts = 0
+        t0 = time.perf_counter()
+        for method in (1, 3):
+            pts += len(oracle.enumerate_dd(dd, method, 0, threads=thr))
+        return pts, time.perf_counter() - t0
+
+    pts, dt = run(threads)
+    pts1, dt1 = run(1)
     return {
         "value": pts / dt,
         "unit": "Pareto pts/s",
@@ -107,12 +113,15 @@ def cpu_baseline(dds, threads):
         "kind": "port",
         "sample": f"{name} (seeded knapsack n=80 d=3) methods 1 and 3, state dominance 0; "
                   f"{pts} Pareto points in {dt:.2f} s",
+        "value_1_thread": pts1 / dt1,
+        "sample_1_thread": f"the same sample on 1 thread: {dt1:.2f} s",
     }
 
 
-def serving_leg(tmpdir, requests, slots, method):
+def serving_leg(tmpdir, requests, slots, method, rate_factors=(0.5, 1.0, 1.5)):
     """Plain (one slot) vs batched (`slots` slots) on the first `requests`
-    requests of the serving stream."""
+    requests of the serving stream: closed loop, then Poisson arrivals at
+    each factor of the plain path's closed-loop capacity."""
     from pmdd_amd import serving
     specs = serving.mixed_stream(requests)
     paths = serving.write_stream(specs, os.path.join(tmpdir, "stream"))
@@ -134,19 +143,26 @@ def serving_leg(tmpdir, requests, slots, method):
     same = all(i in res_plain and i in res_batched and
                res_plain[i].shape == res_batched[i].shape and
                bool((res_plain[i] == res_batched[i]).all()) for i in range(requests))
-    lam = plain["inst_per_s"]
-    arrivals = serving.poisson_arrivals(requests, lam)
-    plain_o = serving.run_stream(paths, specs, arrivals, slots=1, method=method,
-                                 progress=progress)
-    log("serving: plain open loop done")
-    batched_o = serving.run_stream(paths, specs, arrivals, slots=slots, method=method,
-                                   progress=progress)
-    log("serving: batched open loop done")
+    cap = plain["inst_per_s"]
     out["closed_loop"] = {"plain": {k: plain[k] for k in keys},
                           "batched": {k: batched[k] for k in keys}}
-    out["poisson"] = {"rate_per_s": lam,
-                      "plain": {k: plain_o[k] for k in keys},
-                      "batched": {k: batched_o[k] for k in keys}}
+    sweep = []
+    for f in sorted(set(rate_factors) | {1.0}):
+        lam = f * cap
+        arrivals = serving.poisson_arrivals(requests, lam)
+        plain_o = serving.run_stream(paths, specs, arrivals, slots=1, method=method,
+                                     progress=progress)
+        log(f"serving: plain open loop at {f}x capacity done")
+        batched_o = serving.run_stream(paths, specs, arrivals, slots=slots, method=method,
+                                       progress=progress)
+        log(f"serving: batched open loop at {f}x capacity done")
+        entry = {"rate_factor": f, "rate_per_s": lam,
+                 "plain": {k: plain_o[k] for k in keys},
+                 "batched": {k: batched_o[k] for k in keys}}
+        sweep.append(entry)
+        if f == 1.0:
+            out["poisson"] = {k: entry[k] for k in ("rate_per_s", "plain", "batched")}
+    out["poisson_sweep"] = sweep
     out["batched_over_plain_inst_per_s"] = batched["inst_per_s"] / max(plain["inst_per_s"], 1e-12)
     out["bit_exact_batched_vs_plain"] = same
     out["wall_s"] = time.perf_counter() - t0
@@ -234,6 +250,8 @@ def main():
                     help="requests of the serving stream to replay (0: skip; 2000: all)")
     ap.add_argument("--serving-slots", type=int, default=8)
     ap.add_argument("--serving-method", type=int, default=1)
+    ap.add_argument("--serving-rates", default="0.5,1,1.5",
+                    help="Poisson arrival rates as factors of the plain path's capacity")
     ap.add_argument("--paper", action="store_true",
                     help="run the reference paper's instance families instead of the step")
     ap.add_argument("--paper-guard-s", type=float, default=120.0)
@@ -382,7 +400,9 @@ def main():
 
     srv = None
     if args.serving_requests > 0:
-        srv = serving_leg(tmp, args.serving_requests, args.serving_slots, args.serving_method)
+        rates = tuple(float(x) for x in args.serving_rates.split(",") if x)
+        srv = serving_leg(tmp, args.serving_requests, args.serving_slots, args.serving_method,
+                          rates)
         if not srv["bit_exact_batched_vs_plain"]:
             raise SystemExit("serving: batched and plain frontiers differ")
 
