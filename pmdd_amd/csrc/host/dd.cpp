@@ -2,6 +2,8 @@
 #include "dd.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <unordered_map>
 #include <climits>
 #include <cstring>
@@ -256,6 +258,22 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
     return partner;
 }
 
+// Runs f(begin, end, t) over T contiguous ranges of [0, n) on T host threads
+// (T = 1: on the calling thread).
+template <class F> static void parallel_ranges(int64_t n, int T, F f) {
+    if (T <= 1) {
+        f(int64_t(0), n, 0);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; ++t) {
+        pool.emplace_back([&, t]() { f(n * t / T, n * (t + 1) / T, t); });
+    }
+    for (std::thread &th : pool) {
+        th.join();
+    }
+}
+
 BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
                                 std::vector<int32_t> *var_layer,
                                 std::vector<std::vector<int32_t>> *partners) {
@@ -308,27 +326,42 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
         active[v] = v;
     }
     std::vector<int64_t> counter(n, 0);
-    std::vector<int64_t> byte_hist(static_cast<size_t>(W) * 8 * 256, 0);
+    const size_t hist_len = static_cast<size_t>(W) * 8 * 256;
+    // wide layers are split over host threads (the node order is unchanged)
+    const int max_threads = static_cast<int>(
+        std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<int64_t> byte_hist(hist_len * max_threads, 0);
     WordKeyMap next_map(W);
     std::vector<uint64_t> child_keys;
-    std::vector<int64_t> child_slot, child_ids;
+    std::vector<int64_t> child_slot, child_ids, child_base(max_threads + 1);
 
     for (int l = 1; l <= n; ++l) {
         const int64_t width = static_cast<int64_t>(cur.size()) / W;
+        const int T = width >= 65536 ? max_threads : 1;
         // min-in-state: the active vertex contained in the fewest states,
         // first in active-list order on ties
-        // per byte position, how many states hold each byte value; a
-        // vertex's count is the sum over the values with its bit set
-        std::fill(byte_hist.begin(), byte_hist.end(), 0);
-        for (int64_t i = 0; i < width; ++i) {
-            const uint64_t *s = cur.data() + i * W;
-            for (int w = 0; w < W; ++w) {
-                uint64_t bits = s[w];
-                int64_t *h = byte_hist.data() + static_cast<size_t>(w) * 8 * 256;
-                for (int b = 0; b < 8; ++b) {
-                    h[b * 256 + (bits & 0xFF)]++;
-                    bits >>= 8;
+        // per byte position, how many states hold each byte value (per
+        // thread, then summed); a vertex's count is the sum over the values
+        // with its bit set
+        std::fill(byte_hist.begin(), byte_hist.begin() + hist_len * T, 0);
+        parallel_ranges(width, T, [&](int64_t i0, int64_t i1, int t) {
+            int64_t *ht = byte_hist.data() + hist_len * t;
+            for (int64_t i = i0; i < i1; ++i) {
+                const uint64_t *s = cur.data() + i * W;
+                for (int w = 0; w < W; ++w) {
+                    uint64_t bits = s[w];
+                    int64_t *h = ht + static_cast<size_t>(w) * 8 * 256;
+                    for (int b = 0; b < 8; ++b) {
+                        h[b * 256 + (bits & 0xFF)]++;
+                        bits >>= 8;
+                    }
                 }
+            }
+        });
+        for (int t = 1; t < T; ++t) {
+            const int64_t *ht = byte_hist.data() + hist_len * t;
+            for (size_t k = 0; k < hist_len; ++k) {
+                byte_hist[k] += ht[k];
             }
         }
         for (int32_t v : active) {
@@ -362,25 +395,40 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
         const uint64_t vb = 1ULL << (vertex % 64);
         const uint64_t *cm = compl_mask.data() + static_cast<size_t>(vertex) * W;
         // child states in state order (0-child, then the 1-child when the
-        // vertex is eligible), inserted in that order
-        child_keys.clear();
-        child_slot.clear();
-        for (int64_t i = 0; i < width; ++i) {
-            const uint64_t *s = cur.data() + i * W;
-            const bool was_set = (s[vw] & vb) != 0;
-            const size_t at = child_keys.size();
-            child_keys.insert(child_keys.end(), s, s + W);
-            child_keys[at + vw] &= ~vb;
-            child_slot.push_back(2 * i);
-            if (was_set) {
-                const size_t at1 = child_keys.size();
-                child_keys.resize(at1 + W);
-                for (int w = 0; w < W; ++w) {
-                    child_keys[at1 + w] = child_keys[at + w] & cm[w];
-                }
-                child_slot.push_back(2 * i + 1);
+        // vertex is eligible), inserted in that order; each thread writes
+        // the children of its states at the offset its predecessors leave
+        std::fill(child_base.begin(), child_base.end(), 0);
+        parallel_ranges(width, T, [&](int64_t i0, int64_t i1, int t) {
+            int64_t c = 0;
+            for (int64_t i = i0; i < i1; ++i) {
+                c += 1 + ((cur[static_cast<size_t>(i) * W + vw] & vb) != 0 ? 1 : 0);
             }
+            child_base[t + 1] = c;
+        });
+        for (int t = 0; t < T; ++t) {
+            child_base[t + 1] += child_base[t];
         }
+        const int64_t n_children = child_base[T];
+        child_keys.resize(static_cast<size_t>(n_children) * W);
+        child_slot.resize(static_cast<size_t>(n_children));
+        parallel_ranges(width, T, [&](int64_t i0, int64_t i1, int t) {
+            int64_t at = child_base[t];
+            for (int64_t i = i0; i < i1; ++i) {
+                const uint64_t *s = cur.data() + i * W;
+                const bool was_set = (s[vw] & vb) != 0;
+                uint64_t *k0 = child_keys.data() + at * W;
+                std::copy(s, s + W, k0);
+                k0[vw] &= ~vb;
+                child_slot[at++] = 2 * i;
+                if (was_set) {
+                    uint64_t *k1 = child_keys.data() + at * W;
+                    for (int w = 0; w < W; ++w) {
+                        k1[w] = k0[w] & cm[w];
+                    }
+                    child_slot[at++] = 2 * i + 1;
+                }
+            }
+        });
         child_ids.resize(child_slot.size());
         next_map.insert_many(child_keys.data(), static_cast<int64_t>(child_slot.size()),
                              child_ids.data());
@@ -416,24 +464,27 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
     }
     const int64_t total = out->layer_off[L];
     out->in_off.assign(total + 1, 0);
-    // every arc once, so the arc arrays are sized once
-    int64_t total_arcs = 0;
+    // arcs into each layer, so every layer knows where its arcs start and the
+    // arc arrays are sized once
+    std::vector<int64_t> layer_arc0(L + 1, 0);
     for (int l = 1; l < L; ++l) {
+        int64_t a = 0;
         for (const std::array<int32_t, 2> &c : bdd.child[l - 1]) {
-            total_arcs += (c[0] >= 0 ? 1 : 0) + (c[1] >= 0 ? 1 : 0);
+            a += (c[0] >= 0 ? 1 : 0) + (c[1] >= 0 ? 1 : 0);
         }
+        layer_arc0[l + 1] = layer_arc0[l] + a;
     }
+    const int64_t total_arcs = layer_arc0[L];
     out->arc_src.assign(total_arcs, 0);
     out->arc_val.assign(total_arcs, 0);
     out->arc_w.assign(total_arcs * d, 0);
 
-    // root: no in-arcs
-    int64_t arc = 0;
-    out->in_off[1] = 0;
-    for (int l = 1; l < L; ++l) {
+    // One layer's in-arcs: each head's 1-arcs then 0-arcs, parents in
+    // ascending index order within each group.  Writes in_off of the layer's
+    // own nodes and its own arc range only.
+    auto fill_layer = [&](int l) {
         const int64_t n_here = bdd.layer_size(l);
         const std::vector<std::array<int32_t, 2>> &parents = bdd.child[l - 1];
-        // in-degree per head and arc type
         std::vector<int64_t> cnt1(n_here, 0), cnt0(n_here, 0);
         for (size_t p = 0; p < parents.size(); ++p) {
             if (parents[p][1] >= 0) {
@@ -444,17 +495,15 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
             }
         }
         const int64_t base_node = out->layer_off[l];
-        std::vector<int64_t> pos1(n_here), pos0(n_here);
-        int64_t cursor = arc;
+        std::vector<int64_t> &pos1 = cnt1, &pos0 = cnt0; // counts become cursors
+        int64_t cursor = layer_arc0[l];
         for (int64_t v = 0; v < n_here; ++v) {
             out->in_off[base_node + v] = cursor;
+            const int64_t c1 = cnt1[v], c0 = cnt0[v];
             pos1[v] = cursor;
-            pos0[v] = cursor + cnt1[v];
-            cursor += cnt1[v] + cnt0[v];
+            pos0[v] = cursor + c1;
+            cursor += c1 + c0;
         }
-        out->in_off[base_node + n_here] = cursor;
-        const int64_t layer_arcs = cursor - arc;
-        // parents in ascending index order within each (head, type) group
         for (size_t p = 0; p < parents.size(); ++p) {
             for (int t = 1; t >= 0; --t) {
                 const int32_t h = parents[p][t];
@@ -470,8 +519,29 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
                 }
             }
         }
-        arc += layer_arcs;
+    };
+    // the root has no in-arcs; layers 1..L-1 on up to 8 host threads
+    const int threads = static_cast<int>(std::min<int64_t>(
+        {8, std::max<int64_t>(1, std::thread::hardware_concurrency()), std::max(L - 1, 1),
+         std::max<int64_t>(1, total_arcs / 200000)}));
+    std::atomic<int> next_layer{1};
+    auto worker = [&]() {
+        for (int l = next_layer++; l < L; l = next_layer++) {
+            fill_layer(l);
+        }
+    };
+    if (threads <= 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t) {
+            pool.emplace_back(worker);
+        }
+        for (std::thread &th : pool) {
+            th.join();
+        }
     }
+    out->in_off[total] = total_arcs;
     if (min_weight != nullptr) {
         out->min_weight.assign(total, 0);
         for (int l = 0; l < L; ++l) {
