@@ -231,3 +231,23 @@ def test_tsp15_5obj_seed13095_known_count(engine):
     assert len(a) == 117171
     b = engine.enumerate(dd, method=1).points
     assert _same(a, b)
+
+
+@pytest.mark.parametrize("family,name,kind,sd", [
+    ("mokp_n40_k7", "knapsack-40-7-1000-12144.dat", 1, 1),
+    ("mospp_n150_k7", "bp-150-30-7-10-14816.dat", 2, 0),
+    ("motsp_n15_k5", "tsp-nobj5-ncities15-seed11093.dat", 3, 0),
+])
+def test_paper_instances_methods_agree(engine, family, name, kind, sd):
+    """Instances of the reference paper's benchmark (too large for the
+    oracle): methods 1 and 3 (and 2, for the knapsack) give the same
+    frontier."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "paper", family, name)
+    print(f"case paper {family}/{name} sd={sd}", flush=True)
+    dd = DecisionDiagram.from_file(path, kind)
+    ref = engine.enumerate(dd, method=1, state_dominance=sd).points
+    assert len(ref) > 1000
+    methods = (2, 3) if kind == 1 else (3,)
+    for method in methods:
+        got = engine.enumerate(dd, method=method, state_dominance=sd).points
+        assert _same(got, ref), (name, method, len(got), len(ref))
