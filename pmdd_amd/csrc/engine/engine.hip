@@ -604,7 +604,9 @@ struct Launch {
 // that many of their workgroups fit one compute unit at once.
 constexpr int kSmallNode = 256;
 constexpr int kMidNode = 1024; // a middle tier where a full chunk holds 2048 points
-// threads of the chunk filter for chunks of 1024 or more candidates: their
+// Threads of the chunk filter: 128 for nodes of at most 256 candidates (most
+// hold far fewer, so half the waves of a 256-thread workgroup would idle);
+// for chunks of 1024 or more candidates: their
 // LDS allows two workgroups per CU, so more threads means more waves to hide
 // the LDS latency of the survivor scans (512 measured faster than 256 and 1024;
 // a step of 1024 candidates tests too many peers that die anyway)
@@ -627,7 +629,7 @@ template <int D> struct LaunchImpl {
     static void filter(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                        const void *src, int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns,
                        int32_t *na, unsigned long long *pt) {
-        constexpr int NT = C >= 1024 ? kFilterThreadsBig : pmdd_k::kBlock;
+        constexpr int NT = C >= 1024 ? kFilterThreadsBig : (C <= 256 ? 128 : pmdd_k::kBlock);
         hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, C, Src, PRE, NT>), g, dim3(NT), 0, s,
                            which, c, *static_cast<const Src *>(src), out, stride, alive, ns, na,
                            pt);
