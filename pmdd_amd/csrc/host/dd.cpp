@@ -274,6 +274,97 @@ template <class F> static void parallel_ranges(int64_t n, int T, F f) {
     }
 }
 
+// Distinct keys of keys[0 .. n) (W words each) in order of first occurrence:
+// ids[c] is the rank of key c's first occurrence among the distinct keys, the
+// same ids a map that hands out ids in insertion order gives, and uniq holds
+// the distinct keys in id order.  On T threads: each thread keeps the keys of
+// one hash partition in its own table, scanning the children in order.
+static void dedup_in_order(const uint64_t *keys, int64_t n, int W, int T,
+                           std::vector<int64_t> *ids, std::vector<uint64_t> *uniq) {
+    std::vector<uint64_t> h(static_cast<size_t>(n));
+    std::vector<int64_t> rep(static_cast<size_t>(n));
+    auto hash_key = [&](const uint64_t *k) {
+        uint64_t x = 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(W);
+        for (int i = 0; i < W; ++i) {
+            uint64_t z = k[i] + 0x9e3779b97f4a7c15ULL + (x << 6) + (x >> 2);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+            x ^= z ^ (z >> 31);
+        }
+        return x;
+    };
+    parallel_ranges(n, T, [&](int64_t c0, int64_t c1, int) {
+        for (int64_t c = c0; c < c1; ++c) {
+            h[c] = hash_key(keys + c * W);
+        }
+    });
+    // partition t: the children whose hash's high bits select t; its table
+    // stores the first child index of each distinct key
+    parallel_ranges(T, T, [&](int64_t, int64_t, int t) {
+        int64_t count = 0;
+        for (int64_t c = 0; c < n; ++c) {
+            count += static_cast<int64_t>((h[c] >> 40) % static_cast<uint64_t>(T)) == t ? 1 : 0;
+        }
+        size_t cap = 16;
+        while (static_cast<int64_t>(cap) < 2 * count) {
+            cap *= 2;
+        }
+        std::vector<int64_t> table(cap, -1);
+        const size_t mask = cap - 1;
+        for (int64_t c = 0; c < n; ++c) {
+            if (static_cast<int64_t>((h[c] >> 40) % static_cast<uint64_t>(T)) != t) {
+                continue;
+            }
+            size_t pos = static_cast<size_t>(h[c]) & mask;
+            while (true) {
+                const int64_t e = table[pos];
+                if (e < 0) {
+                    table[pos] = c;
+                    rep[c] = c;
+                    break;
+                }
+                if (h[e] == h[c] &&
+                    std::memcmp(keys + e * W, keys + c * W, sizeof(uint64_t) * W) == 0) {
+                    rep[c] = e;
+                    break;
+                }
+                pos = (pos + 1) & mask;
+            }
+        }
+    });
+    // ids of first occurrences by a prefix count, then every child's id
+    std::vector<int64_t> base(static_cast<size_t>(T) + 1, 0);
+    parallel_ranges(n, T, [&](int64_t c0, int64_t c1, int t) {
+        int64_t k = 0;
+        for (int64_t c = c0; c < c1; ++c) {
+            k += rep[c] == c ? 1 : 0;
+        }
+        base[t + 1] = k;
+    });
+    for (int t = 0; t < T; ++t) {
+        base[t + 1] += base[t];
+    }
+    ids->resize(static_cast<size_t>(n));
+    uniq->resize(static_cast<size_t>(base[T]) * W);
+    parallel_ranges(n, T, [&](int64_t c0, int64_t c1, int t) {
+        int64_t next = base[t];
+        for (int64_t c = c0; c < c1; ++c) {
+            if (rep[c] == c) {
+                (*ids)[c] = next;
+                std::copy(keys + c * W, keys + (c + 1) * W, uniq->data() + next * W);
+                ++next;
+            }
+        }
+    });
+    parallel_ranges(n, T, [&](int64_t c0, int64_t c1, int) {
+        for (int64_t c = c0; c < c1; ++c) {
+            if (rep[c] != c) {
+                (*ids)[c] = (*ids)[rep[c]];
+            }
+        }
+    });
+}
+
 BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
                                 std::vector<int32_t> *var_layer,
                                 std::vector<std::vector<int32_t>> *partners) {
@@ -429,15 +520,17 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
                 }
             }
         });
-        child_ids.resize(child_slot.size());
-        next_map.insert_many(child_keys.data(), static_cast<int64_t>(child_slot.size()),
-                             child_ids.data());
+        if (T > 1) {
+            dedup_in_order(child_keys.data(), n_children, W, T, &child_ids, &cur);
+        } else {
+            child_ids.resize(child_slot.size());
+            next_map.insert_many(child_keys.data(), n_children, child_ids.data());
+            cur.assign(next_map.key(0), next_map.key(0) + next_map.size() * W);
+        }
         for (size_t c = 0; c < child_slot.size(); ++c) {
             bdd.child[l - 1][child_slot[c] / 2][child_slot[c] % 2] =
                 static_cast<int32_t>(child_ids[c]);
         }
-        const int64_t nn = next_map.size();
-        cur.assign(next_map.key(0), next_map.key(0) + nn * W);
         if (partners != nullptr) {
             (*partners)[l] = setpacking_partners(cur, W, n, bdd.child[l - 1]);
         }
