@@ -165,6 +165,22 @@ std::vector<std::vector<int32_t>> knapsack_min_weights(const BinaryDD &bdd,
 // Set packing as an independent-set BDD
 // ---------------------------------------------------------------------------
 
+// Runs f(begin, end, t) over T contiguous ranges of [0, n) on T host threads
+// (T = 1: on the calling thread).
+template <class F> static void parallel_ranges(int64_t n, int T, F f) {
+    if (T <= 1) {
+        f(int64_t(0), n, 0);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; ++t) {
+        pool.emplace_back([&, t]() { f(n * t / T, n * (t + 1) / T, t); });
+    }
+    for (std::thread &th : pool) {
+        th.join();
+    }
+}
+
 // Set-packing state dominance (filter_dominance_setpacking_cpu,
 // dominance.cpp:88-173): the nodes of a layer are bucketed by the size of
 // their state (eligible vertices); a node in a bucket of more than one node
@@ -213,6 +229,11 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
         first[i] = f;
         bucket[c].push_back(static_cast<int32_t>(i));
     }
+    // per bucket with more than one node: the next non-empty bucket's nodes
+    // by smallest vertex, in bucket order; then each node's search, on host
+    // threads when there are many (every node's partner is its own)
+    std::vector<std::unordered_map<int32_t, std::vector<int32_t>>> next_by_first;
+    std::vector<std::pair<int32_t, int32_t>> tasks; // node, index into next_by_first
     for (int b1 = 0; b1 + 1 <= n_vars; ++b1) {
         if (bucket[b1].size() <= 1) {
             continue;
@@ -224,12 +245,23 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
         if (b2 > n_vars) {
             continue;
         }
-        // the next bucket's nodes by smallest vertex, in bucket order
-        std::unordered_map<int32_t, std::vector<int32_t>> by_first;
+        next_by_first.emplace_back();
         for (int32_t j : bucket[b2]) {
-            by_first[first[j]].push_back(j);
+            next_by_first.back()[first[j]].push_back(j);
         }
         for (int32_t i : bucket[b1]) {
+            tasks.emplace_back(i, static_cast<int32_t>(next_by_first.size() - 1));
+        }
+    }
+    const int64_t n_tasks = static_cast<int64_t>(tasks.size());
+    const int T = n_tasks >= 65536 ? static_cast<int>(std::min<unsigned>(
+                                         8, std::max(1u, std::thread::hardware_concurrency())))
+                                   : 1;
+    parallel_ranges(n_tasks, T, [&](int64_t k0, int64_t k1, int) {
+        for (int64_t k = k0; k < k1; ++k) {
+            const int32_t i = tasks[k].first;
+            const std::unordered_map<int32_t, std::vector<int32_t>> &by_first =
+                next_by_first[tasks[k].second];
             auto it = by_first.find(first[i]);
             if (it == by_first.end()) {
                 continue;
@@ -254,24 +286,8 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
                 }
             }
         }
-    }
+    });
     return partner;
-}
-
-// Runs f(begin, end, t) over T contiguous ranges of [0, n) on T host threads
-// (T = 1: on the calling thread).
-template <class F> static void parallel_ranges(int64_t n, int T, F f) {
-    if (T <= 1) {
-        f(int64_t(0), n, 0);
-        return;
-    }
-    std::vector<std::thread> pool;
-    for (int t = 0; t < T; ++t) {
-        pool.emplace_back([&, t]() { f(n * t / T, n * (t + 1) / T, t); });
-    }
-    for (std::thread &th : pool) {
-        th.join();
-    }
 }
 
 // Distinct keys of keys[0 .. n) (W words each) in order of first occurrence:
