@@ -413,7 +413,10 @@ __global__ __launch_bounds__(NT) void k_filter_chunks(
     }
 }
 
-// Block-wide sum of one int per thread (one LDS slot per wave).
+// Block-wide sum of one int per thread (one LDS slot per wave).  Ends with a
+// barrier after every wave has read the slots, so wave_tot may be written
+// again at once (block_rank writes its slot before its own barrier: without
+// this one, a wave still summing could read another wave's new rank count).
 __device__ __forceinline__ int block_sum(int v, int *wave_tot) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -431,6 +434,7 @@ __device__ __forceinline__ int block_sum(int v, int *wave_tot) {
     for (int w = 0; w < kWaves; ++w) {
         sum += wave_tot[w];
     }
+    __syncthreads();
     return sum;
 }
 
