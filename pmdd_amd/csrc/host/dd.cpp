@@ -293,12 +293,39 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
 // Distinct keys of keys[0 .. n) (W words each) in order of first occurrence:
 // ids[c] is the rank of key c's first occurrence among the distinct keys, the
 // same ids a map that hands out ids in insertion order gives, and uniq holds
-// the distinct keys in id order.  On T threads: each thread keeps the keys of
-// one hash partition in its own table, scanning the children in order.
+// the distinct keys in id order.  On T threads: the children are bucketed by
+// hash partition (ascending within a bucket), and each thread keeps the keys
+// of one partition in its own table, taking its bucket in order.  Which
+// partition a key falls in does not change the ids: a key's first occurrence
+// is the first child of its bucket that carries it.
+// Working buffers of dedup_in_order, kept between the layers of one build so
+// that a layer neither allocates nor clears them (they only grow).
+struct DedupScratch {
+    std::vector<uint64_t> h;
+    std::vector<int64_t> rep, order;
+    std::vector<std::vector<uint64_t>> table; // one per partition
+};
+
 static void dedup_in_order(const uint64_t *keys, int64_t n, int W, int T,
-                           std::vector<int64_t> *ids, std::vector<uint64_t> *uniq) {
-    std::vector<uint64_t> h(static_cast<size_t>(n));
-    std::vector<int64_t> rep(static_cast<size_t>(n));
+                           std::vector<int64_t> *ids, std::vector<uint64_t> *uniq,
+                           DedupScratch *scratch) {
+    // A table slot holds a child index in the low kIdBits bits and the top
+    // bits of its hash above them, so most probes skip the key compare.
+    constexpr int kIdBits = 40;
+    constexpr uint64_t kIdMask = (uint64_t(1) << kIdBits) - 1;
+    constexpr uint64_t kEmpty = ~uint64_t(0);
+    constexpr size_t kAhead = 16;
+    if (scratch->h.size() < static_cast<size_t>(n)) {
+        scratch->h.resize(static_cast<size_t>(n));
+        scratch->rep.resize(static_cast<size_t>(n));
+        scratch->order.resize(static_cast<size_t>(n));
+    }
+    if (scratch->table.size() < static_cast<size_t>(T)) {
+        scratch->table.resize(static_cast<size_t>(T));
+    }
+    uint64_t *h = scratch->h.data();
+    int64_t *rep = scratch->rep.data();
+    int64_t *order = scratch->order.data();
     auto hash_key = [&](const uint64_t *k) {
         uint64_t x = 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(W);
         for (int i = 0; i < W; ++i) {
@@ -309,39 +336,72 @@ static void dedup_in_order(const uint64_t *keys, int64_t n, int W, int T,
         }
         return x;
     };
-    parallel_ranges(n, T, [&](int64_t c0, int64_t c1, int) {
+    // partition of a child: its hash's top bits scaled to [0, T)
+    auto part = [&](uint64_t x) {
+        return static_cast<int>(((x >> kIdBits) * static_cast<uint64_t>(T)) >> (64 - kIdBits));
+    };
+    // cnt[t * T + p]: children of thread t's range in partition p
+    std::vector<int64_t> cnt(static_cast<size_t>(T) * T, 0);
+    parallel_ranges(n, T, [&](int64_t c0, int64_t c1, int t) {
+        int64_t *ct = cnt.data() + static_cast<size_t>(t) * T;
         for (int64_t c = c0; c < c1; ++c) {
             h[c] = hash_key(keys + c * W);
+            ++ct[part(h[c])];
         }
     });
-    // partition t: the children whose hash's high bits select t; its table
-    // stores the first child index of each distinct key
-    parallel_ranges(T, T, [&](int64_t, int64_t, int t) {
-        int64_t count = 0;
-        for (int64_t c = 0; c < n; ++c) {
-            count += static_cast<int64_t>((h[c] >> 40) % static_cast<uint64_t>(T)) == t ? 1 : 0;
+    // bucket p holds its children in ascending order: the ranges of threads
+    // 0 .. T-1 one after another
+    std::vector<int64_t> at(static_cast<size_t>(T) * T), bucket(static_cast<size_t>(T) + 1, 0);
+    int64_t run = 0;
+    for (int p = 0; p < T; ++p) {
+        bucket[p] = run;
+        for (int t = 0; t < T; ++t) {
+            at[static_cast<size_t>(t) * T + p] = run;
+            run += cnt[static_cast<size_t>(t) * T + p];
         }
+    }
+    bucket[T] = run;
+    parallel_ranges(n, T, [&](int64_t c0, int64_t c1, int t) {
+        int64_t *a = at.data() + static_cast<size_t>(t) * T;
+        for (int64_t c = c0; c < c1; ++c) {
+            order[a[part(h[c])]++] = c;
+        }
+    });
+    // partition p: one table of the first child index of each distinct key
+    parallel_ranges(T, T, [&](int64_t, int64_t, int p) {
+        const int64_t *mine = order + bucket[p];
+        const size_t count = static_cast<size_t>(bucket[p + 1] - bucket[p]);
         size_t cap = 16;
-        while (static_cast<int64_t>(cap) < 2 * count) {
+        while (cap < 2 * count) {
             cap *= 2;
         }
-        std::vector<int64_t> table(cap, -1);
+        std::vector<uint64_t> &table = scratch->table[p];
+        if (table.size() < cap) {
+            table.resize(cap);
+        }
+        std::fill(table.begin(), table.begin() + static_cast<std::ptrdiff_t>(cap), kEmpty);
         const size_t mask = cap - 1;
-        for (int64_t c = 0; c < n; ++c) {
-            if (static_cast<int64_t>((h[c] >> 40) % static_cast<uint64_t>(T)) != t) {
-                continue;
+        for (size_t i = 0; i < std::min(count, kAhead); ++i) {
+            __builtin_prefetch(&table[static_cast<size_t>(h[mine[i]]) & mask]);
+        }
+        for (size_t i = 0; i < count; ++i) {
+            if (i + kAhead < count) {
+                __builtin_prefetch(&table[static_cast<size_t>(h[mine[i + kAhead]]) & mask]);
             }
+            const int64_t c = mine[i];
+            const uint64_t fp = h[c] >> kIdBits;
             size_t pos = static_cast<size_t>(h[c]) & mask;
             while (true) {
-                const int64_t e = table[pos];
-                if (e < 0) {
-                    table[pos] = c;
+                const uint64_t e = table[pos];
+                if (e == kEmpty) {
+                    table[pos] = (fp << kIdBits) | static_cast<uint64_t>(c);
                     rep[c] = c;
                     break;
                 }
-                if (h[e] == h[c] &&
-                    std::memcmp(keys + e * W, keys + c * W, sizeof(uint64_t) * W) == 0) {
-                    rep[c] = e;
+                const int64_t ec = static_cast<int64_t>(e & kIdMask);
+                if ((e >> kIdBits) == fp &&
+                    std::memcmp(keys + ec * W, keys + c * W, sizeof(uint64_t) * W) == 0) {
+                    rep[c] = ec;
                     break;
                 }
                 pos = (pos + 1) & mask;
@@ -441,6 +501,7 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
     WordKeyMap next_map(W);
     std::vector<uint64_t> child_keys;
     std::vector<int64_t> child_slot, child_ids, child_base(max_threads + 1);
+    DedupScratch dedup_scratch;
 
     for (int l = 1; l <= n; ++l) {
         const int64_t width = static_cast<int64_t>(cur.size()) / W;
@@ -537,7 +598,7 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
             }
         });
         if (T > 1) {
-            dedup_in_order(child_keys.data(), n_children, W, T, &child_ids, &cur);
+            dedup_in_order(child_keys.data(), n_children, W, T, &child_ids, &cur, &dedup_scratch);
         } else {
             child_ids.resize(child_slot.size());
             next_map.insert_many(child_keys.data(), n_children, child_ids.data());
@@ -572,21 +633,52 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
         out->layer_off[l + 1] = out->layer_off[l] + bdd.layer_size(l);
     }
     const int64_t total = out->layer_off[L];
-    out->in_off.assign(total + 1, 0);
+    // layers 1..L-1 (the root has no in-arcs), each on one of up to 8 host
+    // threads, taken in order
+    const int threads = static_cast<int>(std::min<int64_t>(
+        {8, std::max<int64_t>(1, std::thread::hardware_concurrency()), std::max(L - 1, 1),
+         std::max<int64_t>(1, total / 100000)}));
+    auto for_layers = [&](auto &&f) {
+        std::atomic<int> next_layer{1};
+        auto worker = [&]() {
+            for (int l = next_layer++; l < L; l = next_layer++) {
+                f(l);
+            }
+        };
+        if (threads <= 1) {
+            worker();
+            return;
+        }
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t) {
+            pool.emplace_back(worker);
+        }
+        for (std::thread &th : pool) {
+            th.join();
+        }
+    };
     // arcs into each layer, so every layer knows where its arcs start and the
     // arc arrays are sized once
     std::vector<int64_t> layer_arc0(L + 1, 0);
-    for (int l = 1; l < L; ++l) {
+    for_layers([&](int l) {
         int64_t a = 0;
         for (const std::array<int32_t, 2> &c : bdd.child[l - 1]) {
             a += (c[0] >= 0 ? 1 : 0) + (c[1] >= 0 ? 1 : 0);
         }
-        layer_arc0[l + 1] = layer_arc0[l] + a;
+        layer_arc0[l + 1] = a;
+    });
+    for (int l = 1; l < L; ++l) {
+        layer_arc0[l + 1] += layer_arc0[l];
     }
     const int64_t total_arcs = layer_arc0[L];
-    out->arc_src.assign(total_arcs, 0);
-    out->arc_val.assign(total_arcs, 0);
-    out->arc_w.assign(total_arcs * d, 0);
+    // sized without a fill: fill_layer writes every element (its layer's
+    // offsets and arcs, zero weights on 0-arcs), the root layer's offsets and
+    // the closing offset are set below
+    out->in_off.resize(total + 1);
+    out->arc_src.resize(total_arcs);
+    out->arc_val.resize(total_arcs);
+    out->arc_w.resize(total_arcs * d);
+    std::fill(out->in_off.begin(), out->in_off.begin() + out->layer_off[1], int64_t(0));
 
     // One layer's in-arcs: each head's 1-arcs then 0-arcs, parents in
     // ascending index order within each group.  Writes in_off of the layer's
@@ -625,31 +717,13 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
                 if (t == 1) {
                     std::copy(bdd.one_w[l - 1].begin(), bdd.one_w[l - 1].end(),
                               out->arc_w.begin() + a * d);
+                } else {
+                    std::fill_n(out->arc_w.begin() + a * d, d, 0);
                 }
             }
         }
     };
-    // the root has no in-arcs; layers 1..L-1 on up to 8 host threads
-    const int threads = static_cast<int>(std::min<int64_t>(
-        {8, std::max<int64_t>(1, std::thread::hardware_concurrency()), std::max(L - 1, 1),
-         std::max<int64_t>(1, total_arcs / 200000)}));
-    std::atomic<int> next_layer{1};
-    auto worker = [&]() {
-        for (int l = next_layer++; l < L; l = next_layer++) {
-            fill_layer(l);
-        }
-    };
-    if (threads <= 1) {
-        worker();
-    } else {
-        std::vector<std::thread> pool;
-        for (int t = 0; t < threads; ++t) {
-            pool.emplace_back(worker);
-        }
-        for (std::thread &th : pool) {
-            th.join();
-        }
-    }
+    for_layers(fill_layer);
     out->in_off[total] = total_arcs;
     if (min_weight != nullptr) {
         out->min_weight.assign(total, 0);

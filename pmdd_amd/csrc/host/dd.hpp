@@ -4,7 +4,9 @@
 
 #include <array>
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/pmdd_engine.h"
@@ -12,6 +14,24 @@
 #include "instances.hpp"
 
 namespace pmdd {
+
+// Allocator that leaves elements uninitialised when a vector grows without a
+// value (resize(n)), so that arrays of hundreds of MB sized on one thread and
+// filled on host threads are not first zeroed on that one thread.
+template <class T> struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() noexcept = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U> void construct(U *p) noexcept(noexcept(::new (static_cast<void *>(p)) U)) {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <class U, class... A> void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <class T> using RawVec = std::vector<T, NoInitAlloc<T>>;
 
 // Flattened diagram: owns the arrays a pmdd_dd_view points into.
 struct FlatDD {
@@ -21,10 +41,10 @@ struct FlatDD {
     int32_t root = 0;
     int32_t terminal = 0;
     std::vector<int64_t> layer_off;  // [L+1]
-    std::vector<int64_t> in_off;     // [total_nodes+1]
-    std::vector<int32_t> arc_src;    // [A]
-    std::vector<int32_t> arc_val;    // [A]
-    std::vector<int32_t> arc_w;      // [A*d]
+    RawVec<int64_t> in_off;          // [total_nodes+1]
+    RawVec<int32_t> arc_src;         // [A]
+    RawVec<int32_t> arc_val;         // [A]
+    RawVec<int32_t> arc_w;           // [A*d]
     std::vector<int32_t> min_weight; // [total_nodes] or empty
     std::vector<int32_t> sd_partner; // [total_nodes] or empty: set-packing dominance partner
     pmdd_dd_structure structure{};
