@@ -55,6 +55,9 @@ typedef struct pmdd_dd_structure {
  * expect_d == 0 takes d from the file.  flags: PMDD_BUILD_STATE_DOMINANCE
  * also records what state dominance needs that only the construction knows
  * (for set packing, each node's dominance partner, view.sd_partner).
+ * Wide set-packing layers and the flattening run on up to 8 host threads;
+ * the environment variable PMDD_HOST_THREADS, when set to a positive number,
+ * caps them (the diagram is the same on any number of threads).
  * Returns PMDD_OK or PMDD_E_*. */
 #define PMDD_BUILD_STATE_DOMINANCE 1
 int pmdd_dd_build_from_file(const char *path, int problem_type, int expect_d, int flags,

@@ -6,6 +6,7 @@
 #include <thread>
 #include <unordered_map>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 
 #include "word_map.hpp"
@@ -165,6 +166,20 @@ std::vector<std::vector<int32_t>> knapsack_min_weights(const BinaryDD &bdd,
 // Set packing as an independent-set BDD
 // ---------------------------------------------------------------------------
 
+// Host threads a constructor may use: the machine's hardware threads, at
+// most 8, and at most PMDD_HOST_THREADS when that is set to a positive
+// number (read on every build).  The diagrams do not depend on it.
+static int host_threads() {
+    int t = static_cast<int>(std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency())));
+    if (const char *e = std::getenv("PMDD_HOST_THREADS")) {
+        const int v = std::atoi(e);
+        if (v > 0) {
+            t = std::min(t, v);
+        }
+    }
+    return t;
+}
+
 // Runs f(begin, end, t) over T contiguous ranges of [0, n) on T host threads
 // (T = 1: on the calling thread).
 template <class F> static void parallel_ranges(int64_t n, int T, F f) {
@@ -254,9 +269,7 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
         }
     }
     const int64_t n_tasks = static_cast<int64_t>(tasks.size());
-    const int T = n_tasks >= 65536 ? static_cast<int>(std::min<unsigned>(
-                                         8, std::max(1u, std::thread::hardware_concurrency())))
-                                   : 1;
+    const int T = n_tasks >= 65536 ? host_threads() : 1;
     parallel_ranges(n_tasks, T, [&](int64_t k0, int64_t k1, int) {
         for (int64_t k = k0; k < k1; ++k) {
             const int32_t i = tasks[k].first;
@@ -495,8 +508,7 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
     std::vector<int64_t> counter(n, 0);
     const size_t hist_len = static_cast<size_t>(W) * 8 * 256;
     // wide layers are split over host threads (the node order is unchanged)
-    const int max_threads = static_cast<int>(
-        std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency())));
+    const int max_threads = host_threads();
     std::vector<int64_t> byte_hist(hist_len * max_threads, 0);
     WordKeyMap next_map(W);
     std::vector<uint64_t> child_keys;
@@ -633,11 +645,10 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
         out->layer_off[l + 1] = out->layer_off[l] + bdd.layer_size(l);
     }
     const int64_t total = out->layer_off[L];
-    // layers 1..L-1 (the root has no in-arcs), each on one of up to 8 host
+    // layers 1..L-1 (the root has no in-arcs), each on one of the host
     // threads, taken in order
     const int threads = static_cast<int>(std::min<int64_t>(
-        {8, std::max<int64_t>(1, std::thread::hardware_concurrency()), std::max(L - 1, 1),
-         std::max<int64_t>(1, total / 100000)}));
+        {host_threads(), std::max(L - 1, 1), std::max<int64_t>(1, total / 100000)}));
     auto for_layers = [&](auto &&f) {
         std::atomic<int> next_layer{1};
         auto worker = [&]() {

@@ -238,6 +238,47 @@ def test_setpacking_partners_have_every_completion(tmp_path):
     assert pairs > 0
 
 
+def _first_occurrence_ordered(a, num_layers):
+    """Child ids of every layer follow the order in which a scan of the
+    parents (ascending, 0-child then 1-child) first reaches them."""
+    lo, io = a["layer_off"], a["in_off"]
+    for l in range(1, num_layers):
+        n_par = int(lo[l] - lo[l - 1])
+        heads = np.repeat(np.arange(int(lo[l + 1] - lo[l])),
+                          np.diff(io[int(lo[l]):int(lo[l + 1]) + 1]))
+        e0, e1 = int(io[lo[l]]), int(io[lo[l + 1]])
+        # position of each arc in the scan: parent-major, 0-arc before 1-arc
+        pos = a["arc_src"][e0:e1].astype(np.int64) * 2 + a["arc_val"][e0:e1]
+        assert pos.max() < 2 * n_par
+        seq = heads[np.argsort(pos, kind="stable")]
+        first = np.unique(seq, return_index=True)[1]
+        if not np.array_equal(seq[np.sort(first)], np.arange(len(first))):
+            return False
+    return True
+
+
+def test_setpacking_wide_layers_same_on_any_thread_count(tmp_path, monkeypatch):
+    """Layers of 65536+ states are built on host threads (histogram, child
+    generation, deduplication by hash partition, partner search) and the
+    diagram is flattened on threads; the arrays must equal the one-thread
+    build's, with node ids in first-occurrence order."""
+    inst = gen.setpacking(5, 130, 2, m=26)
+    path = gen.write(inst, str(tmp_path / "wide.dat"))
+    for sd in (False, True):
+        built = {}
+        for threads in ("1", "8"):
+            monkeypatch.setenv("PMDD_HOST_THREADS", threads)
+            dd = DecisionDiagram.from_file(path, 2, state_dominance=sd)
+            if threads == "1":
+                assert max(dd.layer_sizes()) >= 65536  # the threaded path is taken
+            built[threads] = dd.arrays()
+        one, many = built["1"], built["8"]
+        assert sorted(one) == sorted(many)
+        for k in one:
+            assert np.array_equal(one[k], many[k]), (sd, k)
+        assert _first_occurrence_ordered(many, len(many["layer_off"]) - 1)
+
+
 def test_setpacking_first_vertex_and_terminal(tmp_path):
     inst = gen.setpacking(3, 16, 3, m=4)
     path = gen.write(inst, str(tmp_path / "s.dat"))
