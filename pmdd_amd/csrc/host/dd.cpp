@@ -114,9 +114,9 @@ void reduce_bdd(BinaryDD *bdd) {
     std::vector<int32_t> below_map(1, 0); // terminal layer keeps its single node
     WordKeyMap pairs(1);
     for (int l = L - 2; l >= 0; --l) {
-        std::vector<std::array<int32_t, 2>> &layer = bdd->child[l];
+        ChildLayer &layer = bdd->child[l];
         std::vector<int32_t> this_map(layer.size(), -1);
-        std::vector<std::array<int32_t, 2>> kept;
+        ChildLayer kept;
         kept.reserve(layer.size());
         pairs.clear();
         for (size_t i = 0; i < layer.size(); ++i) {
@@ -211,7 +211,7 @@ static constexpr int kPartnerScan = 64;
 
 static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &states, int W,
                                                 int n_vars,
-                                                const std::vector<std::array<int32_t, 2>> &parents) {
+                                                const ChildLayer &parents) {
     const int64_t N = static_cast<int64_t>(states.size()) / W;
     std::vector<int32_t> partner(N, -1);
     if (N < 2) {
@@ -303,22 +303,44 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
     return partner;
 }
 
-// Distinct keys of keys[0 .. n) (W words each) in order of first occurrence:
-// ids[c] is the rank of key c's first occurrence among the distinct keys, the
-// same ids a map that hands out ids in insertion order gives, and uniq holds
-// the distinct keys in id order.  On T threads: the children are bucketed by
-// hash partition (ascending within a bucket), and each thread keeps the keys
-// of one partition in its own table, taking its bucket in order.  Which
-// partition a key falls in does not change the ids: a key's first occurrence
-// is the first child of its bucket that carries it.
+// Hash of a key of W words (SplitMix64 finaliser folded over the words).
+static inline uint64_t word_hash(const uint64_t *k, int W) {
+    uint64_t x = 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(W);
+    for (int i = 0; i < W; ++i) {
+        uint64_t z = k[i] + 0x9e3779b97f4a7c15ULL + (x << 6) + (x >> 2);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        x ^= z ^ (z >> 31);
+    }
+    return x;
+}
+
 // Working buffers of dedup_in_order, kept between the layers of one build so
 // that a layer neither allocates nor clears them (they only grow).
 struct DedupScratch {
     std::vector<uint64_t> h;
     std::vector<int64_t> rep, order;
     std::vector<std::vector<uint64_t>> table; // one per partition
+
+    void reserve(int64_t n, int T) {
+        if (h.size() < static_cast<size_t>(n)) {
+            h.resize(static_cast<size_t>(n));
+            rep.resize(static_cast<size_t>(n));
+            order.resize(static_cast<size_t>(n));
+        }
+        if (table.size() < static_cast<size_t>(T)) {
+            table.resize(static_cast<size_t>(T));
+        }
+    }
 };
 
+// Distinct keys of keys[0 .. n) (W words each) in order of first occurrence:
+// ids[c] is the rank of key c's first occurrence among the distinct keys, the
+// same ids a map that hands out ids in insertion order gives, and uniq holds
+// the distinct keys in id order.  On T threads: the children are bucketed by
+// hash partition (ascending within a bucket), and each thread keeps the keys
+// of one partition in its own table, taking its bucket in order.  Which partition a key falls in does not change the ids: a key's
+// first occurrence is the first child of its bucket that carries it.
 static void dedup_in_order(const uint64_t *keys, int64_t n, int W, int T,
                            std::vector<int64_t> *ids, std::vector<uint64_t> *uniq,
                            DedupScratch *scratch) {
@@ -328,27 +350,10 @@ static void dedup_in_order(const uint64_t *keys, int64_t n, int W, int T,
     constexpr uint64_t kIdMask = (uint64_t(1) << kIdBits) - 1;
     constexpr uint64_t kEmpty = ~uint64_t(0);
     constexpr size_t kAhead = 16;
-    if (scratch->h.size() < static_cast<size_t>(n)) {
-        scratch->h.resize(static_cast<size_t>(n));
-        scratch->rep.resize(static_cast<size_t>(n));
-        scratch->order.resize(static_cast<size_t>(n));
-    }
-    if (scratch->table.size() < static_cast<size_t>(T)) {
-        scratch->table.resize(static_cast<size_t>(T));
-    }
+    scratch->reserve(n, T);
     uint64_t *h = scratch->h.data();
     int64_t *rep = scratch->rep.data();
     int64_t *order = scratch->order.data();
-    auto hash_key = [&](const uint64_t *k) {
-        uint64_t x = 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(W);
-        for (int i = 0; i < W; ++i) {
-            uint64_t z = k[i] + 0x9e3779b97f4a7c15ULL + (x << 6) + (x >> 2);
-            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-            x ^= z ^ (z >> 31);
-        }
-        return x;
-    };
     // partition of a child: its hash's top bits scaled to [0, T)
     auto part = [&](uint64_t x) {
         return static_cast<int>(((x >> kIdBits) * static_cast<uint64_t>(T)) >> (64 - kIdBits));
@@ -358,7 +363,7 @@ static void dedup_in_order(const uint64_t *keys, int64_t n, int W, int T,
     parallel_ranges(n, T, [&](int64_t c0, int64_t c1, int t) {
         int64_t *ct = cnt.data() + static_cast<size_t>(t) * T;
         for (int64_t c = c0; c < c1; ++c) {
-            h[c] = hash_key(keys + c * W);
+            h[c] = word_hash(keys + c * W, W);
             ++ct[part(h[c])];
         }
     });
@@ -570,7 +575,6 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
         }
 
         next_map.clear();
-        bdd.child[l - 1].assign(width, {-1, -1});
         const int vw = vertex / 64;
         const uint64_t vb = 1ULL << (vertex % 64);
         const uint64_t *cm = compl_mask.data() + static_cast<size_t>(vertex) * W;
@@ -616,10 +620,18 @@ BinaryDD build_setpacking_exact(const SetPackingInstance &inst, int d,
             next_map.insert_many(child_keys.data(), n_children, child_ids.data());
             cur.assign(next_map.key(0), next_map.key(0) + next_map.size() * W);
         }
-        for (size_t c = 0; c < child_slot.size(); ++c) {
-            bdd.child[l - 1][child_slot[c] / 2][child_slot[c] % 2] =
-                static_cast<int32_t>(child_ids[c]);
-        }
+        // each parent's children: -1 where absent; the thread that generated
+        // a range of parents' children fills those parents
+        ChildLayer &layer = bdd.child[l - 1];
+        layer.resize(static_cast<size_t>(width));
+        parallel_ranges(width, T, [&](int64_t i0, int64_t i1, int t) {
+            for (int64_t i = i0; i < i1; ++i) {
+                layer[i] = {-1, -1};
+            }
+            for (int64_t c = child_base[t]; c < child_base[t + 1]; ++c) {
+                layer[child_slot[c] / 2][child_slot[c] % 2] = static_cast<int32_t>(child_ids[c]);
+            }
+        });
         if (partners != nullptr) {
             (*partners)[l] = setpacking_partners(cur, W, n, bdd.child[l - 1]);
         }
@@ -696,7 +708,7 @@ void flatten_bdd(const BinaryDD &bdd, const std::vector<std::vector<int32_t>> *m
     // own nodes and its own arc range only.
     auto fill_layer = [&](int l) {
         const int64_t n_here = bdd.layer_size(l);
-        const std::vector<std::array<int32_t, 2>> &parents = bdd.child[l - 1];
+        const ChildLayer &parents = bdd.child[l - 1];
         std::vector<int64_t> cnt1(n_here, 0), cnt0(n_here, 0);
         for (size_t p = 0; p < parents.size(); ++p) {
             if (parents[p][1] >= 0) {

@@ -59,11 +59,14 @@ struct FlatDD {
 // and every 1-arc leaving layer l carries the same weight vector one_w[l], as
 // in the knapsack (knapsack_bdd.cpp:36-41) and independent-set
 // (indepset_bdd.cpp:98-102) constructors.
+// One layer's children, [node] -> {0-child, 1-child}.
+using ChildLayer = RawVec<std::array<int32_t, 2>>;
+
 struct BinaryDD {
     int d = 0;
     int num_layers = 0;
-    std::vector<std::vector<std::array<int32_t, 2>>> child; // [L-1][node]
-    std::vector<std::vector<int32_t>> one_w;                // [L-1][d]
+    std::vector<ChildLayer> child;           // [L-1][node]
+    std::vector<std::vector<int32_t>> one_w; // [L-1][d]
 
     int64_t layer_size(int l) const {
         return l < num_layers - 1 ? static_cast<int64_t>(child[l].size()) : 1;
