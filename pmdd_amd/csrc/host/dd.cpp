@@ -4,7 +4,6 @@
 #include <algorithm>
 #include <atomic>
 #include <thread>
-#include <unordered_map>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -217,86 +216,119 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
     if (N < 2) {
         return partner;
     }
-    // single in-arc: its parent and arc type
-    std::vector<int32_t> indeg(N, 0), sp(N, -1), sa(N, -1);
-    for (size_t p = 0; p < parents.size(); ++p) {
-        for (int t = 0; t < 2; ++t) {
-            const int32_t c = parents[p][t];
-            if (c >= 0) {
-                indeg[c]++;
-                sp[c] = static_cast<int32_t>(p);
-                sa[c] = t;
+    const int T = N >= 65536 ? host_threads() : 1;
+    // single in-arc: its parent and arc type (sp, sa are read only where the
+    // in-degree is 1, so only one writer matters, and they need no fill)
+    std::vector<int32_t> indeg(N, 0);
+    RawVec<int32_t> sp(N), sa(N);
+    const int64_t P = static_cast<int64_t>(parents.size());
+    parallel_ranges(P, T, [&](int64_t p0, int64_t p1, int) {
+        for (int64_t p = p0; p < p1; ++p) {
+            for (int t = 0; t < 2; ++t) {
+                const int32_t c = parents[p][t];
+                if (c >= 0) {
+                    __atomic_fetch_add(&indeg[c], 1, __ATOMIC_RELAXED);
+                    __atomic_store_n(&sp[c], static_cast<int32_t>(p), __ATOMIC_RELAXED);
+                    __atomic_store_n(&sa[c], t, __ATOMIC_RELAXED);
+                }
             }
         }
-    }
-    std::vector<int32_t> count(N), first(N);
-    std::vector<std::vector<int32_t>> bucket(n_vars + 1);
-    for (int64_t i = 0; i < N; ++i) {
-        const uint64_t *s = states.data() + i * W;
-        int c = 0, f = -1;
-        for (int w = 0; w < W; ++w) {
-            c += __builtin_popcountll(s[w]);
-            if (f < 0 && s[w] != 0) {
-                f = w * 64 + __builtin_ctzll(s[w]);
+    });
+    // bucket of a node: its state size; the nodes sorted stably by (size,
+    // smallest vertex), so a bucket's nodes with one smallest vertex are a
+    // run in node order
+    const int64_t nf = n_vars + 1; // smallest vertex + 1 (0: empty state)
+    const int64_t nkeys = static_cast<int64_t>(n_vars + 1) * nf;
+    RawVec<int32_t> count(N), first(N);
+    std::vector<int64_t> hist(static_cast<size_t>(nkeys) * T, 0);
+    parallel_ranges(N, T, [&](int64_t i0, int64_t i1, int t) {
+        int64_t *ht = hist.data() + static_cast<size_t>(nkeys) * t;
+        for (int64_t i = i0; i < i1; ++i) {
+            const uint64_t *s = states.data() + i * W;
+            int c = 0, f = -1;
+            for (int w = 0; w < W; ++w) {
+                c += __builtin_popcountll(s[w]);
+                if (f < 0 && s[w] != 0) {
+                    f = w * 64 + __builtin_ctzll(s[w]);
+                }
             }
+            count[i] = c;
+            first[i] = f;
+            ++ht[c * nf + (f + 1)];
         }
-        count[i] = c;
-        first[i] = f;
-        bucket[c].push_back(static_cast<int32_t>(i));
-    }
-    // per bucket with more than one node: the next non-empty bucket's nodes
-    // by smallest vertex, in bucket order; then each node's search, on host
-    // threads when there are many (every node's partner is its own)
-    std::vector<std::unordered_map<int32_t, std::vector<int32_t>>> next_by_first;
-    std::vector<std::pair<int32_t, int32_t>> tasks; // node, index into next_by_first
-    for (int b1 = 0; b1 + 1 <= n_vars; ++b1) {
-        if (bucket[b1].size() <= 1) {
-            continue;
-        }
-        int b2 = b1 + 1;
-        while (b2 <= n_vars && bucket[b2].empty()) {
-            ++b2;
-        }
-        if (b2 > n_vars) {
-            continue;
-        }
-        next_by_first.emplace_back();
-        for (int32_t j : bucket[b2]) {
-            next_by_first.back()[first[j]].push_back(j);
-        }
-        for (int32_t i : bucket[b1]) {
-            tasks.emplace_back(i, static_cast<int32_t>(next_by_first.size() - 1));
+    });
+    // key_off[k]: start of key k in the sorted order; at[t][k]: where thread
+    // t's range writes its nodes of key k
+    std::vector<int64_t> key_off(static_cast<size_t>(nkeys) + 1, 0);
+    int64_t run = 0;
+    for (int64_t k = 0; k < nkeys; ++k) {
+        key_off[k] = run;
+        for (int t = 0; t < T; ++t) {
+            int64_t &h = hist[static_cast<size_t>(nkeys) * t + k];
+            const int64_t c = h;
+            h = run;
+            run += c;
         }
     }
-    const int64_t n_tasks = static_cast<int64_t>(tasks.size());
-    const int T = n_tasks >= 65536 ? host_threads() : 1;
-    parallel_ranges(n_tasks, T, [&](int64_t k0, int64_t k1, int) {
-        for (int64_t k = k0; k < k1; ++k) {
-            const int32_t i = tasks[k].first;
-            const std::unordered_map<int32_t, std::vector<int32_t>> &by_first =
-                next_by_first[tasks[k].second];
-            auto it = by_first.find(first[i]);
-            if (it == by_first.end()) {
+    key_off[nkeys] = run;
+    RawVec<int32_t> sorted(N);
+    parallel_ranges(N, T, [&](int64_t i0, int64_t i1, int t) {
+        int64_t *at = hist.data() + static_cast<size_t>(nkeys) * t;
+        for (int64_t i = i0; i < i1; ++i) {
+            sorted[at[count[i] * nf + (first[i] + 1)]++] = static_cast<int32_t>(i);
+        }
+    });
+    // next non-empty bucket after each bucket (n_vars + 1: none)
+    std::vector<int64_t> bucket_size(n_vars + 1);
+    for (int c = 0; c <= n_vars; ++c) {
+        bucket_size[c] = key_off[(c + 1) * nf] - key_off[c * nf];
+    }
+    std::vector<int> next_bucket(n_vars + 1, n_vars + 1);
+    for (int c = n_vars - 1, nb = n_vars + 1; c >= 0; --c) {
+        if (bucket_size[c + 1] > 0) {
+            nb = c + 1;
+        }
+        next_bucket[c] = nb;
+    }
+    // every node's search is its own: a node of a bucket with more than one
+    // node takes the first of (at most kPartnerScan) nodes of the next
+    // non-empty bucket with its smallest vertex, in node order, whose state
+    // contains its own.  Nodes are taken in the sorted order, so the nodes of
+    // one key scan the same run of candidates one after another, and states
+    // are read from a copy in that order.
+    RawVec<uint64_t> ss(static_cast<size_t>(N) * W);
+    parallel_ranges(N, T, [&](int64_t q0, int64_t q1, int) {
+        for (int64_t q = q0; q < q1; ++q) {
+            const uint64_t *s = states.data() + static_cast<int64_t>(sorted[q]) * W;
+            std::copy(s, s + W, ss.data() + q * W);
+        }
+    });
+    parallel_ranges(N, T, [&](int64_t q0, int64_t q1, int) {
+        for (int64_t q = q0; q < q1; ++q) {
+            const int32_t i = sorted[q];
+            const int c = count[i];
+            if (bucket_size[c] <= 1 || next_bucket[c] > n_vars) {
                 continue;
             }
-            const uint64_t *si = states.data() + static_cast<int64_t>(i) * W;
-            int scanned = 0;
-            for (int32_t j : it->second) {
-                if (++scanned > kPartnerScan) {
-                    break;
-                }
-                if (indeg[i] == 1 && indeg[j] == 1 && sp[i] == sp[j] && sa[i] != sa[j]) {
-                    continue;
-                }
-                const uint64_t *sj = states.data() + static_cast<int64_t>(j) * W;
+            const int64_t k = static_cast<int64_t>(next_bucket[c]) * nf + (first[i] + 1);
+            const uint64_t *si = ss.data() + q * W;
+            const int64_t r1 = std::min<int64_t>(key_off[k + 1], key_off[k] + kPartnerScan);
+            for (int64_t r = key_off[k]; r < r1; ++r) {
+                const uint64_t *sj = ss.data() + r * W;
                 bool subset = true;
                 for (int w = 0; w < W && subset; ++w) {
                     subset = (si[w] & ~sj[w]) == 0;
                 }
-                if (subset) {
-                    partner[i] = j;
-                    break;
+                if (!subset) {
+                    continue;
                 }
+                // siblings of one parent through different arcs are skipped
+                const int32_t j = sorted[r];
+                if (indeg[i] == 1 && indeg[j] == 1 && sp[i] == sp[j] && sa[i] != sa[j]) {
+                    continue;
+                }
+                partner[i] = j;
+                break;
             }
         }
     });
