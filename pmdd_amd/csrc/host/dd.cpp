@@ -257,8 +257,8 @@ static std::vector<int32_t> setpacking_partners(const std::vector<uint64_t> &sta
             ++ht[c * nf + (f + 1)];
         }
     });
-    // key_off[k]: start of key k in the sorted order; at[t][k]: where thread
-    // t's range writes its nodes of key k
+    // key_off[k]: start of key k in the sorted order; hist[t][k] becomes
+    // where thread t's range writes its nodes of key k
     std::vector<int64_t> key_off(static_cast<size_t>(nkeys) + 1, 0);
     int64_t run = 0;
     for (int64_t k = 0; k < nkeys; ++k) {
@@ -371,8 +371,9 @@ struct DedupScratch {
 // same ids a map that hands out ids in insertion order gives, and uniq holds
 // the distinct keys in id order.  On T threads: the children are bucketed by
 // hash partition (ascending within a bucket), and each thread keeps the keys
-// of one partition in its own table, taking its bucket in order.  Which partition a key falls in does not change the ids: a key's
-// first occurrence is the first child of its bucket that carries it.
+// of one partition in its own table, taking its bucket in order.  Which
+// partition a key falls in does not change the ids: a key's first occurrence
+// is the first child of its bucket that carries it.
 static void dedup_in_order(const uint64_t *keys, int64_t n, int W, int T,
                            std::vector<int64_t> *ids, std::vector<uint64_t> *uniq,
                            DedupScratch *scratch) {
