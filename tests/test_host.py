@@ -266,16 +266,19 @@ def test_setpacking_wide_layers_same_on_any_thread_count(tmp_path, monkeypatch):
     path = gen.write(inst, str(tmp_path / "wide.dat"))
     for sd in (False, True):
         built = {}
-        for threads in ("1", "8"):
+        # "0", "-3" and "abc" are ignored: the machine's threads, at most 8
+        for threads in ("1", "8") + (("0", "-3", "abc") if not sd else ()):
             monkeypatch.setenv("PMDD_HOST_THREADS", threads)
             dd = DecisionDiagram.from_file(path, 2, state_dominance=sd)
             if threads == "1":
                 assert max(dd.layer_sizes()) >= 65536  # the threaded path is taken
             built[threads] = dd.arrays()
-        one, many = built["1"], built["8"]
-        assert sorted(one) == sorted(many)
-        for k in one:
-            assert np.array_equal(one[k], many[k]), (sd, k)
+        one = built["1"]
+        for threads, many in built.items():
+            assert sorted(one) == sorted(many)
+            for k in one:
+                assert np.array_equal(one[k], many[k]), (sd, threads, k)
+        many = built["8"]
         assert _first_occurrence_ordered(many, len(many["layer_off"]) - 1)
 
 
