@@ -182,7 +182,9 @@ __device__ __forceinline__ int64_t unpack_key(uint64_t w) {
 }
 
 // Block-wide exclusive prefix of a 0/1 flag; returns the rank and writes the
-// block total to *total.  Uses wave ballots and one LDS slot per wave.
+// block total to *total.  Uses wave ballots and one LDS slot per wave.  Each
+// wave writes its slot before the barrier, so a barrier must separate the
+// call from every earlier read of wave_tot (callers end each rank with one).
 template <int NW = kWaves>
 __device__ __forceinline__ int block_rank(bool flag, int *wave_tot, int *total) {
     const int lane = threadIdx.x & (kWave - 1);
