@@ -310,6 +310,20 @@ void validate_view(const pmdd_dd_view *v) {
             }
         }
     }
+    // A set-packing state-dominance partner is -1 or another node of the same
+    // layer (a node's own labels would all drop against themselves).
+    if (v->sd_partner != nullptr) {
+        for (int l = 0; l < L; ++l) {
+            const int64_t n = v->layer_off[l + 1] - v->layer_off[l];
+            const int32_t *pl = v->sd_partner + v->layer_off[l];
+            for (int64_t i = 0; i < n; ++i) {
+                if (pl[i] < -1 || pl[i] >= n || pl[i] == i) {
+                    fail(PMDD_E_INVALID,
+                         "state-dominance partner out of range in layer " + std::to_string(l));
+                }
+            }
+        }
+    }
     // Every path sum stays inside int32: per objective, the sum over layers of
     // the largest |arc weight| entering the layer bounds any partial or full
     // path sum (the reference adds unchecked, bottomup.cu:124, couple.cu:93).
@@ -2302,7 +2316,7 @@ namespace {
 struct ServerJob {
     int64_t ticket = 0;
     std::vector<int64_t> layer_off, in_off;
-    std::vector<int32_t> arc_src, arc_val, arc_w, min_weight;
+    std::vector<int32_t> arc_src, arc_val, arc_w, min_weight, sd_partner;
     pmdd_dd_view view{};
     pmdd_opts opts{};
     int64_t estimate = 0;
@@ -2443,6 +2457,9 @@ int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_op
         if (view->min_weight != nullptr) {
             job->min_weight.assign(view->min_weight, view->min_weight + N);
         }
+        if (view->sd_partner != nullptr) {
+            job->sd_partner.assign(view->sd_partner, view->sd_partner + N);
+        }
         job->view = *view;
         job->view.layer_off = job->layer_off.data();
         job->view.in_off = job->in_off.data();
@@ -2450,6 +2467,7 @@ int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_op
         job->view.arc_val = job->arc_val.data();
         job->view.arc_w = job->arc_w.data();
         job->view.min_weight = view->min_weight != nullptr ? job->min_weight.data() : nullptr;
+        job->view.sd_partner = view->sd_partner != nullptr ? job->sd_partner.data() : nullptr;
         job->opts = *opts;
         job->estimate = estimate_bytes(*view, *opts);
         std::lock_guard<std::mutex> lk(srv->mu);
