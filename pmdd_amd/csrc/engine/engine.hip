@@ -308,6 +308,10 @@ void validate_view(const pmdd_dd_view *v) {
             if (v->arc_src[a] < 0 || v->arc_src[a] >= prev_n) {
                 fail(PMDD_E_INVALID, "arc source out of range in layer " + std::to_string(l));
             }
+            // a BDD arc is a 0-arc or a 1-arc (the bottom-up CSR is grouped so)
+            if (v->kind == PMDD_DD_BDD && v->arc_val[a] != 0 && v->arc_val[a] != 1) {
+                fail(PMDD_E_INVALID, "BDD arc value other than 0 or 1 in layer " + std::to_string(l));
+            }
         }
     }
     // A set-packing state-dominance partner is -1 or another node of the same

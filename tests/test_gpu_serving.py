@@ -1,6 +1,5 @@
 """Serving: every request's frontier is the same alone and in the mix, and a
 rejected request leaves the others unaffected."""
-import ctypes as C
 import gc
 
 import numpy as np
@@ -114,36 +113,3 @@ def test_submit_copies_the_partner_array(engine, tmp_path):
             assert _same(srv.wait(t).points, ref)
         del churn
 
-
-class _PartnerView:
-    """A diagram's view with its partner array replaced (for the engine's
-    input checks); keeps the diagram and the array alive."""
-
-    def __init__(self, dd, partner):
-        self._dd = dd
-        self._partner = np.ascontiguousarray(partner, dtype=np.int32)
-        self._view = abi.DDView.from_buffer_copy(bytes(dd.view_ptr.contents))
-        self._view.sd_partner = self._partner.ctypes.data_as(C.POINTER(C.c_int32))
-        self.view_ptr = C.pointer(self._view)
-        self.problem_type = dd.problem_type
-        self.num_layers = dd.num_layers
-
-
-def test_partner_outside_its_layer_is_rejected(engine, tmp_path):
-    """A partner index past its layer, or a node named as its own partner, is
-    refused when the diagram is checked, before anything reaches the device;
-    the same partners unchanged give the diagram's own result."""
-    p = gen.write(gen.setpacking(21, 40, 2, m=8), str(tmp_path / "s.dat"))
-    dd = DecisionDiagram.from_file(p, 2, state_dominance=True)
-    a = dd.arrays()
-    lo, partner = a["layer_off"], a["sd_partner"]
-    want = engine.enumerate(dd, method=1, state_dominance=1).points
-    assert _same(engine.enumerate(_PartnerView(dd, partner), method=1,
-                                  state_dominance=1).points, want)
-    l = next(l for l in range(dd.num_layers) if lo[l + 1] - lo[l] >= 2)
-    for bad in (0, int(lo[l + 1] - lo[l])):  # node 0 of layer l: itself, one past the layer
-        q = partner.copy()
-        q[lo[l]] = bad
-        with pytest.raises(abi.PmddError) as e:
-            engine.enumerate(_PartnerView(dd, q), method=1, state_dominance=1)
-        assert e.value.code == abi.PMDD_E_INVALID and "partner" in e.value.message
