@@ -252,7 +252,10 @@ int pmdd_server_wait(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_
  * collect results in completion order while others submit. */
 int pmdd_server_wait_any(pmdd_server *srv, int64_t *ticket, pmdd_frontier *out,
                          pmdd_stats *stats, char *err, size_t errlen);
-/* Finishes the running requests, drops queued ones, joins the workers. */
+/* Finishes the running requests, drops queued ones, joins the workers and
+ * frees the frontiers nobody collected.  No other call on srv may be in
+ * progress (a thread blocked in wait or wait_any must have returned) or
+ * follow. */
 void pmdd_server_destroy(pmdd_server *srv);
 
 #ifdef __cplusplus
