@@ -78,8 +78,9 @@ void pmdd_dd_free(pmdd_dd *dd);
  * main() writes (ParetoFrontier::sort_lexicographic_ascending,
  * src/pmdd/enum/pareto_frontier.hpp:418-433, called at src/pmdd/main.cpp:403).
  * The engine returns frontiers unsorted, as the reference's GPU entry points
- * do. */
-void pmdd_frontier_sort(pmdd_frontier *fr);
+ * do.  Returns PMDD_OK, or PMDD_E_NOMEM when the host has no memory for the
+ * sort (the points are then left as they were). */
+int pmdd_frontier_sort(pmdd_frontier *fr);
 
 #ifdef __cplusplus
 }

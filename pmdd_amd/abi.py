@@ -188,7 +188,7 @@ def host_lib() -> C.CDLL:
         lib.pmdd_dd_free.argtypes = [C.c_void_p]
         lib.pmdd_dd_free.restype = None
         lib.pmdd_frontier_sort.argtypes = [C.POINTER(Frontier)]
-        lib.pmdd_frontier_sort.restype = None
+        lib.pmdd_frontier_sort.restype = C.c_int
         _host = lib
     return _host
 

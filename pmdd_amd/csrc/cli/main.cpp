@@ -143,7 +143,11 @@ int main(int argc, char *argv[]) {
 
     // lexicographically ascending, as written by the reference (main.cpp:403)
     const int d = fr.d;
-    pmdd_frontier_sort(&fr);
+    if (pmdd_frontier_sort(&fr) != PMDD_OK) {
+        std::cout << "Error - out of host memory sorting the frontier" << std::endl;
+        pmdd_frontier_free(&fr);
+        return 1;
+    }
     std::vector<int32_t> sorted(fr.points, fr.points + static_cast<size_t>(fr.count) * d);
     if (opts.save_frontier) {
         std::string save_error;

@@ -83,8 +83,9 @@ template <int D> void sort_rows(int32_t *p, int64_t n) {
 } // namespace
 
 // ParetoFrontier::sort_lexicographic_ascending (src/pmdd/enum/pareto_frontier.hpp:418-433),
-// the order main() writes (src/pmdd/main.cpp:403).
-extern "C" void pmdd_frontier_sort(pmdd_frontier *fr) {
+// the order main() writes (src/pmdd/main.cpp:403).  The points change only
+// once the sorted copy is complete.
+static void sort_frontier(pmdd_frontier *fr) {
     if (fr == nullptr || fr->points == nullptr || fr->count < 2) {
         return;
     }
@@ -116,4 +117,13 @@ extern "C" void pmdd_frontier_sort(pmdd_frontier *fr) {
         std::copy(pts + order[i] * d, pts + order[i] * d + d, sorted.begin() + i * d);
     }
     std::copy(sorted.begin(), sorted.end(), fr->points);
+}
+
+extern "C" int pmdd_frontier_sort(pmdd_frontier *fr) {
+    try {
+        sort_frontier(fr);
+        return PMDD_OK;
+    } catch (const std::bad_alloc &) {
+        return PMDD_E_NOMEM;
+    }
 }

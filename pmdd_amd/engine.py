@@ -40,7 +40,10 @@ def _take_frontier(lib, fr: abi.Frontier, sort: bool = False) -> np.ndarray:
     lexicographically ascending when sort is set (pmdd_frontier_sort)."""
     n, d = int(fr.count), int(fr.d)
     if sort and n > 1:
-        abi.host_lib().pmdd_frontier_sort(C.byref(fr))
+        rc = abi.host_lib().pmdd_frontier_sort(C.byref(fr))
+        if rc != abi.PMDD_OK:
+            lib.pmdd_frontier_free(C.byref(fr))
+            raise PmddError(rc, "host out of memory sorting the frontier")
     if n > 0:
         arr = np.ctypeslib.as_array(fr.points, shape=(n * d,)).copy().reshape(n, d)
     else:

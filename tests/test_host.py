@@ -400,7 +400,7 @@ def test_frontier_sort_matches_lexsort(d):
     fr.d = d
     fr.count = len(buf)
     fr.points = buf.ctypes.data_as(C.POINTER(C.c_int32))
-    abi.host_lib().pmdd_frontier_sort(C.byref(fr))
+    assert abi.host_lib().pmdd_frontier_sort(C.byref(fr)) == abi.PMDD_OK
     assert (buf == want).all()
 
 
