@@ -1,14 +1,18 @@
 // CLI argument rules and run outputs.
 //   parse_cli_args            src/pmdd/util/cli_parser.cpp:181-488
-//   write_frontier_gzip_csv   src/pmdd/util/output_utils.cpp:87-152 (zlib in process
-//                             instead of a gzip child; decompressed bytes are the same)
+//   write_frontier_gzip_csv   src/pmdd/util/output_utils.cpp:87-152 (zlib in process,
+//                             blocks of rows as gzip members on host threads, instead
+//                             of a gzip child; decompressed bytes are the same)
 //   write_stats_jsonl         src/pmdd/util/output_utils.cpp:154-300
 //   print_and_save_run_summary src/pmdd/util/output_utils.cpp:302-374
 #include "output.hpp"
 
 #include <zlib.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cctype>
+#include <charconv>
 #include <cerrno>
 #include <climits>
 #include <cstdio>
@@ -16,6 +20,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <thread>
 
 namespace pmdd_cli {
 
@@ -345,6 +350,11 @@ bool parse_cli_args(int argc, char **argv, Options *out, std::string *error) {
     return true;
 }
 
+// Rows are cut into blocks; each block is formatted and deflated (level 6,
+// gzip's default) as a gzip member of its own on a host thread, and the
+// members are written in order.  A gzip file of several members decompresses
+// to the concatenation of their data (RFC 1952, 2.2), so the decompressed
+// bytes are those of one stream over all rows.
 bool write_frontier_gzip_csv(const std::vector<int32_t> &points, int d, const std::string &path,
                              std::string *error) {
     if (path.empty()) {
@@ -353,39 +363,79 @@ bool write_frontier_gzip_csv(const std::vector<int32_t> &points, int d, const st
     if (d <= 0 || points.size() % static_cast<size_t>(d) != 0) {
         return set_err(error, "frontier has invalid dimension");
     }
-    gzFile f = gzopen(path.c_str(), "wb6");
+    constexpr size_t kBlockRows = 32768;
+    const size_t rows = points.size() / static_cast<size_t>(d);
+    const size_t blocks = std::max<size_t>(1, (rows + kBlockRows - 1) / kBlockRows);
+    std::vector<std::string> member(blocks);
+    std::vector<char> failed(blocks, 0);
+    auto compress_block = [&](size_t b) {
+        std::string text;
+        text.reserve(kBlockRows * static_cast<size_t>(d) * 8);
+        char num[16];
+        const size_t r1 = std::min(rows, (b + 1) * kBlockRows);
+        for (size_t r = b * kBlockRows; r < r1; ++r) {
+            for (int o = 0; o < d; ++o) {
+                if (o > 0) {
+                    text += ',';
+                }
+                const auto res = std::to_chars(num, num + sizeof(num),
+                                               points[r * static_cast<size_t>(d) + o]);
+                text.append(num, static_cast<size_t>(res.ptr - num));
+            }
+            text += '\n';
+        }
+        z_stream zs{};
+        if (deflateInit2(&zs, 6, Z_DEFLATED, 15 + 16, 8, Z_DEFAULT_STRATEGY) != Z_OK) {
+            failed[b] = 1;
+            return;
+        }
+        std::string &out = member[b];
+        out.resize(deflateBound(&zs, static_cast<uLong>(text.size())));
+        zs.next_in = reinterpret_cast<Bytef *>(text.data());
+        zs.avail_in = static_cast<uInt>(text.size());
+        zs.next_out = reinterpret_cast<Bytef *>(out.data());
+        zs.avail_out = static_cast<uInt>(out.size());
+        const int rc = deflate(&zs, Z_FINISH);
+        out.resize(zs.total_out);
+        deflateEnd(&zs);
+        if (rc != Z_STREAM_END) {
+            failed[b] = 1;
+        }
+    };
+    const size_t threads = std::min<size_t>(
+        {blocks, 8, std::max<size_t>(1, std::thread::hardware_concurrency())});
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        for (size_t b = next++; b < blocks; b = next++) {
+            compress_block(b);
+        }
+    };
+    if (threads <= 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < threads; ++t) {
+            pool.emplace_back(worker);
+        }
+        for (std::thread &th : pool) {
+            th.join();
+        }
+    }
+    for (char f : failed) {
+        if (f != 0) {
+            return set_err(error, "failed while compressing the frontier");
+        }
+    }
+    std::FILE *f = std::fopen(path.c_str(), "wb");
     if (f == nullptr) {
         return set_err(error, "could not open gzip output");
     }
-    std::string buf;
-    buf.reserve(1 << 20);
-    char num[16];
     bool ok = true;
-    for (size_t i = 0; i < points.size() && ok; i += d) {
-        for (int o = 0; o < d; ++o) {
-            if (o > 0) {
-                buf += ',';
-            }
-            const int n = std::snprintf(num, sizeof(num), "%d", points[i + o]);
-            buf.append(num, n);
-        }
-        buf += '\n';
-        if (buf.size() >= (1u << 20)) {
-            ok = gzwrite(f, buf.data(), static_cast<unsigned>(buf.size())) ==
-                 static_cast<int>(buf.size());
-            buf.clear();
-        }
+    for (const std::string &m : member) {
+        ok = ok && std::fwrite(m.data(), 1, m.size(), f) == m.size();
     }
-    if (ok && !buf.empty()) {
-        ok = gzwrite(f, buf.data(), static_cast<unsigned>(buf.size())) ==
-             static_cast<int>(buf.size());
-    }
-    const int rc = gzclose(f);
-    if (!ok) {
+    if (std::fclose(f) != 0 || !ok) {
         return set_err(error, "failed while writing compressed frontier");
-    }
-    if (rc != Z_OK) {
-        return set_err(error, "gzip stream did not close cleanly");
     }
     return true;
 }
