@@ -92,12 +92,16 @@ typedef struct pmdd_opts {
     int32_t state_dominance; /* 0 or 1 */
     int32_t count_pair_tests;/* 1 = count executed dominance pair tests (slower) */
     int64_t max_cand;        /* --max-cand, candidates per expansion pass (default 20M) */
-    int64_t max_prod;        /* --max-prod, products of the first coupling batches
-                              * (default 625K); later batches grow up to 32M */
+    int64_t max_prod;        /* --max-prod: > 0 caps the products of every coupling batch
+                              * (couple.cu:393-403; a node with more products than the cap
+                              * is split over several batches, so no batch exceeds it);
+                              * PMDD_MAX_PROD_ADAPTIVE (0): batches start at 32K products
+                              * and double up to 32M */
 } pmdd_opts;
 
 #define PMDD_DEFAULT_MAX_CAND 20000000LL
 #define PMDD_DEFAULT_MAX_PROD 625000LL
+#define PMDD_MAX_PROD_ADAPTIVE 0LL
 
 /* Result: count points of d int32 objectives, row-major (point i at
  * points[i*d .. i*d+d-1]).  Allocated with malloc by the engine; release with
@@ -154,6 +158,27 @@ typedef struct pmdd_stats {
     double kernel_cross_s;
     int64_t cross_launches;
     int64_t cross_bytes;
+    /* Executed pair tests per kernel family (0 unless opts.count_pair_tests):
+     * the chunk filter, the frontier check (both modes), the cross filter,
+     * state dominance.  Their sum is dominance_pair_tests. */
+    int64_t pair_tests_filter;
+    int64_t pair_tests_frontier_check;
+    int64_t pair_tests_cross;
+    int64_t pair_tests_state_dominance;
+    /* Coupling batches (method 3) and the most products any one held. */
+    int64_t join_batches;
+    int64_t join_batch_peak_products;
+    /* Device bytes this call held at its peak: the context's buffers (kept
+     * from earlier calls included) plus the diagram, counted exactly from the
+     * engine's own allocations (gpu_mem_peak_used_bytes is device-wide). */
+    int64_t ctx_mem_peak_bytes;
+    /* Serving only: the admission estimate of the request (bytes), the
+     * times it was retried alone after running out of device memory next to
+     * other requests, and seconds from submit to admission. */
+    int64_t mem_estimate_bytes;
+    int32_t nomem_retries;
+    int32_t reserved2;
+    double queue_wait_s;
     int32_t layer_coupling;
     int32_t std_capacity;
     double *std_candidates_per_layer;
@@ -257,6 +282,32 @@ int pmdd_server_wait_any(pmdd_server *srv, int64_t *ticket, pmdd_frontier *out,
  * progress (a thread blocked in wait or wait_any must have returned) or
  * follow. */
 void pmdd_server_destroy(pmdd_server *srv);
+
+/* Admission record of a server (no reference counterpart).  A slot reserves
+ * its request's estimate while it runs and the device buffers its context
+ * keeps while it is idle; the head of the queue is admitted when its
+ * estimate fits the budget next to every other slot's reservation (or
+ * nothing runs).  When it does not fit, idle slots release their kept
+ * buffers.  A request that runs out of device memory while others run is
+ * queued again at the head and runs alone. */
+typedef struct pmdd_server_info {
+    int64_t budget;         /* bytes */
+    int64_t reserved_peak;  /* most bytes reserved at once */
+    int64_t admitted;       /* requests started (a retry counts again) */
+    int64_t rejected;       /* requests refused at submit (estimate above the budget) */
+    int64_t memory_waits;   /* times the head of the queue waited for memory with a slot free */
+    int64_t trims;          /* idle slots that released their kept buffers for the head */
+    int64_t nomem_retries;  /* requests queued again to run alone after PMDD_E_NOMEM */
+    int32_t running_peak;   /* most requests running at once */
+    int32_t slots;
+} pmdd_server_info;
+
+int pmdd_server_get_info(pmdd_server *srv, pmdd_server_info *out);
+
+/* Testing hook (no reference counterpart): the nth device allocation the
+ * engine makes from now on, in any context, fails as out of memory before
+ * any kernel uses the buffer; once.  nth <= 0 disarms it. */
+void pmdd_debug_fail_alloc(int64_t nth);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,7 @@ PMDD_DD_MDD = 1
 
 DEFAULT_MAX_CAND = 20_000_000
 DEFAULT_MAX_PROD = 625_000
+MAX_PROD_ADAPTIVE = 0
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -97,6 +98,17 @@ class Stats(C.Structure):
         ("kernel_cross_s", C.c_double),
         ("cross_launches", C.c_int64),
         ("cross_bytes", C.c_int64),
+        ("pair_tests_filter", C.c_int64),
+        ("pair_tests_frontier_check", C.c_int64),
+        ("pair_tests_cross", C.c_int64),
+        ("pair_tests_state_dominance", C.c_int64),
+        ("join_batches", C.c_int64),
+        ("join_batch_peak_products", C.c_int64),
+        ("ctx_mem_peak_bytes", C.c_int64),
+        ("mem_estimate_bytes", C.c_int64),
+        ("nomem_retries", C.c_int32),
+        ("reserved2", C.c_int32),
+        ("queue_wait_s", C.c_double),
         ("layer_coupling", C.c_int32),
         ("std_capacity", C.c_int32),
         ("std_candidates_per_layer", C.POINTER(C.c_double)),
@@ -109,6 +121,20 @@ class ServerConfig(C.Structure):
         ("device", C.c_int32),
         ("slots", C.c_int32),
         ("memory_budget", C.c_int64),
+    ]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [
+        ("budget", C.c_int64),
+        ("reserved_peak", C.c_int64),
+        ("admitted", C.c_int64),
+        ("rejected", C.c_int64),
+        ("memory_waits", C.c_int64),
+        ("trims", C.c_int64),
+        ("nomem_retries", C.c_int64),
+        ("running_peak", C.c_int32),
+        ("slots", C.c_int32),
     ]
 
 
@@ -150,6 +176,8 @@ ENGINE_SYMBOLS = [
     "pmdd_server_wait",
     "pmdd_server_wait_any",
     "pmdd_server_destroy",
+    "pmdd_server_get_info",
+    "pmdd_debug_fail_alloc",
 ]
 HOST_SYMBOLS = [
     "pmdd_dd_build_from_file",
@@ -250,6 +278,10 @@ def engine_lib() -> C.CDLL:
         lib.pmdd_server_wait_any.restype = C.c_int
         lib.pmdd_server_destroy.argtypes = [C.c_void_p]
         lib.pmdd_server_destroy.restype = None
+        lib.pmdd_server_get_info.argtypes = [C.c_void_p, C.POINTER(ServerInfo)]
+        lib.pmdd_server_get_info.restype = C.c_int
+        lib.pmdd_debug_fail_alloc.argtypes = [C.c_int64]
+        lib.pmdd_debug_fail_alloc.restype = None
         _engine = lib
     return _engine
 

@@ -169,7 +169,7 @@ void print_usage(const char *prog) {
     std::cout << "\t\t--max-cand <N>: GPU candidate batch cap; supports K, M, B suffixes "
                  "(default 20M)\n";
     std::cout << "\t\t--max-prod <N>: GPU coupled batch product cap; supports K, M, B suffixes "
-                 "(default 625K)\n";
+                 "(default 625K); auto: batches of 32K products doubling to 32M\n";
     std::cout << "\t\toptional arguments can be provided in any order\n";
     std::cout << std::endl;
 }
@@ -251,7 +251,11 @@ bool parse_cli_args(int argc, char **argv, Options *out, std::string *error) {
                                       "optionally using K, M, or B suffix.");
             }
             const std::string v(argv[++i]);
-            if (!parse_budget(v, &o.max_prod)) {
+            // "auto" (an extension): adaptive coupling batches, 32K products
+            // doubling to 32M; a number caps every batch
+            if (v == "auto") {
+                o.max_prod = 0;
+            } else if (!parse_budget(v, &o.max_prod)) {
                 return set_err(error, "Error - invalid --max-prod value '" + v +
                                           "' (expected positive integer with optional K, M, or "
                                           "B suffix).");

@@ -61,26 +61,58 @@ void hip_check(hipError_t e, const char *where) {
 
 // Device allocations made so far (device memory use changes only with them).
 static std::atomic<uint64_t> g_device_allocs{0};
+// Testing hook: allocations left before one fails as out of memory (0: off).
+static std::atomic<int64_t> g_fail_alloc_in{0};
+
+// Device bytes held by one context's buffers, and their peak since the last
+// reset.  A buffer is charged to the account that is current on the thread
+// when it is allocated (the context running a call on that thread).
+struct MemAcct {
+    std::atomic<int64_t> cur{0};
+    std::atomic<int64_t> peak{0};
+    void add(int64_t b) {
+        const int64_t now = cur.fetch_add(b, std::memory_order_relaxed) + b;
+        int64_t p = peak.load(std::memory_order_relaxed);
+        while (now > p && !peak.compare_exchange_weak(p, now, std::memory_order_relaxed)) {
+        }
+    }
+    void sub(int64_t b) { cur.fetch_sub(b, std::memory_order_relaxed); }
+    void reset_peak() { peak.store(cur.load(std::memory_order_relaxed), std::memory_order_relaxed); }
+};
+static thread_local MemAcct *t_acct = nullptr;
+
+// Makes `a` the current account of this thread for a scope.
+struct AcctScope {
+    MemAcct *prev;
+    explicit AcctScope(MemAcct *a) : prev(t_acct) { t_acct = a; }
+    ~AcctScope() { t_acct = prev; }
+    AcctScope(const AcctScope &) = delete;
+    AcctScope &operator=(const AcctScope &) = delete;
+};
 
 // Device buffer; grows on demand, contents not kept unless asked.
 
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    MemAcct *acct = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), acct(o.acct) {
         o.p = nullptr;
         o.bytes = 0;
+        o.acct = nullptr;
     }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             bytes = o.bytes;
+            acct = o.acct;
             o.p = nullptr;
             o.bytes = 0;
+            o.acct = nullptr;
         }
         return *this;
     }
@@ -88,9 +120,19 @@ struct DevBuf {
     void release() {
         if (p != nullptr) {
             (void)hipFree(p);
+            if (acct != nullptr) {
+                acct->sub(static_cast<int64_t>(bytes));
+            }
         }
         p = nullptr;
         bytes = 0;
+        acct = nullptr;
+    }
+    // Exchanges the allocations of two buffers (their accounts go with them).
+    void swap(DevBuf &o) noexcept {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+        std::swap(acct, o.acct);
     }
     // Grows by half again (with at most 64 MiB of slack), so a buffer sized
     // by a widening layer is not reallocated on every layer.
@@ -103,8 +145,16 @@ struct DevBuf {
             nb = std::max(nb, std::min(bytes + bytes / 2, need + (size_t(64) << 20)));
         }
         release();
+        if (g_fail_alloc_in.load(std::memory_order_relaxed) > 0 &&
+            g_fail_alloc_in.fetch_sub(1, std::memory_order_relaxed) == 1) {
+            fail(PMDD_E_NOMEM, std::string(what) + ": injected allocation failure");
+        }
         hip_check(hipMalloc(&p, nb), what);
         bytes = nb;
+        acct = t_acct;
+        if (acct != nullptr) {
+            acct->add(static_cast<int64_t>(nb));
+        }
         g_device_allocs.fetch_add(1, std::memory_order_relaxed);
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
@@ -199,6 +249,30 @@ struct pmdd_ctx {
     Staging staging;
     // frontier buffers handed from one enumeration to the next (FrontierLease)
     std::vector<DevFrontier> frontier_pool;
+    // device bytes held by this context's buffers
+    MemAcct acct;
+    bool in_call = false; // inside a one-shot pmdd_enumerate (its peak includes the upload)
+    // largest adaptive coupling batch (products); 0: kJoinBatchCap.  A server
+    // slot sets it so that a request's join stays within its admission
+    // estimate.
+    int64_t join_cap = 0;
+
+    // Releases every device buffer the context keeps between calls (the
+    // stream must be idle).
+    void trim() {
+        for (ChunkBufs *cb : {&cb_main, &cb_aux}) {
+            for (DevBuf *b : {&cb->chunks, &cb->nsurv, &cb->nalive, &cb->out}) {
+                b->release();
+            }
+        }
+        for (DevBuf *b : {&run_off, &which, &which_a, &which_b, &which_c, &which_s, &which_m,
+                          &surv_pts, &alive, &counter, &join_meta, &keep_a, &keep_b, &sort_a,
+                          &sort_b, &big_base, &big_node, &run_start, &grp_run, &join_idx, &join_x,
+                          &join_y, &join_ctr, &front_count, &merge_a, &merge_b}) {
+            b->release();
+        }
+        frontier_pool.clear();
+    }
 };
 
 // A frontier borrowed from the context's pool for one enumeration and handed
@@ -563,8 +637,7 @@ void grow_points(DevFrontier &f, int d, int64_t need, int64_t keep, hipStream_t 
                       "hipMemcpy(frontier grow)");
         }
     }
-    std::swap(f.pts.p, nb.p);
-    std::swap(f.pts.bytes, nb.bytes);
+    f.pts.swap(nb);
     f.cap = ncap;
     hip_check(hipStreamSynchronize(s), "frontier grow sync");
 }
@@ -600,7 +673,7 @@ struct Launch {
     void (*compact)(dim3, hipStream_t, const ChunkDesc *, const int32_t *, const int64_t *,
                     const int32_t *, int64_t, const uint8_t *, int32_t *, int64_t);
     void (*state_dom)(dim3, hipStream_t, const int32_t *, const int64_t *, const int32_t *,
-                      int64_t, uint8_t *);
+                      int64_t, uint8_t *, unsigned long long *);
     void (*compact_nodes)(dim3, hipStream_t, const int64_t *, const int32_t *, int64_t,
                           const uint8_t *, const int64_t *, int32_t *, int64_t);
     void (*gather_runs)(dim3, hipStream_t, const int64_t *, const int32_t *, int32_t,
@@ -640,6 +713,9 @@ constexpr int64_t kRefillWaves = 8192;
 constexpr int64_t kEvictBlock = 2048;
 // Largest coupling batch (products) once the first batches have built R.
 constexpr int64_t kJoinBatchCap = int64_t(1) << 25;
+// The same for requests of a server (adaptive batches): their admission
+// estimate covers a batch of at most this many products.
+constexpr int64_t kServeJoinCap = int64_t(1) << 19;
 
 template <int D> struct LaunchImpl {
     static constexpr int CH = pmdd_k::Cap<D>::CH;
@@ -673,9 +749,10 @@ template <int D> struct LaunchImpl {
                            ss, alive, dst, ds);
     }
     static void state_dom(dim3 g, hipStream_t s, const int32_t *t, const int64_t *off,
-                          const int32_t *pts, int64_t stride, uint8_t *keep) {
+                          const int32_t *pts, int64_t stride, uint8_t *keep,
+                          unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_state_dominance<D>), g, dim3(kBlock), 0, s, t, off, pts,
-                           stride, keep);
+                           stride, keep, pt);
     }
     static void compact_nodes(dim3 g, hipStream_t s, const int64_t *oi, const int32_t *src,
                               int64_t ss, const uint8_t *keep, const int64_t *oo, int32_t *dst,
@@ -842,7 +919,10 @@ struct Run {
     int d;
     hipStream_t s;
     int64_t mem_base = 0;
-    unsigned long long *d_tests = nullptr;
+    // executed pair-test counters (count_pair_tests): chunk filter, frontier
+    // check, cross filter, state dominance; null when not counting
+    unsigned long long *d_tests = nullptr, *t_check = nullptr, *t_cross = nullptr,
+                       *t_sd = nullptr;
     FrontierLease sd_lease;
     DevFrontier &sd_spare; // state dominance output, swapped with the layer's frontier
     // PMDD_DEBUG_JOIN=1 prints per-batch coupling counts to stderr
@@ -854,17 +934,22 @@ struct Run {
         if (opts.max_cand <= 0) {
             opts.max_cand = PMDD_DEFAULT_MAX_CAND;
         }
-        if (opts.max_prod <= 0) {
-            opts.max_prod = PMDD_DEFAULT_MAX_PROD;
+        if (opts.max_prod < 0) {
+            opts.max_prod = PMDD_MAX_PROD_ADAPTIVE;
         }
         size_t fr = 0, tot = 0;
         hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
         mem_base = static_cast<int64_t>(tot) - static_cast<int64_t>(fr);
         st->gpu_mem_peak_reserved_bytes = mem_base;
         if (opts.count_pair_tests) {
-            ctx->counter.ensure(8, "hipMalloc(counter)");
+            ctx->counter.ensure(4 * 8, "hipMalloc(counter)");
             d_tests = ctx->counter.as<unsigned long long>();
-            hip_check(hipMemsetAsync(d_tests, 0, 8, s), "hipMemset(counter)");
+            t_check = d_tests + 1;
+            t_cross = d_tests + 2;
+            t_sd = d_tests + 3;
+            hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(1), dim3(64), 0, s,
+                               ctx->counter.as<uint8_t>(), int64_t(4 * 8), uint8_t(0));
+            hip_check(hipGetLastError(), "k_fill_u8");
         }
     }
 
@@ -1052,7 +1137,7 @@ struct Run {
             hip_check(hipEventRecord(pool_event(2 * launches), s), "hipEventRecord");
             K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + start[w], cb.chunks.as<ChunkDesc>(),
                     pts, stride, alive, cb.nsurv.as<int32_t>(), fv, fv_stride, fin, kCrossWave, G,
-                    d_tests);
+                    t_cross);
             hip_check(hipGetLastError(), "k_cross_chunks");
             hip_check(hipEventRecord(pool_event(2 * launches + 1), s), "hipEventRecord");
             K.append_front(dim3(static_cast<unsigned>(n)), s, wl + start[w],
@@ -1449,7 +1534,8 @@ struct Run {
                            dim3(256), 0, s, ctx->keep_a.as<uint8_t>(), total, uint8_t(1));
         h2d(ctx, ctx->which, targets, "hipMemcpy(dominance targets)");
         K.state_dom(dim3(static_cast<unsigned>(targets.size() / 4)), s, ctx->which.as<int32_t>(),
-                    f.off.as<int64_t>(), f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>());
+                    f.off.as<int64_t>(), f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>(),
+                    t_sd);
         hip_check(hipGetLastError(), "k_state_dominance");
         // kept per node
         ctx->join_meta.ensure(static_cast<size_t>(n) * 8, "hipMalloc(kept counts)");
@@ -1485,10 +1571,8 @@ struct Run {
                         sd_spare.off.as<int64_t>(), sd_spare.pts.as<int32_t>(), sd_spare.cap);
         hip_check(hipGetLastError(), "k_compact_nodes");
         std::swap(f.h_off, sd_spare.h_off);
-        std::swap(f.off.p, sd_spare.off.p);
-        std::swap(f.off.bytes, sd_spare.off.bytes);
-        std::swap(f.pts.p, sd_spare.pts.p);
-        std::swap(f.pts.bytes, sd_spare.pts.bytes);
+        f.off.swap(sd_spare.off);
+        f.pts.swap(sd_spare.pts);
         std::swap(f.cap, sd_spare.cap);
     }
 
@@ -1686,24 +1770,66 @@ struct Run {
         ChunkBufs &cb = ctx->cb_main;
         ChunkBufs &cr = ctx->cb_aux;
         const std::vector<int32_t> none;
-        // Batches start at min(max_prod, 32K) products and double up to
-        // max(max_prod, kJoinBatchCap): the first batches build R cheaply, and
+        // Batch sizes: with an explicit max_prod every batch holds at most
+        // max_prod products.  As in the reference (couple.cu:393-403) a batch
+        // takes whole nodes while they fit; a node with more products than
+        // the cap is split into pieces (runs of its TD rows times all its BU
+        // columns, or runs of columns of one row when one row exceeds the
+        // cap), each its own batch, so unlike the reference no batch exceeds
+        // it.  Adaptive (max_prod 0): batches start at 32K products and
+        // double up to kJoinBatchCap, so the first batches build R cheaply and
         // the later, large ones are mostly filtered by it in few launches.
-        int64_t batch_cap = std::min<int64_t>(opts.max_prod, int64_t(1) << 15);
-        const int64_t batch_limit = std::max<int64_t>(opts.max_prod, kJoinBatchCap);
+        // The result does not depend on the split.
+        const bool adaptive = opts.max_prod <= 0;
+        int64_t batch_cap = adaptive ? (int64_t(1) << 15) : opts.max_prod;
+        const int64_t batch_limit =
+            adaptive ? (ctx->join_cap > 0 ? ctx->join_cap : kJoinBatchCap) : opts.max_prod;
+        // next piece: node nodes[i], its TD rows from cur_r, BU columns from cur_c
+        int64_t cur_r = 0, cur_c = 0;
+        struct Piece {
+            int64_t v, r0, r1, c0, c1;
+        };
+        std::vector<Piece> pieces;
         while (i < nodes.size()) {
             int64_t bprod = 0;
-            size_t j = i;
-            while (j < nodes.size()) {
-                const int64_t p = td.size(nodes[j]) * bu.size(nodes[j]);
-                if (bprod + p > batch_cap && bprod > 0) {
-                    break;
+            pieces.clear();
+            while (i < nodes.size()) {
+                const int64_t v = nodes[i];
+                const int64_t T = td.size(v), Bn = bu.size(v);
+                if (cur_r == 0 && cur_c == 0 && T * Bn <= batch_cap - bprod) {
+                    pieces.push_back({v, 0, T, 0, Bn}); // the whole node
+                    bprod += T * Bn;
+                    ++i;
+                    continue;
                 }
-                bprod += p;
-                ++j;
+                if (bprod > 0) {
+                    break; // it does not fit next to the batch's nodes
+                }
+                if (cur_c == 0 && Bn <= batch_cap) {
+                    const int64_t rows = std::min(T - cur_r, batch_cap / Bn);
+                    pieces.push_back({v, cur_r, cur_r + rows, 0, Bn});
+                    bprod += rows * Bn;
+                    cur_r += rows;
+                } else {
+                    const int64_t cols = std::min(Bn - cur_c, batch_cap);
+                    pieces.push_back({v, cur_r, cur_r + 1, cur_c, cur_c + cols});
+                    bprod += cols;
+                    cur_c += cols;
+                    if (cur_c == Bn) {
+                        cur_c = 0;
+                        ++cur_r;
+                    }
+                }
+                if (cur_r == T) {
+                    cur_r = 0;
+                    ++i;
+                }
+                break; // a piece fills its batch
             }
             batch_cap = std::min<int64_t>(batch_limit, batch_cap * 2);
-            const int32_t Kn = static_cast<int32_t>(j - i);
+            st->join_batches += 1;
+            st->join_batch_peak_products = std::max(st->join_batch_peak_products, bprod);
+            const int32_t Kn = static_cast<int32_t>(pieces.size());
             std::vector<int64_t> meta(4 * static_cast<size_t>(Kn) + 1);
             std::vector<int64_t> idx;
             int64_t *prod_off = meta.data();
@@ -1713,20 +1839,19 @@ struct Run {
             prod_off[0] = 0;
             int64_t batch_inputs = 0;
             for (int32_t k = 0; k < Kn; ++k) {
-                const int64_t v = nodes[i + k];
+                const Piece &pc = pieces[k];
                 row_base[k] = static_cast<int64_t>(idx.size());
-                for (int64_t a = 0; a < td.size(v); ++a) {
-                    idx.push_back(td.h_off[v] + a);
+                for (int64_t a = pc.r0; a < pc.r1; ++a) {
+                    idx.push_back(td.h_off[pc.v] + a);
                 }
                 col_base[k] = static_cast<int64_t>(idx.size());
-                for (int64_t b = 0; b < bu.size(v); ++b) {
-                    idx.push_back(bu.h_off[v] + b);
+                for (int64_t b = pc.c0; b < pc.c1; ++b) {
+                    idx.push_back(bu.h_off[pc.v] + b);
                 }
-                col_size[k] = bu.size(v);
-                prod_off[k + 1] = prod_off[k] + td.size(v) * bu.size(v);
-                batch_inputs += td.size(v) + bu.size(v);
+                col_size[k] = pc.c1 - pc.c0;
+                prod_off[k + 1] = prod_off[k] + (pc.r1 - pc.r0) * (pc.c1 - pc.c0);
+                batch_inputs += (pc.r1 - pc.r0) + (pc.c1 - pc.c0);
             }
-            i = j;
             h2d(ctx, ctx->join_meta, meta, "hipMemcpy(batch table)");
             h2d(ctx, ctx->join_idx, idx, "hipMemcpy(batch rows)");
             pmdd_k::JoinSrc src;
@@ -1783,7 +1908,7 @@ struct Run {
                 hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
                 for (int b = 0; b < nbk; ++b) {
                     K.refill_vs(dim3(grid), s, in, bprod, dc + b, R.pts.as<int32_t>(), R.cap,
-                                bounds[b], bounds[b + 1], outb, bprod, dc + 1 + b, d_tests);
+                                bounds[b], bounds[b + 1], outb, bprod, dc + 1 + b, t_check);
                     hip_check(hipGetLastError(), "k_refill_vs_block");
                     std::swap(in, outb);
                 }
@@ -1971,7 +2096,7 @@ struct Run {
                 const int64_t hi = std::min<int64_t>(btotal, lo + kEvictBlock);
                 K.refill_flags(dim3(rgrid), s, R.pts.as<int32_t>(), R.cap,
                                ctx->join_ctr.as<unsigned long long>(), B.pts.as<int32_t>(), B.cap,
-                               lo, hi, ctx->keep_b.as<uint8_t>(), d_tests);
+                               lo, hi, ctx->keep_b.as<uint8_t>(), t_check);
                 hip_check(hipGetLastError(), "k_refill_vs_block");
                 // R and the block of B read once, R's flags read and written
                 st->frontier_check_launches += 1;
@@ -2018,8 +2143,7 @@ struct Run {
             add_time(ctx->ev_a, ctx->ev_b, &st->kernel_expand_td_s);
             // R takes the merged buffer; its old buffer is kept for the next batch
             DevBuf &won = (merged == M.as<int32_t>()) ? M : Rn;
-            std::swap(R.pts.p, won.p);
-            std::swap(R.pts.bytes, won.bytes);
+            R.pts.swap(won);
             R.cap = total_new;
             r_count = total_new;
             sample_memory();
@@ -2032,10 +2156,15 @@ struct Run {
         hip_check(hipStreamSynchronize(s), "finish sync");
         flush_late();
         if (d_tests != nullptr) {
-            unsigned long long t = 0;
-            hip_check(hipMemcpyAsync(&t, d_tests, 8, hipMemcpyDeviceToHost, s), "hipMemcpy(counter)");
+            unsigned long long t[4] = {0, 0, 0, 0};
+            hip_check(hipMemcpyAsync(t, d_tests, sizeof(t), hipMemcpyDeviceToHost, s),
+                      "hipMemcpy(counter)");
             hip_check(hipStreamSynchronize(s), "counter sync");
-            st->dominance_pair_tests = static_cast<int64_t>(t);
+            st->pair_tests_filter = static_cast<int64_t>(t[0]);
+            st->pair_tests_frontier_check = static_cast<int64_t>(t[1]);
+            st->pair_tests_cross = static_cast<int64_t>(t[2]);
+            st->pair_tests_state_dominance = static_cast<int64_t>(t[3]);
+            st->dominance_pair_tests = static_cast<int64_t>(t[0] + t[1] + t[2] + t[3]);
         }
         sample_memory();
     }
@@ -2084,15 +2213,25 @@ int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmd
     out->d = dd->d;
     out->count = 0;
     out->points = nullptr;
-    Run run(ctx, dd, *opts, st);
-    if (opts->method == 1) {
-        run.topdown(out);
-    } else if (opts->method == 2) {
-        run.bottomup(out);
-    } else {
-        run.coupled(out);
+    // buffers allocated from here on are charged to the context
+    AcctScope scope(&ctx->acct);
+    if (!ctx->in_call) {
+        ctx->acct.reset_peak();
     }
-    run.finish();
+    {
+        Run run(ctx, dd, *opts, st);
+        if (opts->method == 1) {
+            run.topdown(out);
+        } else if (opts->method == 2) {
+            run.bottomup(out);
+        } else {
+            run.coupled(out);
+        }
+        run.finish();
+    }
+    // the diagram counts too when it was uploaded outside this call
+    const int64_t diagram = dd->arena.acct == &ctx->acct ? 0 : static_cast<int64_t>(dd->arena.bytes);
+    st->ctx_mem_peak_bytes = ctx->acct.peak.load(std::memory_order_relaxed) + diagram;
     return PMDD_OK;
 }
 
@@ -2173,23 +2312,7 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
     if (ctx->stream != nullptr) {
         (void)hipStreamSynchronize(ctx->stream);
     }
-    {
-        // release scratch before the stream
-        pmdd_ctx *c = ctx;
-        for (ChunkBufs *cb : {&c->cb_main, &c->cb_aux}) {
-            cb->chunks.release();
-            cb->nsurv.release();
-            cb->nalive.release();
-            cb->out.release();
-        }
-        for (DevBuf *b : {&c->run_off, &c->which, &c->which_a, &c->which_b, &c->which_c,
-                          &c->surv_pts, &c->alive, &c->counter, &c->join_meta, &c->keep_a,
-                          &c->keep_b, &c->sort_a, &c->sort_b, &c->big_base, &c->big_node,
-                          &c->run_start, &c->grp_run, &c->join_idx}) {
-            b->release();
-        }
-        c->frontier_pool.clear();
-    }
+    ctx->trim(); // release scratch before the stream
     if (ctx->ev_a != nullptr) {
         (void)hipEventDestroy(ctx->ev_a);
     }
@@ -2246,6 +2369,14 @@ int pmdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *view, const pmdd_opts *opt
         if (ctx == nullptr) {
             fail(PMDD_E_INVALID, "pmdd_enumerate: NULL context");
         }
+        // the diagram's device arena is charged to the context for this call
+        AcctScope scope(&ctx->acct);
+        ctx->acct.reset_peak();
+        struct InCall {
+            pmdd_ctx *c;
+            explicit InCall(pmdd_ctx *x) : c(x) { c->in_call = true; }
+            ~InCall() { c->in_call = false; }
+        } in_call(ctx);
         const double t0 = now_s();
         std::unique_ptr<pmdd_resident> r(upload(ctx, view));
         const double pack_s = now_s() - t0;
@@ -2324,6 +2455,10 @@ struct ServerJob {
     pmdd_dd_view view{};
     pmdd_opts opts{};
     int64_t estimate = 0;
+    bool exclusive = false; // runs alone (queued again after PMDD_E_NOMEM)
+    int retries = 0;
+    double submit_t = 0.0;
+    double admit_t = 0.0;
     bool done = false;
     int rc = PMDD_OK;
     std::string err;
@@ -2331,26 +2466,36 @@ struct ServerJob {
     pmdd_stats st{};
 };
 
-// Device bytes a request may claim: its resident diagram (both CSR
-// directions) plus two layer frontiers and one pass survivor buffer of at
-// most min(max_cand, 256 * widest layer) points of d int32 and a flag.
+// Device bytes a request may claim, from its layer widths and budgets: its
+// diagram (both CSR directions); the pass buffers of one expansion (survivor
+// points with their flags, and the two merge buffers of big nodes) and three
+// layer frontiers, each of at most min(max_cand, 256 * widest layer) points
+// of d int32; for method 3, a coupling batch's two product buffers and its
+// survivor, sort and merge buffers, at most five times the batch limit the
+// engine uses (max_prod, or kServeJoinCap products when adaptive: a server
+// slot caps the adaptive batches of its requests there); 16 MiB of tables.
+// The frontier term is a bound on the usual case only: frontier sizes are
+// not known before the run (the measured peak is ctx_mem_peak_bytes).
 int64_t estimate_bytes(const pmdd_dd_view &v, const pmdd_opts &o) {
     const int L = v.num_layers;
     const int64_t N = v.layer_off[L];
     const int64_t A = v.in_off[N];
+    const int64_t d = v.d;
     int64_t widest = 1;
     for (int l = 0; l < L; ++l) {
         widest = std::max<int64_t>(widest, v.layer_off[l + 1] - v.layer_off[l]);
     }
-    const int64_t arena = 2 * (8 * (N + L) + 4 * A + 4 * int64_t(v.d) * A);
+    const int64_t arena = 2 * (8 * (N + L) + 4 * A + 4 * d * A);
     const int64_t cap = o.max_cand > 0 ? o.max_cand : PMDD_DEFAULT_MAX_CAND;
     const int64_t pts = std::min<int64_t>(cap, 256 * widest);
+    const int64_t pass = pts * (4 * d + 1) + 2 * pts * 4 * d;
+    const int64_t fronts = 3 * pts * 4 * d;
     int64_t join = 0;
     if (o.method == 3) {
-        // the two product buffers of a coupling batch
-        join = 2 * std::min<int64_t>(kJoinBatchCap, 4 * pts) * 4 * int64_t(v.d);
+        const int64_t limit = o.max_prod > 0 ? o.max_prod : kServeJoinCap;
+        join = 5 * limit * 4 * d;
     }
-    return arena + 3 * pts * (4 * int64_t(v.d) + 1) + join + (int64_t(16) << 20);
+    return arena + pass + fronts + join + (int64_t(16) << 20);
 }
 
 } // namespace
@@ -2359,8 +2504,8 @@ struct pmdd_server {
     int device = 0;
     int slots = 1;
     int64_t budget = 0;
-    int64_t in_use = 0;
     int64_t next_ticket = 1;
+    int running = 0;
     bool stopping = false;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -2369,31 +2514,120 @@ struct pmdd_server {
     std::deque<int64_t> finished; // done, in completion order (may hold collected tickets)
     std::vector<pmdd_ctx *> ctxs;
     std::vector<std::thread> workers;
+    // per slot: bytes reserved (the running request's estimate, or the
+    // buffers the idle context keeps), busy, asked to release its buffers
+    std::vector<int64_t> reserve;
+    std::vector<char> busy, trim_asked;
+    int64_t blocked_ticket = 0; // head of the queue last seen waiting for memory
+    pmdd_server_info info{};
+
+    int64_t reserved_except(int slot) const {
+        int64_t r = 0;
+        for (int s = 0; s < slots; ++s) {
+            r += s == slot ? 0 : reserve[s];
+        }
+        return r;
+    }
+
+    // Whether idle `slot` may start the head of the queue now.
+    bool admits(int slot, const ServerJob &head) const {
+        const int64_t others = reserved_except(slot);
+        if (head.exclusive) {
+            return running == 0 && others == 0;
+        }
+        const int64_t need = std::max(head.estimate, reserve[slot]);
+        return running == 0 || others + need <= budget;
+    }
+
+    // Asks every idle slot that keeps device buffers to release them (the
+    // head of the queue is waiting for memory).
+    void ask_trims(int except) {
+        for (int s = 0; s < slots; ++s) {
+            if (s != except && !busy[s] && reserve[s] > 0 && !trim_asked[s]) {
+                trim_asked[s] = 1;
+            }
+        }
+    }
+
+    // Releases slot's kept buffers; called with the lock held, drops it
+    // while the device calls run.
+    void trim_slot(int slot, std::unique_lock<std::mutex> &lk) {
+        pmdd_ctx *ctx = ctxs[slot];
+        lk.unlock();
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->trim();
+        lk.lock();
+        reserve[slot] = ctx->acct.cur.load(std::memory_order_relaxed);
+        info.trims += 1;
+    }
 
     void worker(int slot) {
         pmdd_ctx *ctx = ctxs[slot];
         std::unique_lock<std::mutex> lk(mu);
         while (true) {
-            cv_work.wait(lk, [&] {
-                return stopping ||
-                       (!queue.empty() &&
-                        (in_use == 0 || queue.front()->estimate <= budget - in_use));
-            });
             if (stopping) {
                 return;
             }
+            if (trim_asked[slot]) {
+                trim_asked[slot] = 0;
+                trim_slot(slot, lk);
+                cv_work.notify_all();
+                continue;
+            }
+            if (queue.empty()) {
+                cv_work.wait(lk);
+                continue;
+            }
             std::shared_ptr<ServerJob> job = queue.front();
+            if (!admits(slot, *job)) {
+                if (blocked_ticket != job->ticket) {
+                    blocked_ticket = job->ticket;
+                    info.memory_waits += 1;
+                }
+                if (job->exclusive && reserve[slot] > 0) {
+                    trim_slot(slot, lk); // this slot's own kept buffers
+                }
+                ask_trims(slot);
+                cv_work.notify_all();
+                if (!admits(slot, *job)) {
+                    cv_work.wait(lk);
+                }
+                continue;
+            }
             queue.pop_front();
-            in_use += job->estimate;
+            busy[slot] = 1;
+            running += 1;
+            reserve[slot] = std::max(job->estimate, reserve[slot]);
+            info.admitted += 1;
+            info.running_peak = std::max(info.running_peak, running);
+            info.reserved_peak = std::max(info.reserved_peak, reserved_except(-1));
+            job->admit_t = now_s();
             lk.unlock();
             cv_work.notify_all();
             char err[1024] = {0};
             job->st.std_capacity = 0;
+            ctx->join_cap = kServeJoinCap;
             job->rc = pmdd_enumerate(ctx, &job->view, &job->opts, &job->fr, &job->st, err,
                                      sizeof(err));
             job->err = err;
             lk.lock();
-            in_use -= job->estimate;
+            busy[slot] = 0;
+            running -= 1;
+            reserve[slot] = ctx->acct.cur.load(std::memory_order_relaxed);
+            if (job->rc == PMDD_E_NOMEM && !job->exclusive && !stopping) {
+                // out of device memory next to other requests: once more, alone
+                pmdd_frontier_free(&job->fr);
+                job->exclusive = true;
+                job->retries += 1;
+                info.nomem_retries += 1;
+                queue.push_front(job);
+                cv_work.notify_all();
+                continue;
+            }
+            job->st.mem_estimate_bytes = job->estimate;
+            job->st.nomem_retries = job->retries;
+            job->st.queue_wait_s = job->admit_t - job->submit_t;
             job->done = true;
             finished.push_back(job->ticket);
             cv_done.notify_all();
@@ -2433,6 +2667,11 @@ int pmdd_server_create(const pmdd_server_config *cfg, pmdd_server **out, char *e
             hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
             srv->budget = static_cast<int64_t>(fr) / 5 * 4;
         }
+        srv->reserve.assign(cfg->slots, 0);
+        srv->busy.assign(cfg->slots, 0);
+        srv->trim_asked.assign(cfg->slots, 0);
+        srv->info.budget = srv->budget;
+        srv->info.slots = cfg->slots;
         pmdd_server *raw = srv.get();
         for (int i = 0; i < cfg->slots; ++i) {
             srv->workers.emplace_back([raw, i] { raw->worker(i); });
@@ -2450,6 +2689,7 @@ int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_op
         }
         validate_view(view);
         auto job = std::make_shared<ServerJob>();
+        job->submit_t = now_s();
         const int L = view->num_layers;
         const int64_t N = view->layer_off[L];
         const int64_t A = view->in_off[N];
@@ -2481,9 +2721,11 @@ int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_op
         if (job->estimate > srv->budget) {
             job->done = true;
             job->rc = PMDD_E_REJECTED;
+            job->st.mem_estimate_bytes = job->estimate;
             job->err = "request needs about " + std::to_string(job->estimate) +
                        " bytes of device memory, more than the server budget of " +
                        std::to_string(srv->budget);
+            srv->info.rejected += 1;
             srv->finished.push_back(job->ticket);
             srv->cv_done.notify_all();
             return PMDD_OK;
@@ -2581,6 +2823,15 @@ int pmdd_server_wait(pmdd_server *srv, int64_t ticket, pmdd_frontier *out, pmdd_
     return server_collect(srv, ticket, true, out, stats, err, errlen);
 }
 
+int pmdd_server_get_info(pmdd_server *srv, pmdd_server_info *out) {
+    if (srv == nullptr || out == nullptr) {
+        return PMDD_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(srv->mu);
+    *out = srv->info;
+    return PMDD_OK;
+}
+
 void pmdd_server_destroy(pmdd_server *srv) {
     if (srv == nullptr) {
         return;
@@ -2601,6 +2852,10 @@ void pmdd_server_destroy(pmdd_server *srv) {
         pmdd_ctx_destroy(c);
     }
     delete srv;
+}
+
+void pmdd_debug_fail_alloc(int64_t nth) {
+    g_fail_alloc_in.store(nth > 0 ? nth : 0, std::memory_order_relaxed);
 }
 
 } // extern "C"

@@ -182,15 +182,20 @@ __device__ __forceinline__ int64_t unpack_key(uint64_t w) {
 }
 
 // Block-wide exclusive prefix of a 0/1 flag; returns the rank and writes the
-// block total to *total.  Uses wave ballots and one LDS slot per wave.  Each
-// wave writes its slot before the barrier, so a barrier must separate the
-// call from every earlier read of wave_tot (callers end each rank with one).
+// block total to *total.  Uses wave ballots and one LDS slot per wave.  It
+// opens with a barrier, so no wave can overwrite its wave_tot slot while a
+// slower wave still reads the slots of an earlier block_rank or block_sum:
+// the call is safe after any other use of wave_tot.  Like every barrier, it
+// must be reached by all threads of the workgroup (every caller calls it in
+// workgroup-uniform control flow).  The barrier also orders the caller's
+// earlier LDS reads before whatever it writes with the returned rank.
 template <int NW = kWaves>
 __device__ __forceinline__ int block_rank(bool flag, int *wave_tot, int *total) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const uint64_t mask = __ballot(flag);
     const int in_wave = __popcll(mask & ((lane == 0) ? 0ULL : (~0ULL >> (kWave - lane))));
+    __syncthreads();
     if (lane == 0) {
         wave_tot[wave] = __popcll(mask);
     }
@@ -385,7 +390,8 @@ __global__ __launch_bounds__(NT) void k_filter_chunks(
         const int wave_end = __builtin_amdgcn_readfirstlane(
             min(tcount, static_cast<int>((tid | (kWave - 1)))));
         dead = scan_until_dead<D, DP>(S + nS * DP, wave_end, min(tid, tcount), m, dead, tests);
-        __syncthreads();
+        // block_rank's opening barrier ends every lane's reads of S before
+        // the survivors are written after it
         int total = 0;
         const int rank = block_rank<NT / kWave>(!dead, wave_tot, &total);
         if (!dead) {
@@ -415,10 +421,10 @@ __global__ __launch_bounds__(NT) void k_filter_chunks(
     }
 }
 
-// Block-wide sum of one int per thread (one LDS slot per wave).  Ends with a
-// barrier after every wave has read the slots, so wave_tot may be written
-// again at once (block_rank writes its slot before its own barrier: without
-// this one, a wave still summing could read another wave's new rank count).
+// Block-wide sum of one int per thread (one LDS slot per wave).  Like
+// block_rank it opens with a barrier before writing its slot; it also ends
+// with one after every wave has read the slots, which orders the caller's
+// earlier LDS reads (a tile scan) before whatever it writes next.
 __device__ __forceinline__ int block_sum(int v, int *wave_tot) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -804,9 +810,10 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_state_dominance(
     const int32_t *__restrict__ targets, // [n][4]: node, cmpA, cmpB (-1 = none), first label
     const int64_t *__restrict__ off, const int32_t *__restrict__ pts, int64_t stride,
-    uint8_t *__restrict__ keep) {
+    uint8_t *__restrict__ keep, unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
     __shared__ __align__(16) int32_t T[kBlock * DP];
+    unsigned long long tests = 0;
     const int32_t *tg = targets + 4 * static_cast<int64_t>(blockIdx.x);
     const int node = tg[0];
     const int64_t b0 = off[node];
@@ -847,12 +854,14 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
                 }
             }
             __syncthreads();
-            unsigned long long tests = 0;
             dead = scan_until_dead<D, DP>(T, tn, tn, m, dead, tests);
         }
     }
     if (valid && dead) {
         keep[b0 + i] = 0;
+    }
+    if (pair_tests != nullptr && tests != 0) {
+        atomicAdd(pair_tests, tests);
     }
 }
 

@@ -5,7 +5,8 @@ MultiobjEnum::pareto_frontier_{topdown,dynamic_layer_cutset}_cuda
 (src/pmdd/enum/multiobj_enum.cpp:72-124): it takes a decision diagram, a
 method (1 top-down, 2 bottom-up, 3 dynamic layer cutset), the problem type and state
 dominance flag, plus the --max-cand / --max-prod budgets, and returns the
-frontier.  Errors raise PmddError carrying the engine's code and message
+frontier.  max_prod defaults to adaptive coupling batches (0); a positive
+value caps every batch, as the reference's --max-prod.  Errors raise PmddError carrying the engine's code and message
 (the reference returns NULL with a reason string instead).
 """
 from __future__ import annotations
@@ -85,7 +86,7 @@ class Engine:
         return o
 
     def enumerate(self, dd, method: int = 1, state_dominance: int = 0,
-                  max_cand: int = abi.DEFAULT_MAX_CAND, max_prod: int = abi.DEFAULT_MAX_PROD,
+                  max_cand: int = abi.DEFAULT_MAX_CAND, max_prod: int = abi.MAX_PROD_ADAPTIVE,
                   count_pair_tests: bool = False, sort: bool = True) -> EnumerationResult:
         o = self._opts(method, dd.problem_type, state_dominance, max_cand, max_prod,
                        count_pair_tests)
@@ -143,7 +144,7 @@ class ResidentDD:
         self.d = d
 
     def enumerate(self, method: int = 1, state_dominance: int = 0,
-                  max_cand: int = abi.DEFAULT_MAX_CAND, max_prod: int = abi.DEFAULT_MAX_PROD,
+                  max_cand: int = abi.DEFAULT_MAX_CAND, max_prod: int = abi.MAX_PROD_ADAPTIVE,
                   count_pair_tests: bool = False, sort: bool = True) -> EnumerationResult:
         eng = self.engine
         o = eng._opts(method, self.problem_type, state_dominance, max_cand, max_prod,

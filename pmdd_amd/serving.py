@@ -47,7 +47,7 @@ class Server:
 
     def submit(self, dd, method: int = 1, state_dominance: int = 0,
                max_cand: int = abi.DEFAULT_MAX_CAND,
-               max_prod: int = abi.DEFAULT_MAX_PROD) -> int:
+               max_prod: int = abi.MAX_PROD_ADAPTIVE) -> int:
         o = abi.Opts(int(method), int(dd.problem_type), int(state_dominance), 0, int(max_cand),
                      int(max_prod))
         t = C.c_int64()
@@ -91,6 +91,14 @@ class Server:
             return int(t.value), PmddError(rc, err.value.decode())
         pts = _take_frontier(self._lib, fr, True)
         return int(t.value), EnumerationResult(pts, _stats_dict(st, 0, None, None))
+
+    def info(self) -> dict:
+        """The server's admission record (pmdd_server_get_info)."""
+        inf = abi.ServerInfo()
+        rc = self._lib.pmdd_server_get_info(self._h, C.byref(inf))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, "pmdd_server_get_info failed")
+        return {name: getattr(inf, name) for name, _ in abi.ServerInfo._fields_}
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
@@ -152,17 +160,22 @@ def write_stream(specs: list[dict], directory: str) -> list[str]:
 
 def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots: int,
                method: int = 1, device: int = 0, builders: int = 16,
-               results: dict | None = None, progress: int = 0) -> dict:
+               results: dict | None = None, progress: int = 0,
+               stats: dict | None = None, memory_budget: int = 0) -> dict:
     """Replay the stream: at each arrival the request's diagram is built on a
     host thread and submitted; one collector thread takes results as they
     finish (pmdd_server_wait_any); latency runs from arrival to result.
-    With progress=k, a line goes to stderr after every k results."""
-    server = Server(device, slots=slots)
+    With progress=k, a line goes to stderr after every k results.  `results`
+    and `stats` receive each request's frontier and statistics by index; the
+    returned summary holds the server's admission record and the largest
+    ratio of a request's measured device memory to its estimate."""
+    server = Server(device, slots=slots, memory_budget=memory_budget)
     n = len(paths)
     done_at = [0.0] * n
     points = [0] * n
     errors = {}
     owner = {}  # ticket -> request
+    mem = {}  # request -> (measured peak device bytes, admission estimate)
     lock = threading.Lock()
     events = queue.Queue()  # one item per request: True once submitted, False if not
     t0 = time.perf_counter()
@@ -199,8 +212,11 @@ def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots
                     errors[i] = (res.code, res.message)
                 else:
                     points[i] = len(res.points)
+                    mem[i] = (res.stats["ctx_mem_peak_bytes"], res.stats["mem_estimate_bytes"])
                     if results is not None:
                         results[i] = res.points
+                    if stats is not None:
+                        stats[i] = res.stats
             done_at[i] = now
 
     collector = threading.Thread(target=collect)
@@ -208,10 +224,16 @@ def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots
     with ThreadPoolExecutor(max_workers=builders) as pool:
         list(pool.map(submit, range(n)))
     collector.join()
+    info = server.info()
     server.close()
     lat = np.array([done_at[i] - (t0 + arrivals[i]) for i in range(n) if i not in errors])
     span = max(done_at) - t0
+    ratios = [peak / est for peak, est in mem.values() if est > 0]
+    over = sum(1 for peak, est in mem.values() if peak > est)
     return {
+        "admission": info,
+        "mem_peak_over_estimate_max": max(ratios) if ratios else None,
+        "mem_peak_above_estimate": over,
         "slots": slots,
         "requests": n,
         "failed": len(errors),

@@ -3,6 +3,8 @@
 Every comparison is bit-exact on the frontier as a lexicographically sorted
 set of int32 vectors, the way the reference writes it (main.cpp:403).
 """
+import hashlib
+import json
 import os
 
 import numpy as np
@@ -10,7 +12,7 @@ import pytest
 
 import oracle
 from conftest import instance_path, problem_type_of
-from pmdd_amd import DecisionDiagram
+from pmdd_amd import DecisionDiagram, abi
 from pmdd_amd import instances as gen
 
 pytestmark = pytest.mark.gpu
@@ -56,13 +58,80 @@ def test_candidate_budget_invariance(engine, name, max_cand):
 
 
 @pytest.mark.parametrize("name", ["KP_p-3_n-30_ins-2.dat", "tsp-nobj3-ncities10-seed12870.dat"])
-@pytest.mark.parametrize("max_prod", [1, 1000, 625000])
+@pytest.mark.parametrize("max_prod", [1, 1000, 625000, 0])
 def test_product_budget_invariance(engine, name, max_prod):
+    """--max-prod caps every coupling batch (couple.cu:393-403); a node with
+    more products is split over batches, so no batch exceeds the cap, and
+    the frontier does not depend on the split.  0 is the adaptive mode."""
     print(f"case {name} max_prod={max_prod}", flush=True)
     dd = _dd(name)
     ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
-    got = engine.enumerate(dd, method=3, max_prod=max_prod).points
-    assert _same(got, ref)
+    res = engine.enumerate(dd, method=3, max_prod=max_prod)
+    assert _same(res.points, ref)
+    s = res.stats
+    assert s["join_batches"] >= 1
+    if max_prod > 0:
+        assert s["join_batch_peak_products"] <= max_prod
+    if max_prod == 1:
+        # every product alone: every node was split down to single products
+        assert s["join_batches"] == s["work_join_products_total"]
+
+
+def test_tsp12_product_cap_holds(engine, tmp_path):
+    """The 3-objective TSP n=12 configuration at --max-cand 20M --max-prod
+    625K: no coupling batch holds more than 625K products, and methods 1 and 3
+    give the oracle's frontier."""
+    path = gen.write(gen.tsp(12_3, 12, 3), str(tmp_path / "tsp12.dat"))
+    dd = DecisionDiagram.from_file(path, 3)
+    ref = oracle.enumerate_dd(dd, 3, 0, threads=8)
+    res = engine.enumerate(dd, method=3, max_cand=20_000_000, max_prod=625_000)
+    assert _same(res.points, ref)
+    assert 0 < res.stats["join_batch_peak_products"] <= 625_000
+    assert _same(engine.enumerate(dd, method=1, max_cand=20_000_000).points, ref)
+
+
+def _write_knapsack(path, profits, weights, capacity):
+    """A knapsack file in the reference format (knapsack_instance.cpp:17-47):
+    "n 1 d", d rows of profits, the weights, the capacity."""
+    d, n = len(profits), len(weights)
+    with open(path, "w") as f:
+        f.write(f"{n} 1 {d}\n")
+        for row in profits:
+            f.write(" ".join(str(x) for x in row) + "\n")
+        f.write(" ".join(str(x) for x in weights) + "\n")
+        f.write(f"{capacity}\n")
+    return str(path)
+
+
+@pytest.mark.parametrize("capacity", [1, 2])
+def test_path_sum_bound_exactly_int32_max(engine, tmp_path, capacity):
+    """The upload check bounds every path sum by the sum over layers of the
+    largest |arc weight|; at exactly INT32_MAX the enumeration runs and is
+    exact (with capacity 2 the single Pareto point has objective 0 equal to
+    INT32_MAX)."""
+    big = 1 << 30
+    path = _write_knapsack(tmp_path / "edge.dat", [[big, big - 1], [1, 2]], [1, 1], capacity)
+    bf = oracle.bruteforce(path, 1)
+    if capacity == 2:
+        assert bf.tolist() == [[2**31 - 1, 3]]
+    else:
+        assert bf.tolist() == [[big - 1, 2], [big, 1]]
+    dd = DecisionDiagram.from_file(path, 1)
+    for method in (1, 2, 3):
+        got = engine.enumerate(dd, method=method).points
+        assert _same(got, bf), method
+
+
+def test_path_sum_bound_above_int32_max_is_refused(engine, tmp_path):
+    """One more than INT32_MAX: PMDD_E_OVERFLOW, before anything runs (the
+    reference adds unchecked, bottomup.cu:124, couple.cu:93-94)."""
+    big = 1 << 30
+    path = _write_knapsack(tmp_path / "over.dat", [[big, big], [1, 2]], [1, 1], 2)
+    dd = DecisionDiagram.from_file(path, 1)
+    for method in (1, 3):
+        with pytest.raises(abi.PmddError) as e:
+            engine.enumerate(dd, method=method)
+        assert e.value.code == abi.PMDD_E_OVERFLOW
 
 
 def test_tsp15_seed108_known_count(engine):
@@ -236,7 +305,6 @@ def test_tsp15_5obj_seed13095_known_count(engine):
 @pytest.mark.parametrize("family,name,kind,sd", [
     ("mokp_n40_k7", "knapsack-40-7-1000-12144.dat", 1, 1),
     ("mospp_n150_k7", "bp-150-30-7-10-14816.dat", 2, 0),
-    ("motsp_n15_k5", "tsp-nobj5-ncities15-seed11093.dat", 3, 0),
 ])
 def test_paper_instances_methods_agree(engine, family, name, kind, sd):
     """Instances of the reference paper's benchmark (too large for the
@@ -251,3 +319,51 @@ def test_paper_instances_methods_agree(engine, family, name, kind, sd):
     for method in methods:
         got = engine.enumerate(dd, method=method, state_dominance=sd).points
         assert _same(got, ref), (name, method, len(got), len(ref))
+
+
+# Solution counts the reference reports for the 5-objective 15-city TSP
+# family, kb/autoresearch/TSP_STATUS.md:28-37 (seed13095 is the known-count
+# test above).
+TSP5_COUNTS = {1081: 55198, 11093: 29505, 14792: 77303, 23138: 51306, 24645: 38883,
+               30131: 24381, 32239: 78849, 6440: 32237, 7702: 53917}
+
+
+@pytest.mark.parametrize("seed", sorted(TSP5_COUNTS))
+def test_tsp15_5obj_reference_counts(engine, seed):
+    path = os.path.join(os.path.dirname(__file__), "golden", "paper", "motsp_n15_k5",
+                        f"tsp-nobj5-ncities15-seed{seed}.dat")
+    print(f"case tsp15 5obj seed{seed}", flush=True)
+    dd = DecisionDiagram.from_file(path, 3)
+    a = engine.enumerate(dd, method=3).points
+    assert len(a) == TSP5_COUNTS[seed]
+    b = engine.enumerate(dd, method=1).points
+    assert _same(a, b)
+
+
+BIGNODE_PATH = os.path.join(os.path.dirname(__file__), "golden", "bignode_digests.json")
+BIGNODE = json.load(open(BIGNODE_PATH)) if os.path.exists(BIGNODE_PATH) else {}
+
+
+def _digest(points):
+    return hashlib.sha256(points.astype("<i4").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", sorted(BIGNODE))
+def test_bignode_instances_match_oracle_digests(engine, name):
+    """6- and 7-objective reference-paper instances whose nodes exceed one
+    chunk (tests/golden/make_bignode_digests.py): the big-node path (merged
+    parent runs, key-ordered blocks, cross filter against the node's front)
+    against the oracle's frontier digest, methods 1 and 3 (knapsack also
+    with state dominance)."""
+    g = BIGNODE[name]
+    path = os.path.join(os.path.dirname(__file__), "golden", "paper", g["family"], name)
+    dd = DecisionDiagram.from_file(path, g["kind"])
+    sds = (0, 1) if g["kind"] == 1 else (0,)
+    for method in (1, 3):
+        for sd in sds:
+            print(f"case bignode {name} m{method} sd{sd}", flush=True)
+            res = engine.enumerate(dd, method=method, state_dominance=sd)
+            assert len(res.points) == g["count"], (name, method, sd)
+            assert _digest(res.points) == g["sha256_sorted"], (name, method, sd)
+            if method == 1 and sd == 0:
+                assert res.stats["cross_launches"] > 0  # big nodes were met
