@@ -46,6 +46,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+T_START = time.perf_counter()
+# coupling batches: adaptive (0), 32K products doubling to 32M
+MAX_PROD = 0
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
 VALU_PEAK_TOPS = 39.3           # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, int32 lane-ops/s (T)
 
@@ -72,11 +75,12 @@ def build_workload(tmpdir):
 
 
 def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (profiles/<kernel>_knapsack_traffic.txt), or None."""
+    """(HBM bytes per launch of `kernel`, source) from the committed rocprofv3
+    PMC summary profiles/<kernel>_knapsack_traffic.txt, or (None, None).  The
+    source names the file and the commit it was measured at."""
     p = os.path.join(ROOT, "profiles", f"{kernel}_knapsack_traffic.txt")
     if not os.path.exists(p):
-        return None
+        return None, None
     vals = {}
     with open(p) as f:
         for line in f:
@@ -84,56 +88,97 @@ def read_traffic(kernel):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
     try:
-        return float(vals["hbm_bytes_per_launch"])
+        return float(vals["hbm_bytes_per_launch"]), \
+            f"profiles/{kernel}_knapsack_traffic.txt (measured at {vals.get('commit', '?')})"
     except (KeyError, ValueError):
-        return None
+        return None, None
 
 
-def cpu_baseline(dds, threads):
-    """Oracle (CPU restatement of the reference OpenMP path) on a bounded
-    sample: the n=80 3-objective knapsack, methods 1 and 3, no state
-    dominance."""
+def host_cores():
+    """(physical cores, logical CPUs this process may run on) of this host."""
+    logical = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (
+        os.cpu_count() or 1)
+    phys = None
+    try:
+        seen = set()
+        with open("/proc/cpuinfo") as f:
+            pid = cid = None
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    cid = line.split(":")[1].strip()
+                elif not line.strip():
+                    if cid is not None:
+                        seen.add((pid, cid))
+                    pid = cid = None
+        phys = len(seen) or None
+    except OSError:
+        pass
+    return phys, logical
+
+
+# CPU sample: (instance, method) pairs of the step's workload, state dominance 0
+CPU_SAMPLE = [("kp_d3_n80", 1), ("kp_d3_n80", 3), ("kp_d5_n50", 3), ("kp_d5_n50", 1)]
+
+
+def cpu_baseline(dds, threads, budget_s):
+    """Oracle (CPU restatement of the reference's OpenMP enumerators, methods
+    1 and 3) on the step's two instances, state dominance 0, on `threads`
+    host threads, in the order of CPU_SAMPLE, stopping before a pair once
+    budget_s seconds are spent (the d=5 method-1 run is the longest)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    name, path, dd = dds[0]
-
-    def run(thr):
-        pts = 0
+    by_name = {name: dd for name, _, dd in dds}
+    pts, spent, ran, skipped = 0, 0.0, [], []
+    for name, method in CPU_SAMPLE:
+        if name not in by_name:
+            continue
+        if ran and spent > budget_s:
+            skipped.append(f"{name} m{method}")
+            continue
         t0 = time.perf_counter()
-        for method in (1, 3):
-            pts += len(oracle.enumerate_dd(dd, method, 0, threads=thr))
-        return pts, time.perf_counter() - t0
-
-    pts, dt = run(threads)
-    pts1, dt1 = run(1)
+        n = len(oracle.enumerate_dd(by_name[name], method, 0, threads=threads))
+        dt = time.perf_counter() - t0
+        pts += n
+        spent += dt
+        ran.append(f"{name} m{method}: {n} points in {dt:.2f} s")
+        log(f"cpu baseline: {name} m{method} {dt:.2f} s")
+    phys, logical = host_cores()
     return {
-        "value": pts / dt,
+        "value": pts / spent,
         "unit": "Pareto pts/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{name} (seeded knapsack n=80 d=3) methods 1 and 3, state dominance 0; "
-                  f"{pts} Pareto points in {dt:.2f} s",
-        "value_1_thread": pts1 / dt1,
-        "sample_1_thread": f"the same sample on 1 thread: {dt1:.2f} s",
+        "sample": "oracle (OpenMP restatement of the reference's CPU enumerators), state "
+                  "dominance 0: " + "; ".join(ran) +
+                  (("; not run (time bound): " + ", ".join(skipped)) if skipped else ""),
+        "host_physical_cores": phys,
+        "host_logical_cpus": logical,
+        "threads_note": "threads = the box's CPU share for one GPU (16) or fewer if the "
+                        "process may use fewer CPUs",
     }
 
 
-def serving_leg(tmpdir, requests, slots, method, rate_factors=(0.5, 1.0, 1.5)):
+def serving_leg(tmpdir, requests, slots, method, rate_factors, deadline):
     """Plain (one slot) vs batched (`slots` slots) on the first `requests`
-    requests of the serving stream: closed loop, then Poisson arrivals at
-    each factor of the plain path's closed-loop capacity."""
+    requests of the serving stream: closed loop (all queued at t=0), then
+    Poisson arrivals at each factor of the plain path's closed-loop capacity.
+    The open-loop legs replay a prefix of the stream short enough to end
+    before `deadline` (perf_counter seconds); the line states its length."""
     from pmdd_amd import serving
     specs = serving.mixed_stream(requests)
     paths = serving.write_stream(specs, os.path.join(tmpdir, "stream"))
-    keys = ("inst_per_s", "pareto_pts_per_s", "p50_latency_s", "p99_latency_s",
-            "mean_latency_s", "span_s", "failed")
+    keys = ("requests", "inst_per_s", "pareto_pts_per_s", "p50_latency_s", "p99_latency_s",
+            "mean_latency_s", "span_s", "failed", "mem_peak_over_estimate_max",
+            "mem_peak_above_estimate", "admission")
     out = {"requests": requests, "method": method, "slots": slots,
            "mix": "60% knapsack n~U[20,70], 25% set packing n~U[60,150] m=n/5, "
                   "15% TSP n~U[8,11]; d~U{3,4,5}; seeded SplitMix64"}
     res_plain, res_batched = {}, {}
     t0 = time.perf_counter()
     closed = [0.0] * requests
-    progress = 200 if requests > 200 else 0
+    progress = 250 if requests > 250 else 0
     plain = serving.run_stream(paths, specs, closed, slots=1, method=method, results=res_plain,
                                progress=progress)
     log(f"serving: plain closed loop {plain['inst_per_s']:.2f} inst/s")
@@ -147,21 +192,28 @@ def serving_leg(tmpdir, requests, slots, method, rate_factors=(0.5, 1.0, 1.5)):
     out["closed_loop"] = {"plain": {k: plain[k] for k in keys},
                           "batched": {k: batched[k] for k in keys}}
     sweep = []
-    for f in sorted(set(rate_factors) | {1.0}):
+    for f in rate_factors:
         lam = f * cap
-        arrivals = serving.poisson_arrivals(requests, lam)
-        plain_o = serving.run_stream(paths, specs, arrivals, slots=1, method=method,
+        # both paths replay n requests at rate lam: about 2 n / lam seconds
+        left = deadline - time.perf_counter()
+        n = min(requests, int(0.85 * left * lam / 2))
+        if n < 50:
+            log(f"serving: no time left for Poisson arrivals at {f}x")
+            continue
+        arrivals = serving.poisson_arrivals(n, lam)
+        plain_o = serving.run_stream(paths[:n], specs[:n], arrivals, slots=1, method=method,
                                      progress=progress)
-        log(f"serving: plain open loop at {f}x capacity done")
-        batched_o = serving.run_stream(paths, specs, arrivals, slots=slots, method=method,
-                                       progress=progress)
-        log(f"serving: batched open loop at {f}x capacity done")
-        entry = {"rate_factor": f, "rate_per_s": lam,
+        log(f"serving: plain open loop at {f}x capacity, {n} requests done")
+        batched_o = serving.run_stream(paths[:n], specs[:n], arrivals, slots=slots,
+                                       method=method, progress=progress)
+        log(f"serving: batched open loop at {f}x capacity, {n} requests done")
+        entry = {"rate_factor": f, "rate_per_s": lam, "requests": n,
                  "plain": {k: plain_o[k] for k in keys},
                  "batched": {k: batched_o[k] for k in keys}}
         sweep.append(entry)
         if f == 1.0:
-            out["poisson"] = {k: entry[k] for k in ("rate_per_s", "plain", "batched")}
+            out["poisson"] = {k: entry[k] for k in ("rate_per_s", "requests", "plain",
+                                                    "batched")}
     out["poisson_sweep"] = sweep
     out["batched_over_plain_inst_per_s"] = batched["inst_per_s"] / max(plain["inst_per_s"], 1e-12)
     out["bit_exact_batched_vs_plain"] = same
@@ -246,11 +298,15 @@ def main():
     ap.add_argument("--only", default="", help="run only this workload instance")
     ap.add_argument("--cases", default="", help="method:state_dominance list, e.g. 3:0,1:0")
     ap.add_argument("--calib-copy-mib", type=int, default=0)
-    ap.add_argument("--serving-requests", type=int, default=100,
+    ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests of the serving stream to replay (0: skip; 2000: all)")
+    ap.add_argument("--time-budget-s", type=float, default=480.0,
+                    help="the Poisson legs of the serving replay are cut to end by then")
+    ap.add_argument("--cpu-budget-s", type=float, default=20.0,
+                    help="the CPU baseline starts no new enumeration after this")
     ap.add_argument("--serving-slots", type=int, default=8)
     ap.add_argument("--serving-method", type=int, default=1)
-    ap.add_argument("--serving-rates", default="0.5,1,1.5",
+    ap.add_argument("--serving-rates", default="1",
                     help="Poisson arrival rates as factors of the plain path's capacity")
     ap.add_argument("--paper", action="store_true",
                     help="run the reference paper's instance families instead of the step")
@@ -295,12 +351,13 @@ def main():
         agg = {"kernel_filter_s": 0.0, "filter_launches": 0, "algorithmic_bytes": 0,
                "kernel_frontier_check_s": 0.0, "frontier_check_launches": 0,
                "frontier_check_bytes": 0, "kernel_cross_s": 0.0, "cross_launches": 0,
-               "cross_bytes": 0, "work_frontier_survivors_total": 0, "kernel_total_s": 0.0}
+               "cross_bytes": 0, "work_frontier_survivors_total": 0, "kernel_total_s": 0.0,
+               "kernel_state_dominance_s": 0.0, "state_dominance_launches": 0, "join_bytes": 0}
         per = {}
         for name, r in res:
             ref = None
             for method, sd in CASES:
-                out = r.enumerate(method=method, state_dominance=sd)
+                out = r.enumerate(method=method, state_dominance=sd, max_prod=MAX_PROD)
                 if ref is None:
                     ref = out.points
                 elif out.points.shape != ref.shape or not (out.points == ref).all():
@@ -330,79 +387,103 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / max(args.steps, 1)
     enumerations = args.steps * len(res) * len(CASES)
+    steps = max(args.steps, 1)
 
-    # executed pair tests of one step (deterministic), from a counting run
-    pair_tests = None
+    # the same step with every diagram uploaded inside the timed call (the
+    # reference's wall_enumeration_s wraps packing and the host-to-device copy,
+    # main.cpp:327-394 -> enum.cu:438-593)
+    t1 = time.perf_counter()
+    for name, _, dd in dds:
+        for method, sd in CASES:
+            eng.enumerate(dd, method=method, state_dominance=sd, max_prod=MAX_PROD)
+    ms_with_upload = 1000.0 * (time.perf_counter() - t1)
+    log("upload-inclusive step done")
+
+    # executed pair tests of one step per kernel family (deterministic), from
+    # a counting run; each enumeration's tests count d compares
+    fam_keys = {"k_filter_chunks": "pair_tests_filter",
+                "k_refill_vs_block": "pair_tests_frontier_check",
+                "k_cross_chunks": "pair_tests_cross",
+                "k_state_dominance": "pair_tests_state_dominance"}
+    tests = {k: 0 for k in fam_keys}
+    lane_ops = {k: 0 for k in fam_keys}
     if not args.skip_pair_count:
-        pair_tests = 0
+        d_of = {name: d for name, _, _, d in WORKLOAD}
         for name, r in res:
             for method, sd in CASES:
-                pair_tests += r.enumerate(method=method, state_dominance=sd,
-                                          count_pair_tests=True).stats["dominance_pair_tests"]
+                st = r.enumerate(method=method, state_dominance=sd, max_prod=MAX_PROD,
+                                 count_pair_tests=True).stats
+                for fam, key in fam_keys.items():
+                    tests[fam] += st[key]
+                    lane_ops[fam] += st[key] * d_of[name]
         log("pair-test count done")
 
-    # dominant kernel: the one with the most device time over the timed steps
-    kernels = {
-        # k_filter_chunks: compulsory bytes of the expanded layers and joins
+    # per kernel family: device time (HIP events on the engine's stream over
+    # the timed steps), launches, compulsory HBM bytes, executed pair tests
+    fam = {
         "k_filter_chunks": (tot["kernel_filter_s"], tot["filter_launches"],
                             tot["algorithmic_bytes"]),
-        # k_refill_vs_block: products read once per block of R, survivors
-        # written, the block read once; and R flagged against blocks of the
-        # batch frontier (R and the block read once, R's flags read and written)
         "k_refill_vs_block": (tot["kernel_frontier_check_s"], tot["frontier_check_launches"],
-                              tot["frontier_check_bytes"]),
-        # k_cross_chunks (one launch per wave): per wave the group's front and
-        # the wave's points read once, survivors appended once, a flag per point
+                              tot["join_bytes"]),
         "k_cross_chunks": (tot["kernel_cross_s"], tot["cross_launches"], tot["cross_bytes"]),
+        "k_state_dominance": (tot["kernel_state_dominance_s"], tot["state_dominance_launches"],
+                              None),
     }
-    kname = max(kernels, key=lambda k: kernels[k][0])
-    k_time, k_launches, k_bytes = kernels[kname]
-    launches = max(k_launches, 1)
-    alg_bytes_per_launch = k_bytes / launches
-    avg_launch_s = k_time / launches
-    achieved_gbs = alg_bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-    traffic = read_traffic(kname)
+    families = {}
+    for k, (t, n, b) in fam.items():
+        e = {"device_ms_per_step": t / steps * 1e3, "launches_per_step": n / steps,
+             "avg_launch_ms": t / max(n, 1) * 1e3,
+             "share_of_engine_kernel_time": t / max(tot["kernel_total_s"], 1e-12)}
+        if not args.skip_pair_count and t > 0:
+            tops = lane_ops[k] * steps / t / 1e12
+            e.update({"pair_tests_per_step": tests[k],
+                      "valu_achieved_tops": tops, "valu_frac": tops / VALU_PEAK_TOPS,
+                      "tests_per_s": tests[k] * steps / t})
+        families[k] = e
+    kname = max(fam, key=lambda k: fam[k][0])
+    k_time, k_launches, k_bytes = fam[kname]
+    avg_launch_s = k_time / max(k_launches, 1)
+    traffic, traffic_src = read_traffic(kname)
+    if args.skip_pair_count:
+        achieved = None
+    else:
+        achieved = lane_ops[kname] * steps / max(k_time, 1e-12) / 1e12
     roofline = {
         "kernel": kname,
-        "bound": "hbm",
-        "achieved": achieved_gbs,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": achieved_gbs / HBM_PEAK_GBS,
+        "bound": "valu",
+        "achieved": achieved,
+        "peak": VALU_PEAK_TOPS,
+        "unit": "TOP/s (int32 compare lane-ops: d per executed pair test)",
+        "frac": achieved / VALU_PEAK_TOPS if achieved is not None else None,
         "traffic": traffic,
+        "traffic_source": traffic_src,
         "avg_launch_ms": avg_launch_s * 1e3,
-        "launches_per_step": k_launches / max(args.steps, 1),
-        "algorithmic_bytes_per_launch": alg_bytes_per_launch,
+        "launches_per_step": k_launches / steps,
         "share_of_engine_kernel_time": k_time / max(tot["kernel_total_s"], 1e-12),
-        # every family's live per-launch average, for comparison with rocprofv3
-        "families": {
-            k: {"avg_launch_ms": v[0] / max(v[1], 1) * 1e3,
-                "launches_per_step": v[1] / max(args.steps, 1),
-                "share_of_engine_kernel_time": v[0] / max(tot["kernel_total_s"], 1e-12)}
-            for k, v in kernels.items()
-        },
+        "families": families,
     }
-    if pair_tests is not None:
-        d_avg = sum(d for _, _, _, d in WORKLOAD) / len(WORKLOAD)
-        tops = pair_tests * d_avg * args.steps / max(tot["kernel_total_s"], 1e-12) / 1e12
-        roofline["valu"] = {
-            "achieved": tops, "peak": VALU_PEAK_TOPS, "unit": "T int32 lane-ops/s",
-            "frac": tops / VALU_PEAK_TOPS,
-            "pair_tests_per_step": pair_tests,
-            "note": "d compares per executed pair test over all engine kernel time",
-        }
+    if k_bytes is not None and k_launches > 0:
+        gbs = k_bytes / k_launches / avg_launch_s / 1e9
+        roofline["hbm"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": gbs / HBM_PEAK_GBS,
+                           "algorithmic_bytes_per_launch": k_bytes / k_launches,
+                           "bytes": "coupling: 4*d*(TD and BU points of each batch + final "
+                                    "frontier)" if kname == "k_refill_vs_block" else
+                                    "compulsory bytes of the family"}
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(dds, threads)
+        phys, logical = host_cores()
+        threads = args.cpu_threads or min(logical, 16)
+        cpu = cpu_baseline(dds, threads, args.cpu_budget_s)
         log(f"cpu baseline done: {cpu['value']:.1f} pts/s on {threads} threads")
 
     srv = None
     if args.serving_requests > 0:
         rates = tuple(float(x) for x in args.serving_rates.split(",") if x)
+        deadline = T_START + args.time_budget_s
         srv = serving_leg(tmp, args.serving_requests, args.serving_slots, args.serving_method,
-                          rates)
+                          rates, deadline)
         if not srv["bit_exact_batched_vs_plain"]:
             raise SystemExit("serving: batched and plain frontiers differ")
 
@@ -414,6 +495,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "ms_per_step_with_upload": ms_with_upload,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -421,18 +503,22 @@ def main():
         "data": "synthetic (seeded SplitMix64 knapsack instances)",
         "config": {
             "workload": "single large knapsack BDDs: d=3 n=80 and d=5 n=50, methods 1 and 3, "
-                        "state dominance 0 and 1 (8 enumerations per step), plain GPU path",
+                        "state dominance 0 and 1 (8 enumerations per step), plain GPU path, "
+                        "diagrams resident in HBM",
             "instances": [n for n, _, _ in dds],
             "max_cand": 20_000_000,
-            "max_prod": 625_000,
+            "max_prod": "adaptive: coupling batches of 32K products doubling to 32M (the "
+                        "frontier does not depend on the batch split)",
         },
         "inst_per_s": enumerations / elapsed,
         "frontier_points_processed_per_s": tot["work_frontier_survivors_total"] / elapsed,
-        "comparisons_per_s": (pair_tests * args.steps / elapsed) if pair_tests else None,
-        "pareto_points_per_step": total_pts / max(args.steps, 1),
+        "comparisons_per_s": (sum(tests.values()) * args.steps / elapsed)
+        if not args.skip_pair_count else None,
+        "pareto_points_per_step": total_pts / steps,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": srv,
+        "wall_s": time.perf_counter() - T_START,
     }
     print(json.dumps(line), flush=True)
     for _, r in res:

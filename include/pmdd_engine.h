@@ -165,6 +165,14 @@ typedef struct pmdd_stats {
     int64_t pair_tests_frontier_check;
     int64_t pair_tests_cross;
     int64_t pair_tests_state_dominance;
+    /* State dominance (k_state_dominance) device time and launches; part of
+     * kernel_dominance_s. */
+    double kernel_state_dominance_s;
+    int64_t state_dominance_launches;
+    /* Compulsory HBM bytes of the coupling: every TD and BU point of the
+     * cutset read once per batch it takes part in, the final frontier
+     * written once, 4*d*(sum over batches of (#TD rows + #BU columns) + #R). */
+    int64_t join_bytes;
     /* Coupling batches (method 3) and the most products any one held. */
     int64_t join_batches;
     int64_t join_batch_peak_products;
@@ -172,6 +180,10 @@ typedef struct pmdd_stats {
      * from earlier calls included) plus the diagram, counted exactly from the
      * engine's own allocations (gpu_mem_peak_used_bytes is device-wide). */
     int64_t ctx_mem_peak_bytes;
+    /* The same, counting only the bytes this call asked for (each buffer at
+     * the largest size the call needed of it, the diagram included): what
+     * the call needs on a context that kept nothing from earlier calls. */
+    int64_t mem_need_peak_bytes;
     /* Serving only: the admission estimate of the request (bytes), the
      * times it was retried alone after running out of device memory next to
      * other requests, and seconds from submit to admission. */

@@ -102,9 +102,13 @@ class Stats(C.Structure):
         ("pair_tests_frontier_check", C.c_int64),
         ("pair_tests_cross", C.c_int64),
         ("pair_tests_state_dominance", C.c_int64),
+        ("kernel_state_dominance_s", C.c_double),
+        ("state_dominance_launches", C.c_int64),
+        ("join_bytes", C.c_int64),
         ("join_batches", C.c_int64),
         ("join_batch_peak_products", C.c_int64),
         ("ctx_mem_peak_bytes", C.c_int64),
+        ("mem_need_peak_bytes", C.c_int64),
         ("mem_estimate_bytes", C.c_int64),
         ("nomem_retries", C.c_int32),
         ("reserved2", C.c_int32),

@@ -70,6 +70,18 @@ static std::atomic<int64_t> g_fail_alloc_in{0};
 struct MemAcct {
     std::atomic<int64_t> cur{0};
     std::atomic<int64_t> peak{0};
+    // What the current call asked for: every buffer it grew or reused counts
+    // at the largest size the call asked of it (buffers kept from earlier
+    // calls and not touched do not count).  Only the thread running the call
+    // updates these.
+    int64_t epoch = 0;
+    int64_t need_cur = 0;
+    int64_t need_peak = 0;
+    void begin_call() {
+        ++epoch;
+        need_cur = 0;
+        need_peak = 0;
+    }
     void add(int64_t b) {
         const int64_t now = cur.fetch_add(b, std::memory_order_relaxed) + b;
         int64_t p = peak.load(std::memory_order_relaxed);
@@ -96,27 +108,35 @@ struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     MemAcct *acct = nullptr;
+    MemAcct *need_acct = nullptr; // account of the call that last asked for this buffer
+    int64_t epoch = -1;           // that call
+    int64_t call_need = 0;        // the most that call asked for
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), acct(o.acct) {
-        o.p = nullptr;
-        o.bytes = 0;
-        o.acct = nullptr;
-    }
+    DevBuf(DevBuf &&o) noexcept { take(o); }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
             release();
-            p = o.p;
-            bytes = o.bytes;
-            acct = o.acct;
-            o.p = nullptr;
-            o.bytes = 0;
-            o.acct = nullptr;
+            take(o);
         }
         return *this;
     }
     ~DevBuf() { release(); }
+    void take(DevBuf &o) noexcept {
+        p = o.p;
+        bytes = o.bytes;
+        acct = o.acct;
+        need_acct = o.need_acct;
+        epoch = o.epoch;
+        call_need = o.call_need;
+        o.p = nullptr;
+        o.bytes = 0;
+        o.acct = nullptr;
+        o.need_acct = nullptr;
+        o.epoch = -1;
+        o.call_need = 0;
+    }
     void release() {
         if (p != nullptr) {
             (void)hipFree(p);
@@ -124,20 +144,48 @@ struct DevBuf {
                 acct->sub(static_cast<int64_t>(bytes));
             }
         }
+        if (need_acct != nullptr && need_acct == t_acct && epoch == t_acct->epoch) {
+            t_acct->need_cur -= call_need;
+        }
         p = nullptr;
         bytes = 0;
         acct = nullptr;
+        need_acct = nullptr;
+        epoch = -1;
+        call_need = 0;
     }
     // Exchanges the allocations of two buffers (their accounts go with them).
     void swap(DevBuf &o) noexcept {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
         std::swap(acct, o.acct);
+        std::swap(need_acct, o.need_acct);
+        std::swap(epoch, o.epoch);
+        std::swap(call_need, o.call_need);
+    }
+    // Records that the current call asks for `need` bytes of this buffer.
+    void note_need(size_t need) {
+        MemAcct *a = t_acct;
+        if (a == nullptr) {
+            return;
+        }
+        if (need_acct != a || epoch != a->epoch) {
+            need_acct = a;
+            epoch = a->epoch;
+            call_need = 0;
+        }
+        const int64_t n = static_cast<int64_t>(need);
+        if (n > call_need) {
+            a->need_cur += n - call_need;
+            call_need = n;
+            a->need_peak = std::max(a->need_peak, a->need_cur);
+        }
     }
     // Grows by half again (with at most 64 MiB of slack), so a buffer sized
     // by a widening layer is not reallocated on every layer.
     void ensure(size_t need, const char *what) {
         if (need <= bytes && p != nullptr) {
+            note_need(need);
             return;
         }
         size_t nb = std::max<size_t>(need, 256);
@@ -156,6 +204,7 @@ struct DevBuf {
             acct->add(static_cast<int64_t>(nb));
         }
         g_device_allocs.fetch_add(1, std::memory_order_relaxed);
+        note_need(need); // release() took back what the call asked of the old one
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
@@ -257,20 +306,28 @@ struct pmdd_ctx {
     // estimate.
     int64_t join_cap = 0;
 
-    // Releases every device buffer the context keeps between calls (the
-    // stream must be idle).
-    void trim() {
+    // Every device buffer the context keeps between calls.
+    template <class F> void for_each_buffer(F &&f) {
         for (ChunkBufs *cb : {&cb_main, &cb_aux}) {
             for (DevBuf *b : {&cb->chunks, &cb->nsurv, &cb->nalive, &cb->out}) {
-                b->release();
+                f(*b);
             }
         }
         for (DevBuf *b : {&run_off, &which, &which_a, &which_b, &which_c, &which_s, &which_m,
                           &surv_pts, &alive, &counter, &join_meta, &keep_a, &keep_b, &sort_a,
                           &sort_b, &big_base, &big_node, &run_start, &grp_run, &join_idx, &join_x,
                           &join_y, &join_ctr, &front_count, &merge_a, &merge_b}) {
-            b->release();
+            f(*b);
         }
+        for (DevFrontier &fr : frontier_pool) {
+            f(fr.off);
+            f(fr.pts);
+        }
+    }
+    // Releases every device buffer the context keeps between calls (the
+    // stream must be idle).
+    void trim() {
+        for_each_buffer([](DevBuf &b) { b.release(); });
         frontier_pool.clear();
     }
 };
@@ -625,6 +682,7 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
 // Grow a SoA point buffer to hold `need` points, keeping the first `keep`.
 void grow_points(DevFrontier &f, int d, int64_t need, int64_t keep, hipStream_t s) {
     if (need <= f.cap && f.pts.p != nullptr) {
+        f.pts.note_need(static_cast<size_t>(need) * d * 4);
         return;
     }
     const int64_t ncap = std::max<int64_t>(need, f.cap + f.cap / 2);
@@ -1549,6 +1607,8 @@ struct Run {
         const double ms = elapsed_ms();
         st->kernel_dominance_s += ms * 1e-3;
         st->kernel_total_s += ms * 1e-3;
+        st->kernel_state_dominance_s += ms * 1e-3;
+        st->state_dominance_launches += 1;
         DevFrontier g;
         g.n_nodes = n;
         g.h_off.assign(n + 1, 0);
@@ -1600,6 +1660,8 @@ struct Run {
     }
 
     void fetch_points(const DevFrontier &f, int64_t begin, int64_t count, pmdd_frontier *out) {
+        out->d = d;
+        out->count = count;
         std::vector<int32_t> soa(static_cast<size_t>(count) * d);
         for (int o = 0; o < d && count > 0; ++o) {
             hip_check(hipMemcpyAsync(soa.data() + o * count, f.pts.as<int32_t>() + o * f.cap + begin,
@@ -1607,15 +1669,15 @@ struct Run {
                       "hipMemcpy(result)");
         }
         hip_check(hipStreamSynchronize(s), "result sync");
-        out->d = d;
-        out->count = count;
-        out->points = static_cast<int32_t *>(std::malloc(sizeof(int32_t) * std::max<int64_t>(count * d, 1)));
+        out->points = static_cast<int32_t *>(
+            std::malloc(sizeof(int32_t) * std::max<int64_t>(count * d, 1)));
         if (out->points == nullptr) {
             fail(PMDD_E_NOMEM, "host allocation of the result failed");
         }
+        int32_t *dst = out->points;
         for (int64_t i = 0; i < count; ++i) {
             for (int o = 0; o < d; ++o) {
-                out->points[i * d + o] = soa[o * count + i];
+                dst[i * d + o] = soa[o * count + i];
             }
         }
     }
@@ -2049,6 +2111,7 @@ struct Run {
             st->filter_launches += sorted_filters;
             st->filter_candidates += ytotal;
             st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + btotal);
+            st->join_bytes += 4 * static_cast<int64_t>(d) * batch_inputs;
             if (btotal == 0) {
                 continue;
             }
@@ -2148,6 +2211,7 @@ struct Run {
             r_count = total_new;
             sample_memory();
         }
+        st->join_bytes += 4 * static_cast<int64_t>(d) * r_count;
         fetch_points(R, 0, r_count, out);
         count_layer(0, r_count);
     }
@@ -2217,6 +2281,7 @@ int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmd
     AcctScope scope(&ctx->acct);
     if (!ctx->in_call) {
         ctx->acct.reset_peak();
+        ctx->acct.begin_call();
     }
     {
         Run run(ctx, dd, *opts, st);
@@ -2232,6 +2297,7 @@ int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmd
     // the diagram counts too when it was uploaded outside this call
     const int64_t diagram = dd->arena.acct == &ctx->acct ? 0 : static_cast<int64_t>(dd->arena.bytes);
     st->ctx_mem_peak_bytes = ctx->acct.peak.load(std::memory_order_relaxed) + diagram;
+    st->mem_need_peak_bytes = ctx->acct.need_peak + diagram;
     return PMDD_OK;
 }
 
@@ -2372,6 +2438,7 @@ int pmdd_enumerate(pmdd_ctx *ctx, const pmdd_dd_view *view, const pmdd_opts *opt
         // the diagram's device arena is charged to the context for this call
         AcctScope scope(&ctx->acct);
         ctx->acct.reset_peak();
+        ctx->acct.begin_call();
         struct InCall {
             pmdd_ctx *c;
             explicit InCall(pmdd_ctx *x) : c(x) { c->in_call = true; }
@@ -2529,14 +2596,16 @@ struct pmdd_server {
         return r;
     }
 
-    // Whether idle `slot` may start the head of the queue now.
+    // Whether idle `slot` may start the head of the queue now: its estimate
+    // (or the buffers this slot keeps, when larger) fits the budget next to
+    // every other slot's reservation.  A request queued again after running
+    // out of memory waits until no other slot runs or keeps buffers.
     bool admits(int slot, const ServerJob &head) const {
         const int64_t others = reserved_except(slot);
         if (head.exclusive) {
             return running == 0 && others == 0;
         }
-        const int64_t need = std::max(head.estimate, reserve[slot]);
-        return running == 0 || others + need <= budget;
+        return others + std::max(head.estimate, reserve[slot]) <= budget;
     }
 
     // Asks every idle slot that keeps device buffers to release them (the
@@ -2585,8 +2654,8 @@ struct pmdd_server {
                     blocked_ticket = job->ticket;
                     info.memory_waits += 1;
                 }
-                if (job->exclusive && reserve[slot] > 0) {
-                    trim_slot(slot, lk); // this slot's own kept buffers
+                if (reserve[slot] > 0 && (job->exclusive || reserve[slot] > job->estimate)) {
+                    trim_slot(slot, lk); // this slot's own kept buffers are in the way
                 }
                 ask_trims(slot);
                 cv_work.notify_all();

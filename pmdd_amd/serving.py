@@ -212,7 +212,7 @@ def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots
                     errors[i] = (res.code, res.message)
                 else:
                     points[i] = len(res.points)
-                    mem[i] = (res.stats["ctx_mem_peak_bytes"], res.stats["mem_estimate_bytes"])
+                    mem[i] = (res.stats["mem_need_peak_bytes"], res.stats["mem_estimate_bytes"])
                     if results is not None:
                         results[i] = res.points
                     if stats is not None:
