@@ -244,6 +244,33 @@ int pmdd_enumerate_resident(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *o
                             pmdd_frontier *out, pmdd_stats *stats, char *err, size_t errlen);
 void pmdd_resident_free(pmdd_resident *dd);
 
+/* Device bytes of the scratch array of the no-allocation entry point below:
+ * each GPU the library runs on holds one such array for the life of the
+ * process. */
+#define PMDD_NOALLOC_SCRATCH_BYTES (128LL << 20)
+
+/* No-allocation variant of pmdd_enumerate_resident.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The call makes no hipMalloc, hipFree or pinned host
+ * allocation: every device buffer the enumeration needs beyond its input
+ * (the resident diagram) is a block of the device's static scratch array
+ * (PMDD_NOALLOC_SCRATCH_BYTES), buffers the context already holds are used
+ * when large enough, and blocks the context stops using are freed by its
+ * next allocating call.  The frontier (unsorted, row-major, d int32 per
+ * point) goes to the caller's host buffer points[capacity * d]; *count
+ * receives its size.
+ *   PMDD_E_INVALID      capacity < *count (*count is set; call again with more)
+ *   PMDD_E_UNSUPPORTED  the diagram's own tables exceed the scratch array
+ *   PMDD_E_NOMEM        the enumeration outgrew the scratch array mid-run
+ * Every call on one GPU shares that GPU's array, which keeps the last call's
+ * contents: two no-allocation calls that overlap on one GPU (from different
+ * contexts or threads) race; make them one at a time per device.  Adaptive
+ * coupling batches (max_prod 0) stay at 512K products in this call. */
+int pmdd_enumerate_resident_noalloc(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts,
+                                    int32_t *points, int64_t capacity, int64_t *count,
+                                    pmdd_stats *stats, char *err, size_t errlen);
+
 void pmdd_frontier_free(pmdd_frontier *f);
 
 /* Engine build identity, e.g. "pmdd-engine gfx950 d=1..8". */

@@ -27,6 +27,7 @@ PMDD_DD_BDD = 0
 PMDD_DD_MDD = 1
 
 DEFAULT_MAX_CAND = 20_000_000
+NOALLOC_SCRATCH_BYTES = 128 << 20
 DEFAULT_MAX_PROD = 625_000
 MAX_PROD_ADAPTIVE = 0
 
@@ -172,6 +173,7 @@ ENGINE_SYMBOLS = [
     "pmdd_upload",
     "pmdd_enumerate_resident",
     "pmdd_resident_free",
+    "pmdd_enumerate_resident_noalloc",
     "pmdd_frontier_free",
     "pmdd_engine_version",
     "pmdd_server_create",
@@ -265,6 +267,10 @@ def engine_lib() -> C.CDLL:
         lib.pmdd_enumerate_resident.restype = C.c_int
         lib.pmdd_resident_free.argtypes = [C.c_void_p]
         lib.pmdd_resident_free.restype = None
+        lib.pmdd_enumerate_resident_noalloc.argtypes = [
+            C.c_void_p, C.c_void_p, C.POINTER(Opts), _i32p, C.c_int64, C.POINTER(C.c_int64),
+            C.POINTER(Stats)] + err
+        lib.pmdd_enumerate_resident_noalloc.restype = C.c_int
         lib.pmdd_frontier_free.argtypes = [C.POINTER(Frontier)]
         lib.pmdd_frontier_free.restype = None
         lib.pmdd_engine_version.argtypes = []

@@ -41,6 +41,14 @@
 using pmdd_k::ChunkDesc;
 using pmdd_k::kBlock;
 
+// Scratch array of the no-allocation entry point
+// (pmdd_enumerate_resident_noalloc): every GPU the library runs on holds one
+// copy, at its own address, for the life of the process.  Host code never
+// passes this name to a kernel or a copy: it is a host-side stand-in, and the
+// device address comes from hipGetSymbolAddress (scratch_address below).
+constexpr size_t kScratchBytes = static_cast<size_t>(PMDD_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 struct EngineError {
@@ -102,11 +110,33 @@ struct AcctScope {
     AcctScope &operator=(const AcctScope &) = delete;
 };
 
+// No-allocation calls (pmdd_enumerate_resident_noalloc): every device buffer
+// the call grows is carved from the device's scratch array instead of
+// hipMalloc'd; allocations a buffer held before are not freed during the
+// call but handed to the context's deferred list.
+struct ScratchArena {
+    char *base = nullptr; // device address of this device's scratch array
+    size_t cap = 0;
+    size_t used = 0;
+    std::vector<void *> *deferred = nullptr; // hipMalloc'd blocks to free later
+};
+static thread_local ScratchArena *t_arena = nullptr;
+
+// Frees a hipMalloc'd block, or defers it during a no-allocation call.
+inline void free_block(void *p) {
+    if (t_arena != nullptr && t_arena->deferred != nullptr) {
+        t_arena->deferred->push_back(p);
+    } else {
+        (void)hipFree(p);
+    }
+}
+
 // Device buffer; grows on demand, contents not kept unless asked.
 
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool in_scratch = false; // carved from the scratch array: never freed
     MemAcct *acct = nullptr;
     MemAcct *need_acct = nullptr; // account of the call that last asked for this buffer
     int64_t epoch = -1;           // that call
@@ -126,20 +156,22 @@ struct DevBuf {
     void take(DevBuf &o) noexcept {
         p = o.p;
         bytes = o.bytes;
+        in_scratch = o.in_scratch;
         acct = o.acct;
         need_acct = o.need_acct;
         epoch = o.epoch;
         call_need = o.call_need;
         o.p = nullptr;
         o.bytes = 0;
+        o.in_scratch = false;
         o.acct = nullptr;
         o.need_acct = nullptr;
         o.epoch = -1;
         o.call_need = 0;
     }
     void release() {
-        if (p != nullptr) {
-            (void)hipFree(p);
+        if (p != nullptr && !in_scratch) {
+            free_block(p);
             if (acct != nullptr) {
                 acct->sub(static_cast<int64_t>(bytes));
             }
@@ -149,15 +181,28 @@ struct DevBuf {
         }
         p = nullptr;
         bytes = 0;
+        in_scratch = false;
         acct = nullptr;
         need_acct = nullptr;
         epoch = -1;
         call_need = 0;
     }
+    // Forgets a scratch-array block (nothing to free).
+    void drop_scratch() {
+        if (in_scratch) {
+            p = nullptr;
+            bytes = 0;
+            in_scratch = false;
+            need_acct = nullptr;
+            epoch = -1;
+            call_need = 0;
+        }
+    }
     // Exchanges the allocations of two buffers (their accounts go with them).
     void swap(DevBuf &o) noexcept {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
+        std::swap(in_scratch, o.in_scratch);
         std::swap(acct, o.acct);
         std::swap(need_acct, o.need_acct);
         std::swap(epoch, o.epoch);
@@ -185,6 +230,22 @@ struct DevBuf {
     // by a widening layer is not reallocated on every layer.
     void ensure(size_t need, const char *what) {
         if (need <= bytes && p != nullptr) {
+            note_need(need);
+            return;
+        }
+        if (t_arena != nullptr) {
+            // no-allocation call: a block of the scratch array (256-aligned)
+            const size_t nb = (std::max<size_t>(need, 256) + 255) & ~size_t(255);
+            if (t_arena->used + nb > t_arena->cap) {
+                fail(PMDD_E_NOMEM, std::string(what) + ": the no-allocation scratch array (" +
+                                       std::to_string(t_arena->cap >> 20) +
+                                       " MiB) is exhausted; use pmdd_enumerate_resident");
+            }
+            release();
+            p = t_arena->base + t_arena->used;
+            t_arena->used += nb;
+            bytes = nb;
+            in_scratch = true;
             note_need(need);
             return;
         }
@@ -298,6 +359,10 @@ struct pmdd_ctx {
     Staging staging;
     // frontier buffers handed from one enumeration to the next (FrontierLease)
     std::vector<DevFrontier> frontier_pool;
+    // this device's scratch array (no-allocation calls) and the hipMalloc'd
+    // blocks such a call stopped using, freed by the next allocating call
+    char *scratch = nullptr;
+    std::vector<void *> deferred;
     // device bytes held by this context's buffers
     MemAcct acct;
     bool in_call = false; // inside a one-shot pmdd_enumerate (its peak includes the upload)
@@ -324,11 +389,19 @@ struct pmdd_ctx {
             f(fr.pts);
         }
     }
+    // Frees the blocks a no-allocation call stopped using (stream idle).
+    void free_deferred() {
+        for (void *p : deferred) {
+            (void)hipFree(p);
+        }
+        deferred.clear();
+    }
     // Releases every device buffer the context keeps between calls (the
     // stream must be idle).
     void trim() {
         for_each_buffer([](DevBuf &b) { b.release(); });
         frontier_pool.clear();
+        free_deferred();
     }
 };
 
@@ -901,11 +974,16 @@ inline int ceil_log2(int64_t x) {
 
 // A pinned region of n bytes of the context's staging arena; when the arena
 // is full, waits for the stream (every region is then free) and grows it.
+// In a no-allocation call the arena is not grown: nullptr when it has no
+// room, and the caller copies from or to its own (pageable) memory instead.
 inline void *staging_take(pmdd_ctx *ctx, size_t n) {
     void *p = ctx->staging.take(n);
     if (p == nullptr) {
         hip_check(hipStreamSynchronize(ctx->stream), "staging sync");
         if (n + 256 > ctx->staging.cap) {
+            if (t_arena != nullptr) {
+                return nullptr;
+            }
             ctx->staging.grow(n + 256);
         } else {
             ctx->staging.reset();
@@ -924,6 +1002,13 @@ void h2d(pmdd_ctx *ctx, DevBuf &b, const std::vector<T> &v, const char *what) {
     if (!v.empty()) {
         const size_t bytes = v.size() * sizeof(T);
         void *p = staging_take(ctx, bytes);
+        if (p == nullptr) {
+            // straight from the vector, which must outlive the copy
+            hip_check(hipMemcpyAsync(b.p, v.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+                      what);
+            hip_check(hipStreamSynchronize(ctx->stream), what);
+            return;
+        }
         std::memcpy(p, v.data(), bytes);
         hip_check(hipMemcpyAsync(b.p, p, bytes, hipMemcpyHostToDevice, ctx->stream), what);
     }
@@ -936,11 +1021,12 @@ void d2h(pmdd_ctx *ctx, std::vector<T> &v, const DevBuf &b, size_t n, const char
     void *p = nullptr;
     if (n > 0) {
         p = staging_take(ctx, n * sizeof(T));
-        hip_check(hipMemcpyAsync(p, b.p, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
+        hip_check(hipMemcpyAsync(p != nullptr ? p : static_cast<void *>(v.data()), b.p,
+                                 n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
                   what);
     }
     hip_check(hipStreamSynchronize(ctx->stream), what);
-    if (n > 0) {
+    if (n > 0 && p != nullptr) {
         std::memcpy(v.data(), p, n * sizeof(T));
     }
     ctx->staging.reset();
@@ -1659,9 +1745,22 @@ struct Run {
         st->work_frontier_peak_points = std::max(st->work_frontier_peak_points, surv);
     }
 
+    // Caller-owned output (no-allocation calls): the frontier is written
+    // there instead of a malloc'd array.
+    int32_t *user_pts = nullptr;
+    int64_t user_cap = 0;
+    int64_t *user_count = nullptr;
+
     void fetch_points(const DevFrontier &f, int64_t begin, int64_t count, pmdd_frontier *out) {
         out->d = d;
         out->count = count;
+        if (user_count != nullptr) {
+            *user_count = count;
+            if (count > user_cap) {
+                fail(PMDD_E_INVALID, "the output buffer holds " + std::to_string(user_cap) +
+                                         " points; the frontier has " + std::to_string(count));
+            }
+        }
         std::vector<int32_t> soa(static_cast<size_t>(count) * d);
         for (int o = 0; o < d && count > 0; ++o) {
             hip_check(hipMemcpyAsync(soa.data() + o * count, f.pts.as<int32_t>() + o * f.cap + begin,
@@ -1669,12 +1768,15 @@ struct Run {
                       "hipMemcpy(result)");
         }
         hip_check(hipStreamSynchronize(s), "result sync");
-        out->points = static_cast<int32_t *>(
-            std::malloc(sizeof(int32_t) * std::max<int64_t>(count * d, 1)));
-        if (out->points == nullptr) {
-            fail(PMDD_E_NOMEM, "host allocation of the result failed");
+        int32_t *dst = user_pts;
+        if (user_count == nullptr) {
+            out->points = static_cast<int32_t *>(
+                std::malloc(sizeof(int32_t) * std::max<int64_t>(count * d, 1)));
+            if (out->points == nullptr) {
+                fail(PMDD_E_NOMEM, "host allocation of the result failed");
+            }
+            dst = out->points;
         }
-        int32_t *dst = out->points;
         for (int64_t i = 0; i < count; ++i) {
             for (int o = 0; o < d; ++o) {
                 dst[i * d + o] = soa[o * count + i];
@@ -2258,8 +2360,15 @@ void reset_stats(pmdd_stats *st) {
     }
 }
 
+// Where a no-allocation call wants its frontier (host memory of the caller).
+struct UserOut {
+    int32_t *points;
+    int64_t capacity;
+    int64_t *count;
+};
+
 int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmdd_frontier *out,
-                    pmdd_stats *stats) {
+                    pmdd_stats *stats, const UserOut *user = nullptr) {
     if (opts == nullptr || out == nullptr) {
         fail(PMDD_E_INVALID, "NULL options or output");
     }
@@ -2283,8 +2392,17 @@ int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmd
         ctx->acct.reset_peak();
         ctx->acct.begin_call();
     }
+    if (t_arena == nullptr && !ctx->deferred.empty()) {
+        hip_check(hipStreamSynchronize(ctx->stream), "deferred free sync");
+        ctx->free_deferred();
+    }
     {
         Run run(ctx, dd, *opts, st);
+        if (user != nullptr) {
+            run.user_pts = user->points;
+            run.user_cap = user->capacity;
+            run.user_count = user->count;
+        }
         if (opts->method == 1) {
             run.topdown(out);
         } else if (opts->method == 2) {
@@ -2336,6 +2454,25 @@ int pmdd_device_count(void) {
     return n;
 }
 
+// Device address of this device's scratch array, resolved once per device
+// with that device current.
+static char *scratch_address(int device) {
+    static std::mutex mu;
+    static std::unordered_map<int, char *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        return it->second;
+    }
+    void *ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (ptr == nullptr) {
+        fail(PMDD_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    }
+    addr[device] = static_cast<char *>(ptr);
+    return static_cast<char *>(ptr);
+}
+
 int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
     return guarded(err, errlen, [&]() -> int {
         if (out == nullptr) {
@@ -2357,6 +2494,7 @@ int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
         std::unique_ptr<pmdd_ctx> c(new pmdd_ctx());
         c->device = device;
         hip_check(hipSetDevice(device), "hipSetDevice");
+        c->scratch = scratch_address(device);
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&c->ev_a), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
@@ -2426,6 +2564,61 @@ int pmdd_enumerate_resident(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *o
             fail(PMDD_E_INVALID, "pmdd_enumerate_resident: NULL argument");
         }
         return run_enumeration(ctx, dd, opts, out, stats);
+    });
+}
+
+int pmdd_enumerate_resident_noalloc(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts,
+                                    int32_t *points, int64_t capacity, int64_t *count,
+                                    pmdd_stats *stats, char *err, size_t errlen) {
+    return guarded(err, errlen, [&]() -> int {
+        if (ctx == nullptr || dd == nullptr || opts == nullptr || count == nullptr ||
+            capacity < 0 || (capacity > 0 && points == nullptr)) {
+            fail(PMDD_E_INVALID, "pmdd_enumerate_resident_noalloc: NULL or negative argument");
+        }
+        *count = 0;
+        if (ctx->scratch == nullptr) {
+            fail(PMDD_E_HIP, "no scratch array address for this device");
+        }
+        // the part of the need fixed by the diagram: arc run offsets and the
+        // node offsets of the layer frontiers of the widest layer
+        int64_t fixed = 0;
+        for (int l = 0; l < dd->L; ++l) {
+            const int64_t e = std::max(dd->td[l].n_arcs, dd->bu[l].n_arcs);
+            fixed = std::max(fixed, 8 * e + 5 * 8 * (dd->layer_size[l] + 1));
+        }
+        if (fixed > static_cast<int64_t>(kScratchBytes)) {
+            fail(PMDD_E_UNSUPPORTED,
+                 "the diagram's tables alone need " + std::to_string(fixed) +
+                     " bytes of scratch, more than the " + std::to_string(kScratchBytes >> 20) +
+                     " MiB the no-allocation entry point has; use pmdd_enumerate_resident");
+        }
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        ScratchArena arena;
+        arena.base = ctx->scratch;
+        arena.cap = kScratchBytes;
+        arena.deferred = &ctx->deferred;
+        // for the call: buffers come from the scratch array, adaptive
+        // coupling batches stay at kServeJoinCap products; afterwards the
+        // context forgets every scratch block, so no later call of this
+        // context touches the array
+        struct Scope {
+            pmdd_ctx *c;
+            ScratchArena *prev;
+            int64_t join_cap;
+            Scope(pmdd_ctx *x, ScratchArena *a) : c(x), prev(t_arena), join_cap(x->join_cap) {
+                t_arena = a;
+                c->join_cap = kServeJoinCap;
+            }
+            ~Scope() {
+                (void)hipStreamSynchronize(c->stream);
+                c->for_each_buffer([](DevBuf &b) { b.drop_scratch(); });
+                c->join_cap = join_cap;
+                t_arena = prev;
+            }
+        } scope(ctx, &arena);
+        pmdd_frontier fr{};
+        const UserOut user{points, capacity, count};
+        return run_enumeration(ctx, dd, opts, &fr, stats, &user);
     });
 }
 

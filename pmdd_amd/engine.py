@@ -159,6 +159,33 @@ class ResidentDD:
         pts = _take_frontier(eng._lib, fr, sort)
         return EnumerationResult(pts, _stats_dict(st, self.num_layers, None, None))
 
+    def enumerate_noalloc(self, capacity: int, method: int = 1, state_dominance: int = 0,
+                          max_cand: int = abi.DEFAULT_MAX_CAND,
+                          max_prod: int = abi.MAX_PROD_ADAPTIVE):
+        """pmdd_enumerate_resident_noalloc: the same enumeration with every
+        device buffer taken from the engine's static scratch array.  Returns
+        (points sorted lexicographically, stats); PmddError with code
+        PMDD_E_INVALID when the frontier holds more than `capacity` points
+        (its size is then the error's `count`)."""
+        eng = self.engine
+        o = eng._opts(method, self.problem_type, state_dominance, max_cand, max_prod, False)
+        buf = np.zeros(max(capacity, 1) * self.d, dtype=np.int32)
+        cnt = C.c_int64(0)
+        st = abi.Stats()
+        err = C.create_string_buffer(1024)
+        rc = eng._lib.pmdd_enumerate_resident_noalloc(
+            eng._h, self._h, C.byref(o), buf.ctypes.data_as(C.POINTER(C.c_int32)),
+            int(capacity), C.byref(cnt), C.byref(st), err, len(err))
+        if rc != abi.PMDD_OK:
+            e = PmddError(rc, err.value.decode())
+            e.count = int(cnt.value)
+            raise e
+        n = int(cnt.value)
+        pts = buf[:n * self.d].reshape(n, self.d)
+        if n > 1:
+            pts = pts[np.lexsort(pts.T[::-1])]
+        return pts, _stats_dict(st, self.num_layers, None, None)
+
     def close(self) -> None:
         if self._h is not None:
             self.engine._lib.pmdd_resident_free(self._h)
