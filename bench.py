@@ -175,48 +175,51 @@ def serving_leg(tmpdir, requests, slots, method, rate_factors, deadline):
     out = {"requests": requests, "method": method, "slots": slots,
            "mix": "60% knapsack n~U[20,70], 25% set packing n~U[60,150] m=n/5, "
                   "15% TSP n~U[8,11]; d~U{3,4,5}; seeded SplitMix64"}
-    res_plain, res_batched = {}, {}
+    # the three paths: one request at a time; `slots` worker slots, each its
+    # own context and stream; the same slots plus the packers (method-1
+    # requests on small diagrams share packed layers)
+    paths_cfg = [("plain", dict(slots=1)), ("batched", dict(slots=slots)),
+                 ("packed", dict(slots=slots, pack=True))]
     t0 = time.perf_counter()
     closed = [0.0] * requests
     progress = 250 if requests > 250 else 0
-    plain = serving.run_stream(paths, specs, closed, slots=1, method=method, results=res_plain,
-                               progress=progress)
-    log(f"serving: plain closed loop {plain['inst_per_s']:.2f} inst/s")
-    batched = serving.run_stream(paths, specs, closed, slots=slots, method=method,
-                                 results=res_batched, progress=progress)
-    log(f"serving: batched closed loop {batched['inst_per_s']:.2f} inst/s")
-    same = all(i in res_plain and i in res_batched and
-               res_plain[i].shape == res_batched[i].shape and
-               bool((res_plain[i] == res_batched[i]).all()) for i in range(requests))
-    cap = plain["inst_per_s"]
-    out["closed_loop"] = {"plain": {k: plain[k] for k in keys},
-                          "batched": {k: batched[k] for k in keys}}
+    res, closed_out = {}, {}
+    for name, kw in paths_cfg:
+        res[name] = {}
+        r = serving.run_stream(paths, specs, closed, method=method, results=res[name],
+                               progress=progress, **kw)
+        closed_out[name] = {k: r[k] for k in keys}
+        log(f"serving: {name} closed loop {r['inst_per_s']:.2f} inst/s")
+    same = all(i in res[p] and res[p][i].shape == res["plain"][i].shape and
+               bool((res[p][i] == res["plain"][i]).all())
+               for p in ("batched", "packed") for i in range(requests) if i in res["plain"])
+    cap = closed_out["plain"]["inst_per_s"]
+    out["closed_loop"] = closed_out
     sweep = []
     for f in rate_factors:
         lam = f * cap
-        # both paths replay n requests at rate lam: about 2 n / lam seconds
+        # the three paths replay n requests at rate lam: about 3 n / lam seconds
         left = deadline - time.perf_counter()
-        n = min(requests, int(0.85 * left * lam / 2))
+        n = min(requests, int(0.85 * left * lam / 3))
         if n < 50:
             log(f"serving: no time left for Poisson arrivals at {f}x")
             continue
         arrivals = serving.poisson_arrivals(n, lam)
-        plain_o = serving.run_stream(paths[:n], specs[:n], arrivals, slots=1, method=method,
-                                     progress=progress)
-        log(f"serving: plain open loop at {f}x capacity, {n} requests done")
-        batched_o = serving.run_stream(paths[:n], specs[:n], arrivals, slots=slots,
-                                       method=method, progress=progress)
-        log(f"serving: batched open loop at {f}x capacity, {n} requests done")
-        entry = {"rate_factor": f, "rate_per_s": lam, "requests": n,
-                 "plain": {k: plain_o[k] for k in keys},
-                 "batched": {k: batched_o[k] for k in keys}}
+        entry = {"rate_factor": f, "rate_per_s": lam, "requests": n}
+        for name, kw in paths_cfg:
+            r = serving.run_stream(paths[:n], specs[:n], arrivals, method=method,
+                                   progress=progress, **kw)
+            entry[name] = {k: r[k] for k in keys}
+            log(f"serving: {name} open loop at {f}x capacity, {n} requests done")
         sweep.append(entry)
         if f == 1.0:
             out["poisson"] = {k: entry[k] for k in ("rate_per_s", "requests", "plain",
-                                                    "batched")}
+                                                    "batched", "packed")}
     out["poisson_sweep"] = sweep
-    out["batched_over_plain_inst_per_s"] = batched["inst_per_s"] / max(plain["inst_per_s"], 1e-12)
-    out["bit_exact_batched_vs_plain"] = same
+    out["batched_over_plain_inst_per_s"] = (closed_out["batched"]["inst_per_s"] /
+                                            max(cap, 1e-12))
+    out["packed_over_plain_inst_per_s"] = closed_out["packed"]["inst_per_s"] / max(cap, 1e-12)
+    out["bit_exact_batched_and_packed_vs_plain"] = same
     out["wall_s"] = time.perf_counter() - t0
     return out
 
@@ -484,7 +487,7 @@ def main():
         deadline = T_START + args.time_budget_s
         srv = serving_leg(tmp, args.serving_requests, args.serving_slots, args.serving_method,
                           rates, deadline)
-        if not srv["bit_exact_batched_vs_plain"]:
+        if not srv["bit_exact_batched_and_packed_vs_plain"]:
             raise SystemExit("serving: batched and plain frontiers differ")
 
     line = {

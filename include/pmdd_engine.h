@@ -173,6 +173,9 @@ typedef struct pmdd_stats {
      * cutset read once per batch it takes part in, the final frontier
      * written once, 4*d*(sum over batches of (#TD rows + #BU columns) + #R). */
     int64_t join_bytes;
+    /* Replays of the captured graph of a queued expansion pass (packed
+     * serving layers), one per pass. */
+    int64_t graph_launches;
     /* Coupling batches (method 3) and the most products any one held. */
     int64_t join_batches;
     int64_t join_batch_peak_products;
@@ -290,6 +293,17 @@ typedef struct pmdd_server_config {
     int32_t device;
     int32_t slots;         /* concurrent requests (>= 1); 1 = one at a time */
     int64_t memory_budget; /* bytes of HBM for admitted requests; 0 = 80% of free memory */
+    /* Packed serving (1) or not (0).  When on, requests that run method 1
+     * without state dominance on diagrams of at most 8,192 nodes per layer
+     * and 1M nodes in all go to a packer of their objective count instead of
+     * a slot: one host thread and one stream that expands the current layers
+     * of up to pack_inflight in-flight requests as one packed layer per
+     * step, so the nodes of many instances (at different layers) share every
+     * launch.  Requests join a packer at any step as room and memory free up
+     * and leave it at their terminal layer.  The other requests use the
+     * slots. */
+    int32_t pack;
+    int32_t pack_inflight; /* requests per packer (0 = 32) */
 } pmdd_server_config;
 
 typedef struct pmdd_server pmdd_server;
@@ -337,6 +351,9 @@ typedef struct pmdd_server_info {
     int64_t memory_waits;   /* times the head of the queue waited for memory with a slot free */
     int64_t trims;          /* idle slots that released their kept buffers for the head */
     int64_t nomem_retries;  /* requests queued again to run alone after PMDD_E_NOMEM */
+    int64_t packed;         /* requests served by a packer */
+    int64_t pack_steps;     /* packed layers expanded */
+    int64_t pack_requests_per_step_x100; /* 100 x the mean requests in a packed step */
     int32_t running_peak;   /* most requests running at once */
     int32_t slots;
 } pmdd_server_info;

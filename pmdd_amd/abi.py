@@ -106,6 +106,7 @@ class Stats(C.Structure):
         ("kernel_state_dominance_s", C.c_double),
         ("state_dominance_launches", C.c_int64),
         ("join_bytes", C.c_int64),
+        ("graph_launches", C.c_int64),
         ("join_batches", C.c_int64),
         ("join_batch_peak_products", C.c_int64),
         ("ctx_mem_peak_bytes", C.c_int64),
@@ -126,6 +127,8 @@ class ServerConfig(C.Structure):
         ("device", C.c_int32),
         ("slots", C.c_int32),
         ("memory_budget", C.c_int64),
+        ("pack", C.c_int32),
+        ("pack_inflight", C.c_int32),
     ]
 
 
@@ -138,6 +141,9 @@ class ServerInfo(C.Structure):
         ("memory_waits", C.c_int64),
         ("trims", C.c_int64),
         ("nomem_retries", C.c_int64),
+        ("packed", C.c_int64),
+        ("pack_steps", C.c_int64),
+        ("pack_requests_per_step_x100", C.c_int64),
         ("running_peak", C.c_int32),
         ("slots", C.c_int32),
     ]

@@ -366,6 +366,13 @@ struct pmdd_ctx {
     // device bytes held by this context's buffers
     MemAcct acct;
     bool in_call = false; // inside a one-shot pmdd_enumerate (its peak includes the upload)
+    // queued passes (packed layers): the tier descriptors, the queue heads,
+    // and the captured graph of a pass's launches with what it was captured
+    // for (objective count, the two buffers and the pair-test counter)
+    DevBuf qtier, qhead;
+    hipGraphExec_t pass_graph = nullptr;
+    const void *graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int64_t graph_replays = 0;
     // largest adaptive coupling batch (products); 0: kJoinBatchCap.  A server
     // slot sets it so that a request's join stays within its admission
     // estimate.
@@ -381,7 +388,7 @@ struct pmdd_ctx {
         for (DevBuf *b : {&run_off, &which, &which_a, &which_b, &which_c, &which_s, &which_m,
                           &surv_pts, &alive, &counter, &join_meta, &keep_a, &keep_b, &sort_a,
                           &sort_b, &big_base, &big_node, &run_start, &grp_run, &join_idx, &join_x,
-                          &join_y, &join_ctr, &front_count, &merge_a, &merge_b}) {
+                          &join_y, &join_ctr, &front_count, &merge_a, &merge_b, &qtier, &qhead}) {
             f(*b);
         }
         for (DevFrontier &fr : frontier_pool) {
@@ -398,7 +405,17 @@ struct pmdd_ctx {
     }
     // Releases every device buffer the context keeps between calls (the
     // stream must be idle).
+    void drop_graph() {
+        if (pass_graph != nullptr) {
+            (void)hipGraphExecDestroy(pass_graph);
+            pass_graph = nullptr;
+        }
+        for (const void *&k : graph_key) {
+            k = nullptr;
+        }
+    }
     void trim() {
+        drop_graph();
         for_each_buffer([](DevBuf &b) { b.release(); });
         frontier_pool.clear();
         free_deferred();
@@ -819,6 +836,13 @@ struct Launch {
     void (*refill_flags)(dim3, hipStream_t, const int32_t *, int64_t,
                          const unsigned long long *, const int32_t *, int64_t, int64_t, int64_t,
                          uint8_t *, unsigned long long *);
+    // persistent chunk filters of a queued pass, one per tier (nodes of at
+    // most kSmallNode candidates, of at most mid_cap, of at most CH), with
+    // the grid that keeps every compute unit full for each
+    void (*queue[3])(dim3, hipStream_t, const pmdd_k::QueueTier *, int32_t *,
+                     unsigned long long *);
+    int queue_grid[3];
+    void (*queue_grids)(int *); // fills queue_grid (queries the current device)
 };
 
 // Nodes of at most kSmallNode candidates are filtered by an instantiation with
@@ -925,8 +949,47 @@ template <int D> struct LaunchImpl {
         hipLaunchKernelGGL((pmdd_k::k_stage_chunks<D>), g, dim3(kBlock), 0, s, which, c, src, out,
                            stride, alive, ns, na);
     }
+    template <int C> static constexpr int threads_for() {
+        return C >= 1024 ? kFilterThreadsBig : (C <= 256 ? 128 : pmdd_k::kBlock);
+    }
+    template <int C>
+    static void queue_tier(dim3 g, hipStream_t s, const pmdd_k::QueueTier *t, int32_t *head,
+                           unsigned long long *pt) {
+        constexpr int NT = threads_for<C>();
+        hipLaunchKernelGGL((pmdd_k::k_filter_queue<D, C, NT>), g, dim3(NT), 0, s, t, head, pt);
+    }
+    // Workgroups of k_filter_queue<D, C> resident on the whole device at
+    // once (compute units times the kernel's occupancy per compute unit).
+    template <int C> static int queue_grid_for() {
+        static const int grid = [] {
+            int dev = 0, cus = 0, per_cu = 0;
+            hip_check(hipGetDevice(&dev), "hipGetDevice");
+            hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                      "hipDeviceGetAttribute");
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &per_cu,
+                          reinterpret_cast<const void *>(
+                              &pmdd_k::k_filter_queue<D, C, threads_for<C>()>),
+                          threads_for<C>(), 0),
+                      "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+            return std::max(1, cus) * std::max(1, per_cu);
+        }();
+        return grid;
+    }
     static Launch table() {
         Launch L;
+        constexpr int MID = CH > kMidNode ? kMidNode : CH;
+        L.queue[0] = queue_tier<kSmallNode>;
+        L.queue[1] = queue_tier<MID>;
+        L.queue[2] = queue_tier<CH>;
+        L.queue_grid[0] = 0; // filled by queue_grids on first use (needs the device)
+        L.queue_grid[1] = 0;
+        L.queue_grid[2] = 0;
+        L.queue_grids = [](int *g) {
+            g[0] = queue_grid_for<kSmallNode>();
+            g[1] = queue_grid_for<(CH > kMidNode ? kMidNode : CH)>();
+            g[2] = queue_grid_for<CH>();
+        };
         L.stage = stage;
         L.join_products = join_products;
         L.refill_vs = refill_vs;
@@ -1073,8 +1136,12 @@ struct Run {
     const bool debug_join = std::getenv("PMDD_DEBUG_JOIN") != nullptr;
 
     Run(pmdd_ctx *c, pmdd_resident *r, const pmdd_opts &o, pmdd_stats *stats)
-        : ctx(c), dd(r), opts(o), st(stats), K(launch_for(r->d)), d(r->d), s(c->stream),
-          sd_lease(c, r->d), sd_spare(sd_lease.f) {
+        : Run(c, r, r->d, o, stats) {}
+    // r may be null for a run that only expands layers it is handed (the
+    // serving packer's packed layers)
+    Run(pmdd_ctx *c, pmdd_resident *r, int d_, const pmdd_opts &o, pmdd_stats *stats)
+        : ctx(c), dd(r), opts(o), st(stats), K(launch_for(d_)), d(d_), s(c->stream),
+          sd_lease(c, d_), sd_spare(sd_lease.f) {
         if (opts.max_cand <= 0) {
             opts.max_cand = PMDD_DEFAULT_MAX_CAND;
         }
@@ -1207,6 +1274,69 @@ struct Run {
         return 1;
     }
 
+    // Packed layers (the serving packer): a pass's small-node chunks are
+    // filtered by the persistent queue kernels, launched as one captured
+    // graph that every pass replays.
+    bool queued = false;
+
+    // The three tier lists of a pass (already on the device) through
+    // k_filter_queue: the tiers' descriptors go to ctx->qtier, then the
+    // pass graph (queue heads zeroed by a fill kernel, then one persistent
+    // launch per tier) is replayed, captured first when the buffers it was
+    // captured for have moved.  Returns the number of tier launches.
+    int run_queue_pass(ChunkBufs &cb, const pmdd_k::ExpandSrc &src,
+                       const std::vector<int32_t> &tiny, const std::vector<int32_t> &mid,
+                       const std::vector<int32_t> &full, int32_t *out, int64_t stride,
+                       uint8_t *alive) {
+        if (K.queue_grid[0] == 0) {
+            K.queue_grids(K.queue_grid);
+        }
+        const std::vector<int32_t> *lists[3] = {&tiny, &mid, &full};
+        const DevBuf *bufs[3] = {&ctx->which_s, &ctx->which_m, &ctx->which_a};
+        std::vector<pmdd_k::QueueTier> tiers(3);
+        for (int t = 0; t < 3; ++t) {
+            pmdd_k::QueueTier &q = tiers[t];
+            q.src = src;
+            q.chunks = cb.chunks.as<ChunkDesc>();
+            q.which = lists[t]->empty() ? nullptr : bufs[t]->as<int32_t>();
+            q.count = static_cast<int32_t>(lists[t]->size());
+            q.pad = 0;
+            q.out_pts = out;
+            q.out_stride = stride;
+            q.alive = alive;
+            q.nsurv = cb.nsurv.as<int32_t>();
+            q.nalive = cb.nalive.as<int32_t>();
+        }
+        h2d(ctx, ctx->qtier, tiers, "hipMemcpy(queue tiers)");
+        ctx->qhead.ensure(3 * sizeof(int32_t), "hipMalloc(queue heads)");
+        const void *key[4] = {reinterpret_cast<const void *>(static_cast<intptr_t>(d)),
+                              ctx->qtier.p, ctx->qhead.p, d_tests};
+        if (ctx->pass_graph == nullptr || !std::equal(key, key + 4, ctx->graph_key)) {
+            ctx->drop_graph();
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal),
+                      "hipStreamBeginCapture");
+            hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(1), dim3(64), 0, s,
+                               ctx->qhead.as<uint8_t>(), int64_t(3 * sizeof(int32_t)), uint8_t(0));
+            const pmdd_k::QueueTier *qt = ctx->qtier.as<pmdd_k::QueueTier>();
+            for (int t = 0; t < 3; ++t) {
+                K.queue[t](dim3(static_cast<unsigned>(K.queue_grid[t])), s, qt + t,
+                           ctx->qhead.as<int32_t>() + t, d_tests);
+            }
+            const hipError_t cap = hipGetLastError();
+            hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+            hip_check(cap, "k_filter_queue (capture)");
+            const hipError_t inst = hipGraphInstantiate(&ctx->pass_graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hip_check(inst, "hipGraphInstantiate");
+            std::copy(key, key + 4, ctx->graph_key);
+        }
+        hip_check(hipGraphLaunch(ctx->pass_graph, s), "hipGraphLaunch");
+        ctx->graph_replays += 1;
+        st->graph_launches += 1;
+        return 3;
+    }
+
     // Copy the listed chunks' candidates from a key-ordered source, all alive.
     void run_stage(ChunkBufs &cb, DevBuf &wbuf, const std::vector<int32_t> &which,
                    const pmdd_k::SortedSrc &src, int32_t *out, int64_t stride, uint8_t *alive) {
@@ -1241,20 +1371,29 @@ struct Run {
             return 0;
         }
         const int waves = static_cast<int>(ceil_div(max_count, kCrossWave));
-        // per wave: every chunk of the groups with more than one chunk
-        std::vector<int32_t> lists;
+        // per wave: every chunk of the groups with more than one chunk, in
+        // chunk order (a counting pass, so the cost is chunks + waves)
         std::vector<size_t> start(waves + 1, 0);
+        for (size_t i = 0; i < chunks.size(); ++i) {
+            const ChunkDesc &c = chunks[i];
+            if (c.group_count > 1) {
+                start[(static_cast<int>(i) - c.group_first) / kCrossWave + 1] += 1;
+            }
+        }
         for (int w = 0; w < waves; ++w) {
-            start[w] = lists.size();
+            start[w + 1] += start[w];
+        }
+        std::vector<int32_t> lists(start[waves]);
+        {
+            std::vector<size_t> fill(start.begin(), start.end() - 1);
             for (size_t i = 0; i < chunks.size(); ++i) {
                 const ChunkDesc &c = chunks[i];
-                const int k = static_cast<int>(i) - c.group_first;
-                if (c.group_count > 1 && k / kCrossWave == w) {
-                    lists.push_back(static_cast<int32_t>(i));
+                if (c.group_count > 1) {
+                    lists[fill[(static_cast<int>(i) - c.group_first) / kCrossWave]++] =
+                        static_cast<int32_t>(i);
                 }
             }
         }
-        start[waves] = lists.size();
         h2d(ctx, ctx->which_c, lists, "hipMemcpy(wave lists)");
         // front counts, double-buffered by wave, indexed by a group's first chunk
         const size_t nc = chunks.size();
@@ -1516,12 +1655,21 @@ struct Run {
             }
             // only k_filter_chunks launches between ev_d and ev_c
             hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
-            int filters = run_filter(K.filter_expand_small, cb, ctx->which_s, which_tiny, 0, &src,
+            int filters = 0;
+            if (queued) {
+                filters = run_queue_pass(cb, src, which_tiny, which_mid, which_small,
+                                         ctx->surv_pts.as<int32_t>(), pass_c,
+                                         ctx->alive.as<uint8_t>());
+            } else {
+                filters = run_filter(K.filter_expand_small, cb, ctx->which_s, which_tiny, 0, &src,
                                      ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
-            filters += run_filter(K.filter_expand_mid, cb, ctx->which_m, which_mid, 0, &src,
-                                  ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
-            filters += run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
-                                  ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
+                filters += run_filter(K.filter_expand_mid, cb, ctx->which_m, which_mid, 0, &src,
+                                      ctx->surv_pts.as<int32_t>(), pass_c,
+                                      ctx->alive.as<uint8_t>());
+                filters += run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
+                                      ctx->surv_pts.as<int32_t>(), pass_c,
+                                      ctx->alive.as<uint8_t>());
+            }
             hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
             int cross_waves = 0;
             if (!which_cross.empty()) {
@@ -2758,6 +2906,174 @@ int64_t estimate_bytes(const pmdd_dd_view &v, const pmdd_opts &o) {
     return arena + pass + fronts + join + (int64_t(16) << 20);
 }
 
+// ---------------------------------------------------------------------------
+// Packed serving: one packer per objective count expands the current layers
+// of its in-flight requests as one layer per step (Run::expand on the packed
+// layer, its small-node chunks filtered by the persistent queue kernels in a
+// replayed graph).  Request i's current layer occupies nodes [base_i,
+// base_i + N_i) of the packed frontier; its next layer's arcs are gathered
+// into the packed layer with their parents moved by base_i.  Nodes of
+// requests that have left stay in the packed frontier until the next step
+// drops them (no arc refers to them).
+// ---------------------------------------------------------------------------
+// Packed requests: diagrams of at most kPackWidest nodes per layer and
+// kPackNodes in all.  On the serving stream the wider ones (knapsacks with
+// four or five objectives and n of about 40 or more, the large set-packing
+// diagrams) hold almost all of the work (for the first 100 requests, the 42
+// above 8192 nodes per layer take 99% of the CPU oracle's time): they run
+// best next to each other in the slots, while packing pays for the many
+// light requests whose layers are mostly launch and planning overhead.
+constexpr int64_t kPackWidest = 8192;
+constexpr int64_t kPackNodes = int64_t(1) << 20;
+constexpr int kPackInflight = 32;
+
+struct PackedReq {
+    std::shared_ptr<ServerJob> job;
+    pmdd_resident *dd = nullptr; // its diagram, uploaded when it joined
+    int layer = 0;               // its layer now in the packed frontier
+    int64_t base = 0;            // first node of that layer there
+};
+
+struct Packer {
+    pmdd_ctx *ctx = nullptr;
+    int d = 0;
+    pmdd_stats st{}; // the packer's own totals
+    std::unique_ptr<Run> run;
+    std::unique_ptr<FrontierLease> la, lb;
+    DevFrontier *cur = nullptr, *nxt = nullptr;
+    std::vector<PackedReq> live;
+    LayerCSR layer; // the packed layer (host tables for planning)
+    DevBuf d_off, d_src, d_w, d_parts;
+    std::vector<pmdd_k::PackPart> parts;
+    // packed layers expanded, and requests they carried (read by the
+    // server's other threads)
+    std::atomic<int64_t> steps{0}, stepped{0};
+
+    Packer(pmdd_ctx *c, int d_) : ctx(c), d(d_) { reset(); }
+    ~Packer() { drop(); }
+
+    void drop() {
+        for (PackedReq &r : live) {
+            delete r.dd;
+        }
+        live.clear();
+        run.reset();
+        la.reset();
+        lb.reset();
+    }
+    // Empty packed frontier, no request.
+    void reset() {
+        drop();
+        pmdd_opts o{};
+        o.method = 1;
+        o.max_cand = PMDD_DEFAULT_MAX_CAND;
+        o.max_prod = PMDD_MAX_PROD_ADAPTIVE;
+        la.reset(new FrontierLease(ctx, d));
+        lb.reset(new FrontierLease(ctx, d));
+        cur = &la->f;
+        nxt = &lb->f;
+        cur->n_nodes = 0;
+        cur->h_off.assign(1, 0);
+        run.reset(new Run(ctx, nullptr, d, o, &st));
+        run->queued = true;
+    }
+
+    // Uploads a request's diagram and puts its root layer (frontier {0^d} at
+    // the root) at the end of the packed frontier; upload_offsets follows.
+    void add(const std::shared_ptr<ServerJob> &job) {
+        PackedReq r;
+        r.job = job;
+        r.dd = upload(ctx, &job->view);
+        r.base = cur->n_nodes;
+        const int64_t n0 = r.dd->layer_size[0];
+        const int64_t total = cur->total();
+        for (int64_t v = 0; v < n0; ++v) {
+            cur->h_off.push_back(total + (v >= r.dd->root ? 1 : 0));
+        }
+        cur->n_nodes += n0;
+        grow_points(*cur, d, total + 1, total, ctx->stream);
+        for (int o = 0; o < d; ++o) {
+            hip_check(hipMemsetAsync(cur->pts.as<int32_t>() + o * cur->cap + total, 0, 4,
+                                     ctx->stream),
+                      "hipMemset(seed)");
+        }
+        live.push_back(r);
+    }
+    void upload_offsets_now() { upload_offsets(ctx, *cur); }
+
+    // One packed layer: every live request's next layer.  Requests that
+    // reach their terminal layer get their frontier and move to `done`.
+    void step(std::vector<PackedReq> &done) {
+        int64_t N = 0, E = 0;
+        layer.off.assign(1, 0);
+        layer.src.clear();
+        parts.clear();
+        for (const PackedReq &r : live) {
+            const LayerCSR &c = r.dd->td[r.layer + 1];
+            const int64_t n = r.dd->layer_size[r.layer + 1];
+            for (int64_t v = 1; v <= n; ++v) {
+                layer.off.push_back(E + c.off[v]);
+            }
+            for (int64_t e = 0; e < c.n_arcs; ++e) {
+                layer.src.push_back(static_cast<int32_t>(c.src[e] + r.base));
+            }
+            parts.push_back({c.d_src, c.d_w, c.n_arcs, E, r.base});
+            N += n;
+            E += c.n_arcs;
+        }
+        layer.n_arcs = E;
+        h2d(ctx, d_off, layer.off, "hipMemcpy(packed offsets)");
+        h2d(ctx, d_parts, parts, "hipMemcpy(packed parts)");
+        d_src.ensure(static_cast<size_t>(std::max<int64_t>(E, 1)) * 4, "hipMalloc(packed arcs)");
+        d_w.ensure(static_cast<size_t>(std::max<int64_t>(E, 1)) * 4 * d,
+                   "hipMalloc(packed weights)");
+        if (E > 0) {
+            hipLaunchKernelGGL(pmdd_k::k_pack_arcs,
+                               dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kBlock), 4096))),
+                               dim3(kBlock), 0, ctx->stream, d_parts.as<pmdd_k::PackPart>(),
+                               static_cast<int32_t>(parts.size()), E, d, d_src.as<int32_t>(),
+                               d_w.as<int32_t>());
+            hip_check(hipGetLastError(), "k_pack_arcs");
+        }
+        layer.d_off = d_off.as<int64_t>();
+        layer.d_src = d_src.as<int32_t>();
+        layer.d_w = d_w.as<int32_t>();
+        int64_t cand = 0;
+        run->expand(*cur, layer, N, *nxt, &cand, nullptr);
+        steps += 1;
+        stepped += static_cast<int64_t>(live.size());
+        int64_t base = 0;
+        std::vector<PackedReq> keep;
+        for (PackedReq &r : live) {
+            r.layer += 1;
+            r.base = base;
+            base += r.dd->layer_size[r.layer];
+            if (r.layer == r.dd->L - 1) {
+                const int64_t t = r.base + r.dd->terminal;
+                run->fetch_points(*nxt, nxt->h_off[t], nxt->size(t), &r.job->fr);
+                done.push_back(r);
+            } else {
+                keep.push_back(r);
+            }
+        }
+        live.swap(keep);
+        std::swap(cur, nxt);
+    }
+};
+
+// Whether a request goes to a packer: method 1 without state dominance, on a
+// diagram small enough to share packed layers with others.
+bool packable(const pmdd_dd_view &v, const pmdd_opts &o) {
+    if (o.method != 1 || o.state_dominance != 0 || o.count_pair_tests != 0) {
+        return false;
+    }
+    int64_t widest = 0;
+    for (int l = 0; l < v.num_layers; ++l) {
+        widest = std::max<int64_t>(widest, v.layer_off[l + 1] - v.layer_off[l]);
+    }
+    return widest <= kPackWidest && v.layer_off[v.num_layers] <= kPackNodes;
+}
+
 } // namespace
 
 struct pmdd_server {
@@ -2780,13 +3096,175 @@ struct pmdd_server {
     std::vector<char> busy, trim_asked;
     int64_t blocked_ticket = 0; // head of the queue last seen waiting for memory
     pmdd_server_info info{};
+    // packed serving: a packer (its own context, stream and thread) per
+    // objective count, created on first use, with its queue of requests;
+    // the estimates of the requests the packers hold are reserved
+    bool pack = false;
+    int pack_inflight = kPackInflight;
+    std::unique_ptr<Packer> packers[9];
+    pmdd_ctx *pack_ctx[9] = {};
+    std::deque<std::shared_ptr<ServerJob>> pack_queue[9];
+    std::vector<std::thread> pack_threads;
+    int64_t pack_reserved = 0;
 
     int64_t reserved_except(int slot) const {
-        int64_t r = 0;
+        int64_t r = pack_reserved;
         for (int s = 0; s < slots; ++s) {
             r += s == slot ? 0 : reserve[s];
         }
         return r;
+    }
+
+    // Starts the packer of objective count d (lock held).
+    void start_packer(int d) {
+        if (packers[d] != nullptr) {
+            return;
+        }
+        pmdd_ctx *c = nullptr;
+        char e[512] = {0};
+        const int rc = pmdd_ctx_create(device, &c, e, sizeof(e));
+        if (rc != PMDD_OK) {
+            fail(rc, e);
+        }
+        pack_ctx[d] = c;
+        {
+            AcctScope scope(&c->acct);
+            packers[d].reset(new Packer(c, d));
+        }
+        pack_threads.emplace_back([this, d] { packer_loop(d); });
+    }
+
+    void finish_job(const std::shared_ptr<ServerJob> &job) {
+        job->st.mem_estimate_bytes = job->estimate;
+        job->st.nomem_retries = job->retries;
+        job->st.queue_wait_s = job->admit_t - job->submit_t;
+        job->done = true;
+        finished.push_back(job->ticket);
+        cv_done.notify_all();
+    }
+
+    // Packer thread: admits queued requests of its objective count while
+    // it has room and the budget allows, expands one packed layer, hands
+    // back the requests that finished; repeats while any is in flight.
+    void packer_loop(int d) {
+        Packer &P = *packers[d];
+        pmdd_ctx *ctx = pack_ctx[d];
+        AcctScope scope(&ctx->acct);
+        (void)hipSetDevice(ctx->device);
+        std::unique_lock<std::mutex> lk(mu);
+        std::deque<std::shared_ptr<ServerJob>> &q = pack_queue[d];
+        while (true) {
+            if (stopping) {
+                return;
+            }
+            std::vector<std::shared_ptr<ServerJob>> admit;
+            while (!q.empty() && static_cast<int>(P.live.size() + admit.size()) < pack_inflight) {
+                const std::shared_ptr<ServerJob> &head = q.front();
+                if (running > 0 && reserved_except(-1) + head->estimate > budget) {
+                    if (blocked_ticket != head->ticket) {
+                        blocked_ticket = head->ticket;
+                        info.memory_waits += 1;
+                    }
+                    ask_trims(-1);
+                    cv_work.notify_all();
+                    break;
+                }
+                admit.push_back(head);
+                q.pop_front();
+                pack_reserved += head->estimate;
+                running += 1;
+                head->admit_t = now_s();
+                info.admitted += 1;
+                info.packed += 1;
+                info.running_peak = std::max(info.running_peak, running);
+                info.reserved_peak = std::max(info.reserved_peak, reserved_except(-1));
+            }
+            if (P.live.empty() && admit.empty()) {
+                cv_work.wait(lk);
+                continue;
+            }
+            lk.unlock();
+            std::vector<PackedReq> done;
+            std::vector<std::shared_ptr<ServerJob>> failed, requeue;
+            int frc = PMDD_OK;
+            std::string ferr;
+            for (const std::shared_ptr<ServerJob> &job : admit) {
+                try {
+                    P.add(job);
+                } catch (const EngineError &e) {
+                    job->rc = e.code;
+                    job->err = e.msg;
+                    failed.push_back(job);
+                } catch (const std::bad_alloc &) {
+                    job->rc = PMDD_E_NOMEM;
+                    job->err = "host allocation failed";
+                    failed.push_back(job);
+                }
+            }
+            try {
+                P.upload_offsets_now();
+                if (!P.live.empty()) {
+                    P.step(done);
+                }
+            } catch (const EngineError &e) {
+                frc = e.code;
+                ferr = e.msg;
+            } catch (const std::bad_alloc &) {
+                frc = PMDD_E_NOMEM;
+                ferr = "host allocation failed";
+            }
+            if (frc != PMDD_OK) {
+                // the packed layer failed: its requests run again in the
+                // slots, unpacked, and the packer starts empty
+                (void)hipStreamSynchronize(ctx->stream);
+                for (PackedReq &r : P.live) {
+                    pmdd_frontier_free(&r.job->fr);
+                    requeue.push_back(r.job);
+                }
+                for (PackedReq &r : done) {
+                    pmdd_frontier_free(&r.job->fr);
+                    requeue.push_back(r.job);
+                    delete r.dd;
+                }
+                done.clear();
+                try {
+                    P.reset();
+                } catch (const EngineError &) {
+                }
+            }
+            for (PackedReq &r : done) {
+                delete r.dd;
+                r.dd = nullptr;
+            }
+            lk.lock();
+            int64_t steps = 0, stepped = 0;
+            for (const std::unique_ptr<Packer> &pk : packers) {
+                if (pk != nullptr) {
+                    steps += pk->steps.load(std::memory_order_relaxed);
+                    stepped += pk->stepped.load(std::memory_order_relaxed);
+                }
+            }
+            info.pack_steps = steps;
+            info.pack_requests_per_step_x100 = steps > 0 ? 100 * stepped / steps : 0;
+            for (const std::shared_ptr<ServerJob> &job : failed) {
+                pack_reserved -= job->estimate;
+                running -= 1;
+                finish_job(job);
+            }
+            for (PackedReq &r : done) {
+                pack_reserved -= r.job->estimate;
+                running -= 1;
+                r.job->rc = PMDD_OK;
+                finish_job(r.job);
+            }
+            for (const std::shared_ptr<ServerJob> &job : requeue) {
+                pack_reserved -= job->estimate;
+                running -= 1;
+                job->retries += 1;
+                queue.push_front(job);
+            }
+            cv_work.notify_all();
+        }
     }
 
     // Whether idle `slot` may start the head of the queue now: its estimate
@@ -2887,12 +3365,7 @@ struct pmdd_server {
                 cv_work.notify_all();
                 continue;
             }
-            job->st.mem_estimate_bytes = job->estimate;
-            job->st.nomem_retries = job->retries;
-            job->st.queue_wait_s = job->admit_t - job->submit_t;
-            job->done = true;
-            finished.push_back(job->ticket);
-            cv_done.notify_all();
+            finish_job(job);
             cv_work.notify_all();
         }
     }
@@ -2929,6 +3402,8 @@ int pmdd_server_create(const pmdd_server_config *cfg, pmdd_server **out, char *e
             hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
             srv->budget = static_cast<int64_t>(fr) / 5 * 4;
         }
+        srv->pack = cfg->pack != 0;
+        srv->pack_inflight = cfg->pack_inflight > 0 ? cfg->pack_inflight : kPackInflight;
         srv->reserve.assign(cfg->slots, 0);
         srv->busy.assign(cfg->slots, 0);
         srv->trim_asked.assign(cfg->slots, 0);
@@ -2992,7 +3467,12 @@ int pmdd_server_submit(pmdd_server *srv, const pmdd_dd_view *view, const pmdd_op
             srv->cv_done.notify_all();
             return PMDD_OK;
         }
-        srv->queue.push_back(job);
+        if (srv->pack && packable(*view, *opts)) {
+            srv->start_packer(view->d);
+            srv->pack_queue[view->d].push_back(job);
+        } else {
+            srv->queue.push_back(job);
+        }
         srv->cv_work.notify_all();
         return PMDD_OK;
     });
@@ -3107,8 +3587,19 @@ void pmdd_server_destroy(pmdd_server *srv) {
     for (std::thread &t : srv->workers) {
         t.join();
     }
+    for (std::thread &t : srv->pack_threads) {
+        t.join();
+    }
     for (auto &kv : srv->jobs) {
         pmdd_frontier_free(&kv.second->fr);
+    }
+    for (int dd = 0; dd < 9; ++dd) {
+        if (srv->packers[dd] != nullptr) {
+            (void)hipSetDevice(srv->pack_ctx[dd]->device);
+            (void)hipStreamSynchronize(srv->pack_ctx[dd]->stream);
+            srv->packers[dd].reset();
+            pmdd_ctx_destroy(srv->pack_ctx[dd]);
+        }
     }
     for (pmdd_ctx *c : srv->ctxs) {
         pmdd_ctx_destroy(c);

@@ -303,21 +303,32 @@ __device__ __forceinline__ bool scan_until_dead(const int32_t *S, int n, int lim
 }
 
 // ---------------------------------------------------------------------------
-// k_filter_chunks: per-chunk load, presort, sort-filter skyline.
+// Chunk filter: per-chunk load, presort, sort-filter skyline.  The body is
+// shared by k_filter_chunks (one workgroup per listed chunk) and
+// k_filter_queue (persistent workgroups taking chunks from a queue).
 // ---------------------------------------------------------------------------
-template <int D, int CH, class Src, bool PRESORTED, int NT = kBlock>
-__global__ __launch_bounds__(NT) void k_filter_chunks(
-    const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks, Src src,
-    int32_t *__restrict__ out_pts,
-    int64_t out_stride, uint8_t *__restrict__ alive, int32_t *__restrict__ chunk_nsurv,
-    int32_t *__restrict__ chunk_nalive, unsigned long long *__restrict__ pair_tests) {
-    constexpr int DP = Pad<D>::DP;
-    __shared__ int32_t cs[D][CH];
-    __shared__ uint64_t sk[CH];
-    __shared__ __align__(16) int32_t S[CH * DP];
-    __shared__ int wave_tot[NT / kWave];
+template <int D, int CH, int NT> struct FilterLds {
+    int32_t cs[D][CH];
+    uint64_t sk[CH];
+    alignas(16) int32_t S[CH * Pad<D>::DP];
+    int wave_tot[NT / kWave];
+};
 
-    const int ci = which != nullptr ? which[blockIdx.x] : static_cast<int>(blockIdx.x);
+// Filters chunk ci; adds the pair tests it ran to `tests`.  All threads of
+// the workgroup call it (it has barriers); it ends with one, so the caller
+// may reuse the LDS at once.
+template <int D, int CH, class Src, bool PRESORTED, int NT>
+__device__ __forceinline__ void filter_chunk(int ci, const ChunkDesc *__restrict__ chunks,
+                                             const Src &src, int32_t *__restrict__ out_pts,
+                                             int64_t out_stride, uint8_t *__restrict__ alive,
+                                             int32_t *__restrict__ chunk_nsurv,
+                                             int32_t *__restrict__ chunk_nalive,
+                                             FilterLds<D, CH, NT> &L, unsigned long long &tests) {
+    constexpr int DP = Pad<D>::DP;
+    auto &cs = L.cs;
+    auto &sk = L.sk;
+    auto &S = L.S;
+    auto &wave_tot = L.wave_tot;
     const ChunkDesc c = chunks[ci];
     const int n = c.len;
     const int tid = threadIdx.x;
@@ -363,7 +374,6 @@ __global__ __launch_bounds__(NT) void k_filter_chunks(
 
     // sort-filter skyline, NT candidates per step
     int nS = 0;
-    unsigned long long tests = 0;
     for (int base = 0; base < n; base += NT) {
         const int i = base + tid;
         const bool valid = i < n;
@@ -416,7 +426,69 @@ __global__ __launch_bounds__(NT) void k_filter_chunks(
         chunk_nsurv[ci] = nS;
         chunk_nalive[ci] = nS;
     }
+    __syncthreads(); // every read of L is done
+}
+
+// One workgroup per listed chunk (every chunk when `which` is null).
+template <int D, int CH, class Src, bool PRESORTED, int NT = kBlock>
+__global__ __launch_bounds__(NT) void k_filter_chunks(
+    const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks, Src src,
+    int32_t *__restrict__ out_pts,
+    int64_t out_stride, uint8_t *__restrict__ alive, int32_t *__restrict__ chunk_nsurv,
+    int32_t *__restrict__ chunk_nalive, unsigned long long *__restrict__ pair_tests) {
+    __shared__ FilterLds<D, CH, NT> L;
+    const int ci = which != nullptr ? which[blockIdx.x] : static_cast<int>(blockIdx.x);
+    unsigned long long tests = 0;
+    filter_chunk<D, CH, Src, PRESORTED, NT>(ci, chunks, src, out_pts, out_stride, alive,
+                                            chunk_nsurv, chunk_nalive, L, tests);
     if (pair_tests != nullptr) {
+        atomicAdd(pair_tests, tests);
+    }
+}
+
+// One tier of a queued expansion pass, read by k_filter_queue from device
+// memory, so that the launch's arguments stay the same from one pass to the
+// next (a captured graph of the pass's launches can be replayed).
+struct QueueTier {
+    ExpandSrc src;
+    const ChunkDesc *chunks;
+    const int32_t *which; // chunks of this tier
+    int32_t count;        // entries of which
+    int32_t pad;
+    int32_t *out_pts;
+    int64_t out_stride;
+    uint8_t *alive;
+    int32_t *nsurv;
+    int32_t *nalive;
+};
+
+// Persistent chunk filter: a fixed grid of workgroups takes the chunks of a
+// tier one at a time from a queue (an atomic counter, *head, zero before the
+// launch) until every listed chunk is filtered.  Chunks of many nodes, and
+// of many instances when a layer packs several, share one launch.
+template <int D, int CH, int NT>
+__global__ __launch_bounds__(NT) void k_filter_queue(const QueueTier *__restrict__ tier,
+                                                     int32_t *__restrict__ head,
+                                                     unsigned long long *__restrict__ pair_tests) {
+    __shared__ FilterLds<D, CH, NT> L;
+    __shared__ int item;
+    const QueueTier t = *tier;
+    unsigned long long tests = 0;
+    while (true) {
+        if (threadIdx.x == 0) {
+            item = atomicAdd(head, 1);
+        }
+        __syncthreads();
+        const int k = item;
+        __syncthreads(); // every thread has read item before thread 0 writes it again
+        if (k >= t.count) {
+            break;
+        }
+        filter_chunk<D, CH, ExpandSrc, false, NT>(t.which[k], t.chunks, t.src, t.out_pts,
+                                                  t.out_stride, t.alive, t.nsurv, t.nalive, L,
+                                                  tests);
+    }
+    if (pair_tests != nullptr && tests != 0) {
         atomicAdd(pair_tests, tests);
     }
 }
@@ -1272,6 +1344,45 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
         }
         if (lane == 0) {
             atomicAdd(pair_tests, tests);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Packed layers (serving): the layers of several in-flight instances taken as
+// one.  Part i contributes its layer's arcs at [arc_base, arc_base + arcs) of
+// the packed layer, their parents moved by node_add (the first node of the
+// instance's current layer in the packed frontier).
+// ---------------------------------------------------------------------------
+struct PackPart {
+    const int32_t *src; // [arcs] parent of each arc, layer-local
+    const int32_t *w;   // SoA [d][arcs]
+    int64_t arcs;
+    int64_t arc_base;
+    int64_t node_add;
+};
+
+__global__ __launch_bounds__(kBlock) void k_pack_arcs(const PackPart *__restrict__ parts,
+                                                      int32_t nparts, int64_t total, int d,
+                                                      int32_t *__restrict__ src_out,
+                                                      int32_t *__restrict__ w_out) {
+    const int64_t step = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; e < total;
+         e += step) {
+        int lo = 0, hi = nparts - 1; // last part whose arcs start at or before e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (parts[mid].arc_base <= e) {
+                lo = mid;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        const PackPart p = parts[lo];
+        const int64_t k = e - p.arc_base;
+        src_out[e] = static_cast<int32_t>(p.src[k] + p.node_add);
+        for (int o = 0; o < d; ++o) {
+            w_out[o * total + e] = p.w[o * p.arcs + k];
         }
     }
 }

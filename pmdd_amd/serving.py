@@ -33,10 +33,16 @@ from . import instances as gen
 
 
 class Server:
-    def __init__(self, device: int = 0, slots: int = 8, memory_budget: int = 0):
+    """pack=True turns on packed serving: method-1 requests without state
+    dominance on small diagrams go to a packer of their objective count,
+    which expands the current layers of up to pack_inflight in-flight
+    requests as one packed layer per step; the rest use the slots."""
+
+    def __init__(self, device: int = 0, slots: int = 8, memory_budget: int = 0,
+                 pack: bool = False, pack_inflight: int = 0):
         lib = abi.engine_lib()
         self._lib = lib
-        cfg = abi.ServerConfig(device, slots, memory_budget)
+        cfg = abi.ServerConfig(device, slots, memory_budget, 1 if pack else 0, pack_inflight)
         h = C.c_void_p()
         err = C.create_string_buffer(512)
         rc = lib.pmdd_server_create(C.byref(cfg), C.byref(h), err, len(err))
@@ -161,7 +167,8 @@ def write_stream(specs: list[dict], directory: str) -> list[str]:
 def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots: int,
                method: int = 1, device: int = 0, builders: int = 16,
                results: dict | None = None, progress: int = 0,
-               stats: dict | None = None, memory_budget: int = 0) -> dict:
+               stats: dict | None = None, memory_budget: int = 0, pack: bool = False,
+               pack_inflight: int = 0) -> dict:
     """Replay the stream: at each arrival the request's diagram is built on a
     host thread and submitted; one collector thread takes results as they
     finish (pmdd_server_wait_any); latency runs from arrival to result.
@@ -169,7 +176,8 @@ def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots
     and `stats` receive each request's frontier and statistics by index; the
     returned summary holds the server's admission record and the largest
     ratio of a request's measured device memory to its estimate."""
-    server = Server(device, slots=slots, memory_budget=memory_budget)
+    server = Server(device, slots=slots, memory_budget=memory_budget, pack=pack,
+                    pack_inflight=pack_inflight)
     n = len(paths)
     done_at = [0.0] * n
     points = [0] * n
@@ -197,11 +205,16 @@ def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots
             done_at[i] = time.perf_counter()
             events.put(False)
 
+    last_print = [t0]
+
     def collect():
         for k in range(n):
-            if progress and k % progress == 0 and k > 0:
-                print(f"served {k}/{n} after {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-                      flush=True)
+            now = time.perf_counter()
+            # a line after every `progress` results, and after a result that
+            # comes 10 s or more after the last line
+            if progress and k > 0 and (k % progress == 0 or now - last_print[0] >= 10.0):
+                print(f"served {k}/{n} after {now - t0:.1f} s", file=sys.stderr, flush=True)
+                last_print[0] = now
             if not events.get():
                 continue
             ticket, res = server.wait_any()
@@ -231,6 +244,7 @@ def run_stream(paths: list[str], specs: list[dict], arrivals: list[float], slots
     ratios = [peak / est for peak, est in mem.values() if est > 0]
     over = sum(1 for peak, est in mem.values() if peak > est)
     return {
+        "pack": pack,
         "admission": info,
         "mem_peak_over_estimate_max": max(ratios) if ratios else None,
         "mem_peak_above_estimate": over,

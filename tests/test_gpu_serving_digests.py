@@ -74,6 +74,29 @@ def test_served_prefix_matches_oracle_digests(engine, stream, method):
           f"{m['mem_peak_above_estimate']} requests above their estimate", flush=True)
 
 
+@pytest.mark.parametrize("inflight", [32, 3])
+def test_packed_serving_matches_oracle_digests(engine, stream, inflight):
+    """Packed serving: method-1 requests on small diagrams share packed
+    layers (a packer per objective count expands the current layers of its
+    in-flight requests as one layer, through a captured graph replayed every
+    step); requests join at any step and leave at their terminal layer.
+    Every served frontier matches its digest, requests were packed several
+    to a step, and the rest of the stream ran in the slots."""
+    specs, paths = stream
+    print(f"case packed serving prefix={PREFIX} inflight={inflight}", flush=True)
+    results = {}
+    m = serving.run_stream(paths, specs, [0.0] * PREFIX, slots=8, method=1, results=results,
+                           pack=True, pack_inflight=inflight)
+    assert m["failed"] == 0, m["errors"]
+    _check(results, range(PREFIX))
+    a = m["admission"]
+    assert a["admitted"] == PREFIX
+    assert 0 < a["packed"] < PREFIX  # some packed, the large ones in the slots
+    assert a["pack_steps"] > 0
+    assert a["pack_requests_per_step_x100"] > 100  # more than one request per step
+    print(f"admission: {a}", flush=True)
+
+
 def test_small_budget_makes_the_head_wait(engine, stream):
     """A budget a little above the largest estimate: the head of the queue
     waits for memory while slots are free (it never does under the default
