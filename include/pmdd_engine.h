@@ -215,6 +215,29 @@ typedef struct pmdd_resident pmdd_resident;
 int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen);
 void pmdd_ctx_destroy(pmdd_ctx *ctx);
 
+/* Launch shapes of a context, per objective count d (no reference
+ * counterpart).  A context starts with the shapes of the tuning table
+ * (pmdd_amd/tune/gfx950.json next to the library, or the file named by the
+ * environment variable PMDD_TUNE_FILE) and built-in defaults where the table
+ * has none; the tuner (python -m pmdd_amd.tune) writes that table.  Shapes
+ * never change a result, only the time it takes. */
+typedef struct pmdd_tuning {
+    int32_t filter_threads[3]; /* threads of the chunk filter for nodes of at most 256
+                                * candidates, at most 1024 (d <= 4), at most a whole
+                                * chunk: each 128, 256 or 512 (defaults 128, 512, 512) */
+    int32_t cross_workgroups;  /* workgroups a wave of the block cross filter is spread
+                                * over, 256..16384 (default 2048) */
+    int32_t cross_split_max;   /* most workgroups per block in that wave, 1..16 (16) */
+    int32_t refill_waves;      /* one-wave workgroups of the frontier check,
+                                * 256..32768 (8192) */
+    int32_t source;            /* out: 0 built-in defaults, 1 the table, 2 set by the caller */
+} pmdd_tuning;
+
+/* The shapes a context uses for objective count d (1..8); set_tuning
+ * validates them (PMDD_E_INVALID when out of range). */
+int pmdd_ctx_get_tuning(pmdd_ctx *ctx, int d, pmdd_tuning *out);
+int pmdd_ctx_set_tuning(pmdd_ctx *ctx, int d, const pmdd_tuning *t);
+
 /* Number of visible HIP devices (0 when none); never fails. */
 int pmdd_device_count(void);
 

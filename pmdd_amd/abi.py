@@ -64,6 +64,16 @@ class Opts(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("filter_threads", C.c_int32 * 3),
+        ("cross_workgroups", C.c_int32),
+        ("cross_split_max", C.c_int32),
+        ("refill_waves", C.c_int32),
+        ("source", C.c_int32),
+    ]
+
+
 class Frontier(C.Structure):
     _fields_ = [
         ("d", C.c_int32),
@@ -171,6 +181,8 @@ ENGINE_SYMBOLS = [
     "pmdd_ctx_create",
     "pmdd_ctx_destroy",
     "pmdd_device_count",
+    "pmdd_ctx_get_tuning",
+    "pmdd_ctx_set_tuning",
     "pmdd_enumerate",
     "pmdd_topdown_enumerate",
     "pmdd_topdown_mdd_enumerate",
@@ -245,6 +257,10 @@ def engine_lib() -> C.CDLL:
         lib.pmdd_ctx_destroy.restype = None
         lib.pmdd_device_count.argtypes = []
         lib.pmdd_device_count.restype = C.c_int
+        lib.pmdd_ctx_get_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(Tuning)]
+        lib.pmdd_ctx_get_tuning.restype = C.c_int
+        lib.pmdd_ctx_set_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(Tuning)]
+        lib.pmdd_ctx_set_tuning.restype = C.c_int
         lib.pmdd_enumerate.argtypes = [
             C.c_void_p, C.POINTER(DDView), C.POINTER(Opts), C.POINTER(Frontier),
             C.POINTER(Stats)] + err

@@ -14,9 +14,11 @@
 // reference) -> Run::bottomup, and a serving loop with no reference
 // counterpart (pmdd_server_*: worker slots with their own streams, memory
 // admission).
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -363,6 +365,8 @@ struct pmdd_ctx {
     // blocks such a call stopped using, freed by the next allocating call
     char *scratch = nullptr;
     std::vector<void *> deferred;
+    // launch shapes per objective count (index d)
+    pmdd_tuning tuning[9];
     // device bytes held by this context's buffers
     MemAcct acct;
     bool in_call = false; // inside a one-shot pmdd_enumerate (its peak includes the upload)
@@ -807,9 +811,10 @@ using AppendFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *,
 
 struct Launch {
     int chunk_cap;
-    FilterFn filter_expand;  // ExpandSrc, presorted in LDS
-    FilterFn filter_expand_small; // the same for nodes of at most kSmallNode candidates
-    FilterFn filter_expand_mid;   // ... and of at most mid_cap candidates (0: no such tier)
+    // ExpandSrc chunk filters by tier (nodes of at most kSmallNode
+    // candidates, of at most mid_cap, of at most chunk_cap) and threads per
+    // workgroup (128, 256, 512): the shapes the tuning picks from
+    FilterFn filter_tier[3][3];
     int mid_cap;
     FilterFn filter_join;    // JoinSrc, presorted in LDS
     FilterFn filter_sorted;  // SortedSrc, source already in key order
@@ -874,11 +879,11 @@ constexpr int64_t kServeJoinCap = int64_t(1) << 19;
 
 template <int D> struct LaunchImpl {
     static constexpr int CH = pmdd_k::Cap<D>::CH;
-    template <class Src, bool PRE, int C = CH>
+    template <class Src, bool PRE, int C = CH,
+              int NT = (C >= 1024 ? kFilterThreadsBig : (C <= 256 ? 128 : pmdd_k::kBlock))>
     static void filter(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                        const void *src, int32_t *out, int64_t stride, uint8_t *alive, int32_t *ns,
                        int32_t *na, unsigned long long *pt) {
-        constexpr int NT = C >= 1024 ? kFilterThreadsBig : (C <= 256 ? 128 : pmdd_k::kBlock);
         hipLaunchKernelGGL((pmdd_k::k_filter_chunks<D, C, Src, PRE, NT>), g, dim3(NT), 0, s,
                            which, c, *static_cast<const Src *>(src), out, stride, alive, ns, na,
                            pt);
@@ -979,6 +984,15 @@ template <int D> struct LaunchImpl {
     static Launch table() {
         Launch L;
         constexpr int MID = CH > kMidNode ? kMidNode : CH;
+        L.filter_tier[0][0] = filter<pmdd_k::ExpandSrc, false, kSmallNode, 128>;
+        L.filter_tier[0][1] = filter<pmdd_k::ExpandSrc, false, kSmallNode, 256>;
+        L.filter_tier[0][2] = filter<pmdd_k::ExpandSrc, false, kSmallNode, 512>;
+        L.filter_tier[1][0] = filter<pmdd_k::ExpandSrc, false, MID, 128>;
+        L.filter_tier[1][1] = filter<pmdd_k::ExpandSrc, false, MID, 256>;
+        L.filter_tier[1][2] = filter<pmdd_k::ExpandSrc, false, MID, 512>;
+        L.filter_tier[2][0] = filter<pmdd_k::ExpandSrc, false, CH, 128>;
+        L.filter_tier[2][1] = filter<pmdd_k::ExpandSrc, false, CH, 256>;
+        L.filter_tier[2][2] = filter<pmdd_k::ExpandSrc, false, CH, 512>;
         L.queue[0] = queue_tier<kSmallNode>;
         L.queue[1] = queue_tier<MID>;
         L.queue[2] = queue_tier<CH>;
@@ -996,10 +1010,7 @@ template <int D> struct LaunchImpl {
         L.refill_flags = refill_flags;
         L.filter_dense = filter<pmdd_k::SortedSrc, false>;
         L.chunk_cap = CH;
-        L.filter_expand = filter<pmdd_k::ExpandSrc, false>;
-        L.filter_expand_small = filter<pmdd_k::ExpandSrc, false, kSmallNode>;
         L.mid_cap = CH > kMidNode ? kMidNode : 0;
-        L.filter_expand_mid = filter<pmdd_k::ExpandSrc, false, (CH > kMidNode ? kMidNode : CH)>;
         L.filter_join = filter<pmdd_k::JoinSrc, false>;
         L.filter_sorted = filter<pmdd_k::SortedSrc, true>;
         L.cross = cross;
@@ -1025,6 +1036,232 @@ Launch launch_for(int d) {
     case 8: return LaunchImpl<8>::table();
     default: fail(PMDD_E_UNSUPPORTED, "objective count outside 1..8");
     }
+}
+
+// Index of a filter thread count in Launch::filter_tier (128, 256, 512).
+inline int threads_index(int nt) { return nt <= 128 ? 0 : (nt <= 256 ? 1 : 2); }
+
+// ---------------------------------------------------------------------------
+// Tuning table: launch shapes per objective count, read once per process from
+// a small JSON file (python -m pmdd_amd.tune writes it):
+//   {"tuning": {"<d>": {"filter_threads": [t0, t1, t2], "cross_workgroups": n,
+//                       "cross_split_max": n, "refill_waves": n}, ...}, ...}
+// Other keys are ignored.  The file is PMDD_TUNE_FILE, or tune/gfx950.json
+// beside the library's directory (pmdd_amd/lib/../tune/gfx950.json).
+// ---------------------------------------------------------------------------
+pmdd_tuning default_tuning() {
+    pmdd_tuning t{};
+    t.filter_threads[0] = 128;
+    t.filter_threads[1] = kFilterThreadsBig;
+    t.filter_threads[2] = kFilterThreadsBig;
+    t.cross_workgroups = 2048;
+    t.cross_split_max = 16;
+    t.refill_waves = static_cast<int32_t>(kRefillWaves);
+    t.source = 0;
+    return t;
+}
+
+bool tuning_valid(const pmdd_tuning &t) {
+    for (int i = 0; i < 3; ++i) {
+        const int n = t.filter_threads[i];
+        if (n != 128 && n != 256 && n != 512) {
+            return false;
+        }
+    }
+    return t.cross_workgroups >= 256 && t.cross_workgroups <= 16384 && t.cross_split_max >= 1 &&
+           t.cross_split_max <= 16 && t.refill_waves >= 256 && t.refill_waves <= 32768;
+}
+
+// A JSON value, as much of it as the table needs.
+struct JVal {
+    enum Kind { NUL, NUM, STR, ARR, OBJ, BOOL } kind = NUL;
+    double num = 0.0;
+    std::string str;
+    std::vector<JVal> arr;
+    std::vector<std::pair<std::string, JVal>> obj;
+    const JVal *get(const std::string &k) const {
+        for (const auto &kv : obj) {
+            if (kv.first == k) {
+                return &kv.second;
+            }
+        }
+        return nullptr;
+    }
+};
+
+struct JParser {
+    const std::string &s;
+    size_t i = 0;
+    bool ok = true;
+    explicit JParser(const std::string &x) : s(x) {}
+    void ws() {
+        while (i < s.size() && std::isspace(static_cast<unsigned char>(s[i]))) {
+            ++i;
+        }
+    }
+    bool eat(char c) {
+        ws();
+        if (i < s.size() && s[i] == c) {
+            ++i;
+            return true;
+        }
+        return false;
+    }
+    std::string string_lit() {
+        std::string out;
+        if (!eat('"')) {
+            ok = false;
+            return out;
+        }
+        while (i < s.size() && s[i] != '"') {
+            if (s[i] == '\\' && i + 1 < s.size()) {
+                ++i;
+            }
+            out.push_back(s[i++]);
+        }
+        if (i >= s.size()) {
+            ok = false;
+        } else {
+            ++i;
+        }
+        return out;
+    }
+    JVal value(int depth = 0) {
+        JVal v;
+        ws();
+        if (!ok || i >= s.size() || depth > 32) {
+            ok = false;
+            return v;
+        }
+        const char c = s[i];
+        if (c == '{') {
+            ++i;
+            v.kind = JVal::OBJ;
+            if (eat('}')) {
+                return v;
+            }
+            do {
+                std::string k = string_lit();
+                if (!eat(':')) {
+                    ok = false;
+                    return v;
+                }
+                v.obj.emplace_back(k, value(depth + 1));
+            } while (ok && eat(','));
+            if (!eat('}')) {
+                ok = false;
+            }
+        } else if (c == '[') {
+            ++i;
+            v.kind = JVal::ARR;
+            if (eat(']')) {
+                return v;
+            }
+            do {
+                v.arr.push_back(value(depth + 1));
+            } while (ok && eat(','));
+            if (!eat(']')) {
+                ok = false;
+            }
+        } else if (c == '"') {
+            v.kind = JVal::STR;
+            v.str = string_lit();
+        } else if (s.compare(i, 4, "true") == 0 || s.compare(i, 5, "false") == 0) {
+            v.kind = JVal::BOOL;
+            i += s[i] == 't' ? 4 : 5;
+        } else if (s.compare(i, 4, "null") == 0) {
+            i += 4;
+        } else {
+            char *end = nullptr;
+            v.num = std::strtod(s.c_str() + i, &end);
+            if (end == s.c_str() + i) {
+                ok = false;
+            } else {
+                v.kind = JVal::NUM;
+                i = static_cast<size_t>(end - s.c_str());
+            }
+        }
+        return v;
+    }
+};
+
+// Shapes per objective count from the table file (source 1), or the
+// defaults (source 0) where the file, or its entry for d, is missing or bad.
+struct TuneTable {
+    pmdd_tuning t[9];
+    std::string path;
+};
+
+std::string tune_file_path() {
+    const char *env = std::getenv("PMDD_TUNE_FILE");
+    if (env != nullptr && *env != '\0') {
+        return env;
+    }
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void *>(&default_tuning), &info) != 0 &&
+        info.dli_fname != nullptr) {
+        std::string lib(info.dli_fname);
+        const size_t slash = lib.rfind('/');
+        const std::string dir = slash == std::string::npos ? "." : lib.substr(0, slash);
+        return dir + "/../tune/gfx950.json";
+    }
+    return "";
+}
+
+const TuneTable &tune_table() {
+    static const TuneTable table = [] {
+        TuneTable tt;
+        for (pmdd_tuning &x : tt.t) {
+            x = default_tuning();
+        }
+        tt.path = tune_file_path();
+        std::FILE *f = tt.path.empty() ? nullptr : std::fopen(tt.path.c_str(), "rb");
+        if (f == nullptr) {
+            return tt;
+        }
+        std::string text;
+        char buf[4096];
+        size_t n = 0;
+        while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) {
+            text.append(buf, n);
+        }
+        std::fclose(f);
+        JParser p(text);
+        const JVal root = p.value();
+        const JVal *tun = p.ok ? root.get("tuning") : nullptr;
+        if (tun == nullptr || tun->kind != JVal::OBJ) {
+            return tt;
+        }
+        for (int d = 1; d <= 8; ++d) {
+            const JVal *e = tun->get(std::to_string(d));
+            if (e == nullptr || e->kind != JVal::OBJ) {
+                continue;
+            }
+            pmdd_tuning x = default_tuning();
+            const JVal *ft = e->get("filter_threads");
+            if (ft != nullptr && ft->kind == JVal::ARR && ft->arr.size() == 3) {
+                for (int k = 0; k < 3; ++k) {
+                    x.filter_threads[k] = static_cast<int32_t>(ft->arr[k].num);
+                }
+            }
+            const JVal *v = nullptr;
+            if ((v = e->get("cross_workgroups")) != nullptr && v->kind == JVal::NUM) {
+                x.cross_workgroups = static_cast<int32_t>(v->num);
+            }
+            if ((v = e->get("cross_split_max")) != nullptr && v->kind == JVal::NUM) {
+                x.cross_split_max = static_cast<int32_t>(v->num);
+            }
+            if ((v = e->get("refill_waves")) != nullptr && v->kind == JVal::NUM) {
+                x.refill_waves = static_cast<int32_t>(v->num);
+            }
+            x.source = 1;
+            if (tuning_valid(x)) {
+                tt.t[d] = x;
+            }
+        }
+        return tt;
+    }();
+    return table;
 }
 
 inline int ceil_log2(int64_t x) {
@@ -1360,7 +1597,6 @@ struct Run {
     // Returns the number of k_cross_chunks launches; launch i is bracketed by
     // pool events 2i and 2i+1.
     static constexpr int kCrossWave = 64;
-    static constexpr int64_t kCrossSplit = 16; // most workgroups per chunk
     int run_cross_waves(ChunkBufs &cb, const std::vector<ChunkDesc> &chunks, const int32_t *pts,
                         int64_t stride, uint8_t *alive, int32_t *fv, int64_t fv_stride) {
         int32_t max_count = 1;
@@ -1413,10 +1649,12 @@ struct Run {
             if (n == 0) {
                 continue;
             }
-            // spread each chunk's scan over G workgroups, about 2048 in all
-            // (1536 fit the chip at this kernel's LDS use)
-            const int G = static_cast<int>(
-                std::min<int64_t>(kCrossSplit, std::max<int64_t>(1, ceil_div(2048, n))));
+            // spread each chunk's scan over G workgroups, about
+            // cross_workgroups in all (2048 by default; 1536 fit the chip at
+            // this kernel's LDS use), at most cross_split_max per chunk
+            const pmdd_tuning &T = ctx->tuning[d];
+            const int G = static_cast<int>(std::min<int64_t>(
+                T.cross_split_max, std::max<int64_t>(1, ceil_div(T.cross_workgroups, n))));
             hip_check(hipEventRecord(pool_event(2 * launches), s), "hipEventRecord");
             K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + start[w], cb.chunks.as<ChunkDesc>(),
                     pts, stride, alive, cb.nsurv.as<int32_t>(), fv, fv_stride, fin, kCrossWave, G,
@@ -1661,12 +1899,16 @@ struct Run {
                                          ctx->surv_pts.as<int32_t>(), pass_c,
                                          ctx->alive.as<uint8_t>());
             } else {
-                filters = run_filter(K.filter_expand_small, cb, ctx->which_s, which_tiny, 0, &src,
+                const pmdd_tuning &T = ctx->tuning[d];
+                filters = run_filter(K.filter_tier[0][threads_index(T.filter_threads[0])], cb,
+                                     ctx->which_s, which_tiny, 0, &src,
                                      ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>());
-                filters += run_filter(K.filter_expand_mid, cb, ctx->which_m, which_mid, 0, &src,
+                filters += run_filter(K.filter_tier[1][threads_index(T.filter_threads[1])], cb,
+                                      ctx->which_m, which_mid, 0, &src,
                                       ctx->surv_pts.as<int32_t>(), pass_c,
                                       ctx->alive.as<uint8_t>());
-                filters += run_filter(K.filter_expand, cb, ctx->which_a, which_small, 0, &src,
+                filters += run_filter(K.filter_tier[2][threads_index(T.filter_threads[2])], cb,
+                                      ctx->which_a, which_small, 0, &src,
                                       ctx->surv_pts.as<int32_t>(), pass_c,
                                       ctx->alive.as<uint8_t>());
             }
@@ -2213,8 +2455,8 @@ struct Run {
                 ctr[0] = static_cast<unsigned long long>(bprod);
                 h2d(ctx, ctx->join_ctr, ctr, "hipMemcpy(join counters)");
                 unsigned long long *dc = ctx->join_ctr.as<unsigned long long>();
-                const unsigned grid = static_cast<unsigned>(
-                    std::min<int64_t>(ceil_div(bprod, pmdd_k::kWave), kRefillWaves));
+                const unsigned grid = static_cast<unsigned>(std::min<int64_t>(
+                    ceil_div(bprod, pmdd_k::kWave), ctx->tuning[d].refill_waves));
                 int32_t *in = ctx->join_x.as<int32_t>();
                 int32_t *outb = ctx->join_y.as<int32_t>();
                 hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
@@ -2402,8 +2644,8 @@ struct Run {
                                         static_cast<unsigned long long>(r_count)},
                 "hipMemcpy(frontier count)");
             hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            const unsigned rgrid = static_cast<unsigned>(
-                std::min<int64_t>(ceil_div(r_count, pmdd_k::kWave), kRefillWaves));
+            const unsigned rgrid = static_cast<unsigned>(std::min<int64_t>(
+                ceil_div(r_count, pmdd_k::kWave), ctx->tuning[d].refill_waves));
             hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
             for (int64_t lo = 0; lo < btotal; lo += kEvictBlock) {
                 const int64_t hi = std::min<int64_t>(btotal, lo + kEvictBlock);
@@ -2643,6 +2885,8 @@ int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
         c->device = device;
         hip_check(hipSetDevice(device), "hipSetDevice");
         c->scratch = scratch_address(device);
+        const TuneTable &tt = tune_table();
+        std::copy(tt.t, tt.t + 9, c->tuning);
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&c->ev_a), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
@@ -2689,6 +2933,23 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;
+}
+
+int pmdd_ctx_get_tuning(pmdd_ctx *ctx, int d, pmdd_tuning *out) {
+    if (ctx == nullptr || out == nullptr || d < 1 || d > 8) {
+        return PMDD_E_INVALID;
+    }
+    *out = ctx->tuning[d];
+    return PMDD_OK;
+}
+
+int pmdd_ctx_set_tuning(pmdd_ctx *ctx, int d, const pmdd_tuning *t) {
+    if (ctx == nullptr || t == nullptr || d < 1 || d > 8 || !tuning_valid(*t)) {
+        return PMDD_E_INVALID;
+    }
+    ctx->tuning[d] = *t;
+    ctx->tuning[d].source = 2;
+    return PMDD_OK;
 }
 
 int pmdd_upload(pmdd_ctx *ctx, const pmdd_dd_view *view, pmdd_resident **out, char *err,

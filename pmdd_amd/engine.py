@@ -106,6 +106,27 @@ class Engine:
         pts = _take_frontier(self._lib, fr, sort)
         return EnumerationResult(pts, _stats_dict(st, nl, std_c, std_s))
 
+    def get_tuning(self, d: int) -> dict:
+        """Launch shapes this context uses for objective count d."""
+        t = abi.Tuning()
+        rc = self._lib.pmdd_ctx_get_tuning(self._h, int(d), C.byref(t))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, f"no tuning for d={d}")
+        return {"filter_threads": list(t.filter_threads), "cross_workgroups": t.cross_workgroups,
+                "cross_split_max": t.cross_split_max, "refill_waves": t.refill_waves,
+                "source": t.source}
+
+    def set_tuning(self, d: int, cfg: dict) -> None:
+        t = abi.Tuning()
+        for k in range(3):
+            t.filter_threads[k] = int(cfg["filter_threads"][k])
+        t.cross_workgroups = int(cfg["cross_workgroups"])
+        t.cross_split_max = int(cfg["cross_split_max"])
+        t.refill_waves = int(cfg["refill_waves"])
+        rc = self._lib.pmdd_ctx_set_tuning(self._h, int(d), C.byref(t))
+        if rc != abi.PMDD_OK:
+            raise PmddError(rc, f"invalid tuning {cfg}")
+
     def upload(self, dd) -> "ResidentDD":
         h = C.c_void_p()
         err = C.create_string_buffer(1024)
