@@ -77,7 +77,8 @@ def build_workload(tmpdir):
 def read_traffic(kernel):
     """(HBM bytes per launch of `kernel`, source) from the committed rocprofv3
     PMC summary profiles/<kernel>_knapsack_traffic.txt, or (None, None).  The
-    source names the file and the commit it was measured at."""
+    source names the file, the commit it was measured at, the traffic over the
+    compulsory bytes and the counters' VALU issue fraction."""
     p = os.path.join(ROOT, "profiles", f"{kernel}_knapsack_traffic.txt")
     if not os.path.exists(p):
         return None, None
@@ -88,8 +89,13 @@ def read_traffic(kernel):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
     try:
-        return float(vals["hbm_bytes_per_launch"]), \
-            f"profiles/{kernel}_knapsack_traffic.txt (measured at {vals.get('commit', '?')})"
+        return float(vals["hbm_bytes_per_launch"]), {
+            "file": f"profiles/{kernel}_knapsack_traffic.txt",
+            "commit": vals.get("commit"),
+            "traffic_over_compulsory": float(vals.get("traffic_over_compulsory", "nan")),
+            # share of the VALU issue slots filled with any VALU instruction
+            # (SQ_INSTS_VALU), beside the compare-only fraction of `frac`
+            "valu_issue_frac_pmc": float(vals.get("valu_issue_frac", "nan"))}
     except (KeyError, ValueError):
         return None, None
 
