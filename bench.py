@@ -309,9 +309,9 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0)
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests of the serving stream to replay (0: skip; 2000: all)")
-    ap.add_argument("--time-budget-s", type=float, default=480.0,
+    ap.add_argument("--time-budget-s", type=float, default=360.0,
                     help="the Poisson legs of the serving replay are cut to end by then")
-    ap.add_argument("--cpu-budget-s", type=float, default=20.0,
+    ap.add_argument("--cpu-budget-s", type=float, default=15.0,
                     help="the CPU baseline starts no new enumeration after this")
     ap.add_argument("--serving-slots", type=int, default=8)
     ap.add_argument("--serving-method", type=int, default=1)

@@ -39,6 +39,7 @@
 
 #include "../../../include/pmdd_engine.h"
 #include "kernels.hpp"
+#include "plan.hpp"
 
 using pmdd_k::ChunkDesc;
 using pmdd_k::kBlock;
@@ -279,10 +280,14 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // ---------------------------------------------------------------------------
 struct DevFrontier {
     int64_t n_nodes = 0;
-    std::vector<int64_t> h_off; // [n_nodes + 1]
+    std::vector<int64_t> h_off; // [n_nodes + 1], when on_host
     DevBuf off;
     DevBuf pts;
     int64_t cap = 0; // SoA stride
+    // A layer placed on the device has its offsets there only (h_off empty)
+    // until the host asks for them; `step` is the expansion that built it.
+    bool on_host = true;
+    int step = -1;
     int64_t total() const { return h_off.empty() ? 0 : h_off.back(); }
     int64_t size(int64_t v) const { return h_off[v + 1] - h_off[v]; }
 };
@@ -356,6 +361,14 @@ struct pmdd_ctx {
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
     DevBuf join_idx, join_x, join_y, join_ctr, front_count, merge_a, merge_b;
+    // device-planned layers: per sweep direction the plan of its next layer
+    // (arc run offsets, candidate counts, workgroup prefixes); the block the
+    // host reads back (headers and scores) and its pinned mirror; the first
+    // chunk of each node, the prefix scratch of the survivor placement, the
+    // per-expansion figures and run counters
+    DevBuf plan_run_off[2], plan_gamma[2], plan_part[2];
+    DevBuf plan_rb, node_chunk, scan_part, step_stats, run_ctr;
+    pmdd_k::PlanReadback *h_rb = nullptr;
     std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
     std::vector<hipEvent_t> ev_late; // start/end pairs read after a later wait
     Staging staging;
@@ -392,7 +405,10 @@ struct pmdd_ctx {
         for (DevBuf *b : {&run_off, &which, &which_a, &which_b, &which_c, &which_s, &which_m,
                           &surv_pts, &alive, &counter, &join_meta, &keep_a, &keep_b, &sort_a,
                           &sort_b, &big_base, &big_node, &run_start, &grp_run, &join_idx, &join_x,
-                          &join_y, &join_ctr, &front_count, &merge_a, &merge_b, &qtier, &qhead}) {
+                          &join_y, &join_ctr, &front_count, &merge_a, &merge_b, &qtier, &qhead,
+                          &plan_run_off[0], &plan_run_off[1], &plan_gamma[0], &plan_gamma[1],
+                          &plan_part[0], &plan_part[1], &plan_rb, &node_chunk, &scan_part,
+                          &step_stats, &run_ctr}) {
             f(*b);
         }
         for (DevFrontier &fr : frontier_pool) {
@@ -439,6 +455,8 @@ struct FrontierLease {
         }
         f.n_nodes = 0;
         f.h_off.clear();
+        f.on_host = true;
+        f.step = -1;
         f.cap = f.pts.p != nullptr ? static_cast<int64_t>(f.pts.bytes / (4 * static_cast<size_t>(d)))
                                    : 0;
     }
@@ -804,10 +822,10 @@ using FilterFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *,
                           unsigned long long *);
 using CrossFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
                          int64_t, uint8_t *, const int32_t *, const int32_t *, int64_t,
-                         const int64_t *, int, int, unsigned long long *);
+                         const int64_t *, int, int, int, unsigned long long *);
 using AppendFn = void (*)(dim3, hipStream_t, const int32_t *, const ChunkDesc *, const int32_t *,
                           int64_t, const uint8_t *, const int32_t *, int32_t *, int32_t *, int64_t,
-                          const int64_t *, int64_t *, int);
+                          const int64_t *, int64_t *, int, int, unsigned long long *);
 
 struct Launch {
     int chunk_cap;
@@ -891,16 +909,17 @@ template <int D> struct LaunchImpl {
     static void cross(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       const int32_t *pts, int64_t stride, uint8_t *alive, const int32_t *ns,
                       const int32_t *fv, int64_t fvs, const int64_t *fc, int W, int G,
-                      unsigned long long *pt) {
+                      int only_wave, unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_cross_chunks<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
-                           stride, alive, ns, fv, fvs, fc, W, G, pt);
+                           stride, alive, ns, fv, fvs, fc, W, G, only_wave, pt);
     }
     static void append_front(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                              const int32_t *pts, int64_t stride, const uint8_t *alive,
                              const int32_t *ns, int32_t *na, int32_t *fv, int64_t fvs,
-                             const int64_t *fin, int64_t *fout, int W) {
+                             const int64_t *fin, int64_t *fout, int W, int only_wave,
+                             unsigned long long *bytes) {
         hipLaunchKernelGGL((pmdd_k::k_append_front<D, CH>), g, dim3(kBlock), 0, s, which, c, pts,
-                           stride, alive, ns, na, fv, fvs, fin, fout, W);
+                           stride, alive, ns, na, fv, fvs, fin, fout, W, only_wave, D, bytes);
     }
     static void compact(dim3 g, hipStream_t s, const ChunkDesc *c, const int32_t *ns,
                         const int64_t *out, const int32_t *src, int64_t ss, const uint8_t *alive,
@@ -1334,6 +1353,8 @@ void d2h(pmdd_ctx *ctx, std::vector<T> &v, const DevBuf &b, size_t n, const char
 
 void upload_offsets(pmdd_ctx *ctx, DevFrontier &f) {
     h2d(ctx, f.off, f.h_off, "hipMemcpy(frontier offsets)");
+    f.on_host = true;
+    f.step = -1;
 }
 
 // Frontier holding {0^d} at one node of a layer, empty elsewhere.
@@ -1658,12 +1679,13 @@ struct Run {
             hip_check(hipEventRecord(pool_event(2 * launches), s), "hipEventRecord");
             K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + start[w], cb.chunks.as<ChunkDesc>(),
                     pts, stride, alive, cb.nsurv.as<int32_t>(), fv, fv_stride, fin, kCrossWave, G,
-                    t_cross);
+                    -1, t_cross);
             hip_check(hipGetLastError(), "k_cross_chunks");
             hip_check(hipEventRecord(pool_event(2 * launches + 1), s), "hipEventRecord");
             K.append_front(dim3(static_cast<unsigned>(n)), s, wl + start[w],
                            cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(),
-                           cb.nalive.as<int32_t>(), fv, fv_stride, fin, fout, kCrossWave);
+                           cb.nalive.as<int32_t>(), fv, fv_stride, fin, fout, kCrossWave, -1,
+                           nullptr);
             hip_check(hipGetLastError(), "k_append_front");
             ++launches;
         }
@@ -2174,6 +2196,412 @@ struct Run {
         }
     }
 
+    // ---- device-planned layers -------------------------------------------
+    // A layer is planned on the device (plan.hpp) rather than by host loops
+    // over its arcs and nodes: plan_layer queues the counting kernels as soon
+    // as the parent frontier is queued, read_plans reads the small header
+    // back (the one wait per layer), and expand_dev sizes the buffers from it
+    // and queues the rest of the layer, the survivors' placement included.
+    // The host planner (expand) still serves the packed serving layers, a
+    // layer whose candidates exceed max_cand (cut into passes of whole
+    // nodes), and every layer when PMDD_HOST_PLAN is set in the environment.
+    const bool dev_plan = std::getenv("PMDD_HOST_PLAN") == nullptr;
+    pmdd_k::PlanHeader hdr[2]{};
+    int64_t score_rb[2] = {0, 0};
+
+    // One expansion, folded into the statistics at the end of the run.
+    struct StepRec {
+        int layer = -1; // index into the per-layer std arrays (-1: none)
+        int64_t n_nodes = 0, prev_nodes = 0, n_arcs = 0, cand = 0;
+        int prev_step = -1;     // expansion whose survivors the parent holds, or -1
+        int64_t prev_total = 0; // the parent's points when prev_step is -1
+        bool dev = false;       // survivors and candidate std on the device
+        bool dev_size_std = false;
+        int64_t host_surv_pre = 0, host_dropped = 0;
+        double host_std_c = 0.0, host_std_s = 0.0;
+    };
+    std::vector<StepRec> steps;
+    bool want_std = false;
+
+    pmdd_k::PlanShape plan_shape() const {
+        pmdd_k::PlanShape sh;
+        sh.chunk_cap = K.chunk_cap;
+        sh.small_cap = kSmallNode;
+        sh.mid_cap = K.mid_cap;
+        sh.pad = 0;
+        return sh;
+    }
+    pmdd_k::PlanReadback *rb() const { return ctx->plan_rb.as<pmdd_k::PlanReadback>(); }
+    pmdd_k::StepStat *step_stat(int i) const {
+        return ctx->step_stats.as<pmdd_k::StepStat>() + i;
+    }
+    void fill_bytes(void *p, int64_t n) {
+        if (n <= 0) {
+            return;
+        }
+        hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(n, 256))),
+                           dim3(256), 0, s, static_cast<uint8_t *>(p), n, uint8_t(0));
+        hip_check(hipGetLastError(), "k_fill_u8");
+    }
+
+    // Buffers whose size the diagram bounds, sized once per run (a plan is
+    // queued while the other sweep's expansion may still run, so they must
+    // not move during the run), and the run's counters zeroed.
+    void prepare_plan(int sides) {
+        int64_t max_e = 1, max_n = 1;
+        for (int l = 0; l < dd->L; ++l) {
+            max_e = std::max({max_e, dd->td[l].n_arcs, dd->bu[l].n_arcs});
+            max_n = std::max(max_n, dd->layer_size[l]);
+        }
+        for (int k = 0; k < sides; ++k) {
+            ctx->plan_run_off[k].ensure(static_cast<size_t>(max_e) * 8, "hipMalloc(plan runs)");
+            ctx->plan_gamma[k].ensure(static_cast<size_t>(max_n) * 8, "hipMalloc(plan counts)");
+            ctx->plan_part[k].ensure(static_cast<size_t>(ceil_div(max_n, kBlock)) *
+                                         pmdd_k::kPlanPart * 8,
+                                     "hipMalloc(plan prefixes)");
+        }
+        ctx->plan_rb.ensure(sizeof(pmdd_k::PlanReadback), "hipMalloc(plan header)");
+        ctx->node_chunk.ensure(static_cast<size_t>(max_n + 1) * 4, "hipMalloc(node chunks)");
+        const int64_t ss = static_cast<int64_t>(dd->L) * sizeof(pmdd_k::StepStat);
+        ctx->step_stats.ensure(ss, "hipMalloc(expansion figures)");
+        ctx->run_ctr.ensure(64, "hipMalloc(run counters)");
+        fill_bytes(ctx->step_stats.p, ss);
+        fill_bytes(ctx->run_ctr.p, 64);
+        fill_bytes(ctx->plan_rb.p, sizeof(pmdd_k::PlanReadback));
+        want_std = st->std_capacity > 0 && (st->std_candidates_per_layer != nullptr ||
+                                             st->std_frontier_survivors_per_layer != nullptr);
+        steps.reserve(dd->L);
+    }
+
+    // Queues the plan of expanding `prev` through `csr` into n nodes (its
+    // header goes to the readback block of sweep direction `side`).
+    void plan_layer(int side, const DevFrontier &prev, const LayerCSR &csr, int64_t n) {
+        const int64_t nb = ceil_div(n, kBlock);
+        const auto tp = late_pair(&st->kernel_expand_td_s);
+        hip_check(hipEventRecord(tp.first, s), "hipEventRecord");
+        if (nb > 0) {
+            hipLaunchKernelGGL(pmdd_k::k_plan_count, dim3(static_cast<unsigned>(nb)), dim3(kBlock),
+                               0, s, n, csr.d_off, csr.d_src, prev.off.as<int64_t>(),
+                               ctx->plan_run_off[side].as<int64_t>(),
+                               ctx->plan_gamma[side].as<int64_t>(),
+                               ctx->plan_part[side].as<int64_t>(), plan_shape());
+            hip_check(hipGetLastError(), "k_plan_count");
+        }
+        hipLaunchKernelGGL(pmdd_k::k_plan_scan, dim3(1), dim3(kBlock), 0, s, nb,
+                           ctx->plan_part[side].as<int64_t>(), &rb()->hdr[side]);
+        hip_check(hipGetLastError(), "k_plan_scan");
+        hip_check(hipEventRecord(tp.second, s), "hipEventRecord");
+    }
+
+    // Reads the headers and scores back (waits for the stream).
+    void read_plans() {
+        hip_check(hipMemcpyAsync(ctx->h_rb, ctx->plan_rb.p, sizeof(pmdd_k::PlanReadback),
+                                 hipMemcpyDeviceToHost, s),
+                  "hipMemcpy(plan header)");
+        hip_check(hipStreamSynchronize(s), "plan header sync");
+        ctx->staging.reset();
+        flush_late();
+        for (int k = 0; k < 2; ++k) {
+            hdr[k] = ctx->h_rb->hdr[k];
+            score_rb[k] = static_cast<int64_t>(ctx->h_rb->score[k]);
+        }
+    }
+
+    // The host copy of a frontier's offsets (waits for the stream).
+    void ensure_host(DevFrontier &f) {
+        if (f.on_host) {
+            return;
+        }
+        d2h(ctx, f.h_off, f.off, static_cast<size_t>(f.n_nodes + 1), "frontier offsets");
+        flush_late();
+        f.on_host = true;
+    }
+
+    // Queues the score of a just-built frontier (sum of |F(v)| times the
+    // arcs of `deg` at v) into the readback block's slot `side`.
+    void queue_score(int side, const DevFrontier &f, const LayerCSR &deg) {
+        unsigned long long *slot = &rb()->score[side];
+        fill_bytes(slot, sizeof(*slot));
+        if (deg.d_off == nullptr || deg.off.empty() || f.n_nodes == 0) {
+            return;
+        }
+        hipLaunchKernelGGL(pmdd_k::k_layer_score,
+                           dim3(static_cast<unsigned>(ceil_div(f.n_nodes, kBlock))), dim3(kBlock),
+                           0, s, f.n_nodes, f.off.as<int64_t>(), deg.d_off, slot);
+        hip_check(hipGetLastError(), "k_layer_score");
+    }
+
+    // One layer of a sweep: the expansion (device-planned when its plan was
+    // read and fits max_cand, else host-planned), then state dominance when
+    // sd_layer >= 0, and its record.  `layer` indexes the std arrays (-1:
+    // none).
+    void expand_step(int side, DevFrontier &prev, const LayerCSR &csr, int64_t n_nodes,
+                     DevFrontier &next, int layer, int sd_layer) {
+        StepRec rec;
+        rec.layer = want_std ? layer : -1;
+        rec.n_nodes = n_nodes;
+        rec.prev_nodes = prev.n_nodes;
+        rec.n_arcs = csr.n_arcs;
+        rec.prev_step = prev.on_host ? -1 : prev.step;
+        rec.prev_total = prev.on_host ? prev.total() : 0;
+        const bool dev = dev_plan && hdr[side].tot[pmdd_k::PV_CAND] <= opts.max_cand;
+        if (dev) {
+            rec.dev = true;
+            rec.cand = hdr[side].tot[pmdd_k::PV_CAND];
+            rec.dev_size_std = rec.layer >= 0 && sd_layer < 0;
+            expand_dev(side, prev, csr, n_nodes, next, static_cast<int>(steps.size()),
+                       rec.layer >= 0, rec.dev_size_std);
+        } else {
+            ensure_host(prev);
+            std::vector<int64_t> gamma;
+            expand(prev, csr, n_nodes, next, &rec.cand, rec.layer >= 0 ? &gamma : nullptr);
+            rec.host_surv_pre = next.total();
+            rec.host_std_c = pop_std(gamma);
+        }
+        if (sd_layer >= 0) {
+            ensure_host(next);
+            const int64_t before = next.total();
+            state_dominance(sd_layer, next);
+            rec.host_dropped = before - next.total();
+        }
+        if (rec.layer >= 0 && !rec.dev_size_std) {
+            ensure_host(next);
+            std::vector<int64_t> sz(next.n_nodes);
+            for (int64_t v = 0; v < next.n_nodes; ++v) {
+                sz[v] = next.size(v);
+            }
+            rec.host_std_s = pop_std(sz);
+        }
+        steps.push_back(rec);
+    }
+
+    // Device-planned expansion of a layer whose header read_plans has read:
+    // the stream is idle, so buffers are sized here from the header and the
+    // whole layer is queued without waiting.
+    void expand_dev(int side, const DevFrontier &prev, const LayerCSR &csr, int64_t n_nodes,
+                    DevFrontier &next, int step, bool cand_std, bool size_std) {
+        using namespace pmdd_k;
+        const PlanHeader &h = hdr[side];
+        const int64_t C = h.tot[PV_CAND];
+        const int64_t nch = h.tot[PV_CHUNKS];
+        const int64_t n_tiny = h.tot[PV_TINY], n_mid = h.tot[PV_MID], n_small = h.tot[PV_SMALL];
+        const int64_t n_bign = h.tot[PV_BIGN], bigpos = h.tot[PV_BIGSZ], n_runs = h.tot[PV_RUNS];
+        const int64_t n_bigch = nch - n_tiny - n_mid - n_small;
+        ChunkBufs &cb = ctx->cb_main;
+        next.n_nodes = n_nodes;
+        next.h_off.clear();
+        next.on_host = false;
+        next.step = step;
+        next.off.ensure(static_cast<size_t>(n_nodes + 1) * 8, "hipMalloc(frontier offsets)");
+        grow_points(next, d, std::max<int64_t>(C, 1), 0, s);
+        const size_t nc1 = static_cast<size_t>(std::max<int64_t>(nch, 1));
+        cb.chunks.ensure(nc1 * sizeof(ChunkDesc), "hipMalloc(chunk table)");
+        cb.nsurv.ensure(nc1 * 4, "hipMalloc(chunk counts)");
+        cb.nalive.ensure(nc1 * 4, "hipMalloc(chunk counts)");
+        cb.out.ensure((nc1 + 1) * 8, "hipMalloc(chunk outputs)");
+        ctx->which_s.ensure(std::max<int64_t>(n_tiny, 1) * 4, "hipMalloc(chunk list)");
+        ctx->which_m.ensure(std::max<int64_t>(n_mid, 1) * 4, "hipMalloc(chunk list)");
+        ctx->which_a.ensure(std::max<int64_t>(n_small, 1) * 4, "hipMalloc(chunk list)");
+        ctx->which_b.ensure(std::max<int64_t>(n_bigch, 1) * 4, "hipMalloc(chunk list)");
+        ctx->big_node.ensure(std::max<int64_t>(n_bign, 1) * 4, "hipMalloc(big nodes)");
+        ctx->big_base.ensure(static_cast<size_t>(n_bign + 1) * 8, "hipMalloc(big nodes)");
+        ctx->grp_run.ensure(static_cast<size_t>(n_bign + 1) * 4, "hipMalloc(group table)");
+        ctx->run_start.ensure(static_cast<size_t>(n_runs + 1) * 8, "hipMalloc(run table)");
+        ctx->surv_pts.ensure(static_cast<size_t>(std::max<int64_t>(C, 1)) * d * 4,
+                             "hipMalloc(survivor buffer)");
+        ctx->alive.ensure(static_cast<size_t>(std::max<int64_t>(C, 1)), "hipMalloc(alive flags)");
+        if (bigpos > 0) {
+            ctx->sort_a.ensure(static_cast<size_t>(bigpos) * d * 4, "hipMalloc(sort buffer)");
+            ctx->sort_b.ensure(static_cast<size_t>(bigpos) * d * 4, "hipMalloc(sort buffer)");
+        }
+        const int64_t nbc = ceil_div(nch, kBlock);
+        ctx->scan_part.ensure(static_cast<size_t>(std::max<int64_t>(nbc, 1)) * 8,
+                              "hipMalloc(scan prefixes)");
+        sample_memory();
+
+        const auto t_exp = late_pair(&st->kernel_expand_td_s);
+        hip_check(hipEventRecord(t_exp.first, s), "hipEventRecord");
+        PlanTables t;
+        t.chunks = cb.chunks.as<ChunkDesc>();
+        t.node_chunk = ctx->node_chunk.as<int32_t>();
+        t.tiny = ctx->which_s.as<int32_t>();
+        t.mid = ctx->which_m.as<int32_t>();
+        t.small = ctx->which_a.as<int32_t>();
+        t.big_chunks = ctx->which_b.as<int32_t>();
+        t.big_node = ctx->big_node.as<int32_t>();
+        t.big_base = ctx->big_base.as<int64_t>();
+        t.grp_run = ctx->grp_run.as<int32_t>();
+        t.run_start = ctx->run_start.as<int64_t>();
+        t.ssd_gamma = cand_std ? &step_stat(step)->ssd_gamma : nullptr;
+        const int64_t nb = ceil_div(n_nodes, kBlock);
+        if (nb > 0) {
+            hipLaunchKernelGGL(k_plan_write, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s,
+                               n_nodes, csr.d_off, ctx->plan_run_off[side].as<int64_t>(),
+                               ctx->plan_gamma[side].as<int64_t>(),
+                               ctx->plan_part[side].as<int64_t>(), &rb()->hdr[side], t,
+                               plan_shape());
+            hip_check(hipGetLastError(), "k_plan_write");
+        }
+        ExpandSrc src;
+        src.prev_off = prev.off.as<int64_t>();
+        src.prev_pts = prev.pts.as<int32_t>();
+        src.prev_stride = prev.cap;
+        src.arc_off = csr.d_off;
+        src.arc_src = csr.d_src;
+        src.arc_w = csr.d_w;
+        src.w_stride = std::max<int64_t>(csr.n_arcs, 1);
+        src.run_off = ctx->plan_run_off[side].as<int64_t>();
+        int32_t *pts = ctx->surv_pts.as<int32_t>();
+        uint8_t *alive = ctx->alive.as<uint8_t>();
+        int32_t *sorted = nullptr;
+        if (n_bign > 0) {
+            // big nodes: parent runs gathered and merged into key order, then
+            // staged as they are (the cross filter does all of their checks)
+            K.gather_runs(dim3(static_cast<unsigned>(ceil_div(bigpos, kBlock))), s,
+                          ctx->big_base.as<int64_t>(), ctx->big_node.as<int32_t>(),
+                          static_cast<int32_t>(n_bign), src, ctx->sort_a.as<int32_t>(), bigpos);
+            hip_check(hipGetLastError(), "k_gather_runs");
+            sorted = ctx->sort_a.as<int32_t>();
+            int32_t *oth = ctx->sort_b.as<int32_t>();
+            const int passes = ceil_log2(h.max_runs);
+            const dim3 grid(static_cast<unsigned>(ceil_div(bigpos, kBlock)));
+            for (int p = 0; p < passes; ++p) {
+                K.merge_pass(grid, s, sorted, oth, bigpos, ctx->run_start.as<int64_t>(),
+                             ctx->grp_run.as<int32_t>(), static_cast<int32_t>(n_bign), p, bigpos);
+                hip_check(hipGetLastError(), "k_merge_pass");
+                std::swap(sorted, oth);
+            }
+            K.stage(dim3(static_cast<unsigned>(n_bigch)), s, ctx->which_b.as<int32_t>(),
+                    cb.chunks.as<ChunkDesc>(), SortedSrc{sorted, bigpos}, pts, C, alive,
+                    cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>());
+            hip_check(hipGetLastError(), "k_stage_chunks");
+        }
+        hip_check(hipEventRecord(t_exp.second, s), "hipEventRecord");
+        const auto t_fil = late_pair(&st->kernel_filter_s);
+        const auto t_dom = late_pair(&st->kernel_dominance_s);
+        hip_check(hipEventRecord(t_fil.first, s), "hipEventRecord");
+        hip_check(hipEventRecord(t_dom.first, s), "hipEventRecord");
+        const pmdd_tuning &T = ctx->tuning[d];
+        const int64_t counts[3] = {n_tiny, n_mid, n_small};
+        const DevBuf *lists[3] = {&ctx->which_s, &ctx->which_m, &ctx->which_a};
+        int filters = 0;
+        for (int k = 0; k < 3; ++k) {
+            if (counts[k] <= 0) {
+                continue;
+            }
+            K.filter_tier[k][threads_index(T.filter_threads[k])](
+                dim3(static_cast<unsigned>(counts[k])), s, lists[k]->as<int32_t>(),
+                cb.chunks.as<ChunkDesc>(), &src, pts, C, alive, cb.nsurv.as<int32_t>(),
+                cb.nalive.as<int32_t>(), d_tests);
+            hip_check(hipGetLastError(), "k_filter_chunks");
+            ++filters;
+        }
+        hip_check(hipEventRecord(t_fil.second, s), "hipEventRecord");
+        if (n_bign > 0) {
+            // every wave's chunks in one list; a launch per wave takes its own
+            const int waves = static_cast<int>(ceil_div(h.max_chunks, kCrossWave));
+            ctx->front_count.ensure(2 * nc1 * sizeof(int64_t), "hipMalloc(front counts)");
+            fill_bytes(ctx->front_count.p, static_cast<int64_t>(2 * nc1 * sizeof(int64_t)));
+            int64_t *fc = ctx->front_count.as<int64_t>();
+            const int64_t per_wave = ceil_div(n_bigch, waves);
+            const int G = static_cast<int>(std::min<int64_t>(
+                T.cross_split_max, std::max<int64_t>(1, ceil_div(T.cross_workgroups, per_wave))));
+            unsigned long long *cbytes = ctx->run_ctr.as<unsigned long long>();
+            for (int w = 0; w < waves; ++w) {
+                int64_t *fin = fc + (w % 2) * static_cast<int64_t>(nc1);
+                int64_t *fout = fc + ((w + 1) % 2) * static_cast<int64_t>(nc1);
+                const int only = waves > 1 ? w : -1;
+                const auto t_x = late_pair(&st->kernel_cross_s);
+                hip_check(hipEventRecord(t_x.first, s), "hipEventRecord");
+                K.cross(dim3(static_cast<unsigned>(n_bigch * G)), s, ctx->which_b.as<int32_t>(),
+                        cb.chunks.as<ChunkDesc>(), pts, C, alive, cb.nsurv.as<int32_t>(), sorted,
+                        bigpos, fin, kCrossWave, G, only, t_cross);
+                hip_check(hipGetLastError(), "k_cross_chunks");
+                hip_check(hipEventRecord(t_x.second, s), "hipEventRecord");
+                K.append_front(dim3(static_cast<unsigned>(n_bigch)), s, ctx->which_b.as<int32_t>(),
+                               cb.chunks.as<ChunkDesc>(), pts, C, alive, cb.nsurv.as<int32_t>(),
+                               cb.nalive.as<int32_t>(), sorted, bigpos, fin, fout, kCrossWave,
+                               only, cbytes);
+                hip_check(hipGetLastError(), "k_append_front");
+            }
+            st->cross_launches += waves;
+        }
+        hip_check(hipEventRecord(t_dom.second, s), "hipEventRecord");
+        st->filter_launches += filters;
+        st->filter_candidates += C;
+
+        // placement: chunk output slots, node offsets, then the survivors
+        const auto t_pl = late_pair(&st->kernel_expand_td_s);
+        hip_check(hipEventRecord(t_pl.first, s), "hipEventRecord");
+        int64_t *slot = cb.out.as<int64_t>();
+        if (nbc > 0) {
+            hipLaunchKernelGGL(k_block_sums, dim3(static_cast<unsigned>(nbc)), dim3(kBlock), 0, s,
+                               nch, cb.nalive.as<int32_t>(), ctx->scan_part.as<int64_t>());
+            hip_check(hipGetLastError(), "k_block_sums");
+        }
+        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kBlock), 0, s, nbc,
+                           ctx->scan_part.as<int64_t>(), nch, int64_t(0), slot,
+                           &step_stat(step)->surv);
+        hip_check(hipGetLastError(), "k_scan_blocks");
+        if (nbc > 0) {
+            hipLaunchKernelGGL(k_scan_write, dim3(static_cast<unsigned>(nbc)), dim3(kBlock), 0, s,
+                               nch, cb.nalive.as<int32_t>(), ctx->scan_part.as<int64_t>(),
+                               int64_t(0), slot);
+            hip_check(hipGetLastError(), "k_scan_write");
+        }
+        hipLaunchKernelGGL(k_node_offsets, dim3(static_cast<unsigned>(ceil_div(n_nodes + 1, kBlock))),
+                           dim3(kBlock), 0, s, n_nodes, ctx->node_chunk.as<int32_t>(), slot,
+                           next.off.as<int64_t>(),
+                           size_std ? &step_stat(step)->ssd_size : static_cast<double *>(nullptr));
+        hip_check(hipGetLastError(), "k_node_offsets");
+        if (nch > 0) {
+            K.compact(dim3(static_cast<unsigned>(nch)), s, cb.chunks.as<ChunkDesc>(),
+                      cb.nsurv.as<int32_t>(), slot, pts, C, alive, next.pts.as<int32_t>(),
+                      next.cap);
+            hip_check(hipGetLastError(), "k_compact_chunks");
+        }
+        hip_check(hipEventRecord(t_pl.second, s), "hipEventRecord");
+    }
+
+    // Expansion records into the statistics (the stream is idle).
+    void fold_steps() {
+        if (steps.empty()) {
+            return;
+        }
+        std::vector<pmdd_k::StepStat> ss;
+        d2h(ctx, ss, ctx->step_stats, steps.size(), "expansion figures");
+        std::vector<unsigned long long> ctr;
+        d2h(ctx, ctr, ctx->run_ctr, 1, "run counters");
+        st->cross_bytes += static_cast<int64_t>(ctr[0]);
+        std::vector<int64_t> post(steps.size(), 0);
+        for (size_t i = 0; i < steps.size(); ++i) {
+            const StepRec &r = steps[i];
+            const int64_t pre = r.dev ? ss[i].surv : r.host_surv_pre;
+            const int64_t surv = pre - ss[i].dropped - r.host_dropped;
+            post[i] = surv;
+            count_layer(r.cand, surv);
+            st->dominance_filtered_total += ss[i].dropped;
+            if (r.dev) {
+                const int64_t prev_total = r.prev_step >= 0 ? post[r.prev_step] : r.prev_total;
+                st->algorithmic_bytes +=
+                    4 * (static_cast<int64_t>(d) * (prev_total + pre) + r.n_arcs * (d + 1) +
+                         2 * (r.n_nodes + 1) + (r.prev_nodes + 1));
+            }
+            if (r.layer >= 0 && r.layer < st->std_capacity) {
+                const double n = static_cast<double>(std::max<int64_t>(r.n_nodes, 1));
+                if (st->std_candidates_per_layer != nullptr) {
+                    st->std_candidates_per_layer[r.layer] =
+                        r.dev ? std::sqrt(ss[i].ssd_gamma / n) : r.host_std_c;
+                }
+                if (st->std_frontier_survivors_per_layer != nullptr) {
+                    st->std_frontier_survivors_per_layer[r.layer] =
+                        r.dev_size_std ? std::sqrt(ss[i].ssd_size / n) : r.host_std_s;
+                }
+            }
+        }
+        steps.clear();
+    }
+
     // Method 1 (enumerate_bdd_topdown / enumerate_mdd_topdown).
     void topdown(pmdd_frontier *out) {
         const int L = dd->L;
@@ -2181,26 +2609,21 @@ struct Run {
         DevFrontier &cur = lc.f, &nxt = ln.f;
         seed_frontier(ctx, cur, d, dd->layer_size[0], dd->root);
         const bool sd = state_dominance_applies();
-        std::vector<int64_t> gamma;
+        prepare_plan(1);
+        if (dev_plan) {
+            plan_layer(0, cur, dd->td[1], dd->layer_size[1]);
+        }
         for (int l = 1; l < L; ++l) {
-            int64_t cand = 0;
-            expand(cur, dd->td[l], dd->layer_size[l], nxt, &cand, &gamma);
-            if (sd) {
-                state_dominance(l, nxt);
+            if (dev_plan) {
+                read_plans();
             }
-            count_layer(cand, nxt.total());
-            if (st->std_candidates_per_layer != nullptr && l < st->std_capacity) {
-                st->std_candidates_per_layer[l] = pop_std(gamma);
-            }
-            if (st->std_frontier_survivors_per_layer != nullptr && l < st->std_capacity) {
-                std::vector<int64_t> sz(nxt.n_nodes);
-                for (int64_t v = 0; v < nxt.n_nodes; ++v) {
-                    sz[v] = nxt.size(v);
-                }
-                st->std_frontier_survivors_per_layer[l] = pop_std(sz);
+            expand_step(0, cur, dd->td[l], dd->layer_size[l], nxt, l, sd ? l : -1);
+            if (dev_plan && l + 1 < L) {
+                plan_layer(0, nxt, dd->td[l + 1], dd->layer_size[l + 1]);
             }
             std::swap(cur, nxt);
         }
+        ensure_host(cur);
         fetch_points(cur, cur.h_off[dd->terminal], cur.size(dd->terminal), out);
     }
 
@@ -2214,30 +2637,43 @@ struct Run {
         FrontierLease lc(ctx, d), ln(ctx, d);
         DevFrontier &cur = lc.f, &nxt = ln.f;
         seed_frontier(ctx, cur, d, dd->layer_size[L - 1], dd->terminal);
-        std::vector<int64_t> gamma;
+        prepare_plan(1);
+        if (dev_plan) {
+            plan_layer(0, cur, dd->bu[L - 2], dd->layer_size[L - 2]);
+        }
         for (int l = L - 2; l >= 0; --l) {
-            int64_t cand = 0;
-            expand(cur, dd->bu[l], dd->layer_size[l], nxt, &cand, &gamma);
-            count_layer(cand, nxt.total());
-            if (st->std_candidates_per_layer != nullptr && l < st->std_capacity) {
-                st->std_candidates_per_layer[l] = pop_std(gamma);
+            if (dev_plan) {
+                read_plans();
             }
-            if (st->std_frontier_survivors_per_layer != nullptr && l < st->std_capacity) {
-                std::vector<int64_t> sz(nxt.n_nodes);
-                for (int64_t v = 0; v < nxt.n_nodes; ++v) {
-                    sz[v] = nxt.size(v);
-                }
-                st->std_frontier_survivors_per_layer[l] = pop_std(sz);
+            expand_step(0, cur, dd->bu[l], dd->layer_size[l], nxt, l, -1);
+            if (dev_plan && l > 0) {
+                plan_layer(0, nxt, dd->bu[l - 1], dd->layer_size[l - 1]);
             }
             std::swap(cur, nxt);
         }
+        ensure_host(cur);
         fetch_points(cur, cur.h_off[dd->root], cur.size(dd->root), out);
+    }
+
+    // Expansion score of a frontier (sum of |F(v)| times v's arcs in `deg`),
+    // on the host.
+    static int64_t host_score(const DevFrontier &f, const LayerCSR &deg) {
+        int64_t score = 0;
+        for (int64_t v = 0; v < f.n_nodes; ++v) {
+            const int64_t k = deg.off.empty() ? 0 : deg.off[v + 1] - deg.off[v];
+            score += f.size(v) * k;
+        }
+        return score;
     }
 
     // Method 3 (coupled_bdd_cuda_enumerate enum.cu:595-756, enumerate_mdd_coupled
     // enum.cu:295-436): expand top-down while its score is not above the
     // bottom-up score, else bottom-up; the scores are sum |F(v)| * out-arcs(v)
     // and trunc(1.5 * sum |F(v)| * in-arcs(v)); stop where the sweeps meet.
+    // Device-planned: after a sweep's expansion its score and the plan of its
+    // next layer are queued together and read back in one wait; the other
+    // sweep's plan, queued when it last moved, is still valid (its frontier
+    // has not changed).
     void coupled(pmdd_frontier *out) {
         const int L = dd->L;
         FrontierLease lt_(ctx, d), lb_(ctx, d), lm_(ctx, d);
@@ -2250,38 +2686,44 @@ struct Run {
         const bool sd = state_dominance_applies();
         int lt = 0, lb = L - 1;
         int64_t td_score = 0, bu_score = 0;
+        prepare_plan(2);
+        if (dev_plan) {
+            plan_layer(0, td, dd->td[1], dd->layer_size[1]);
+            plan_layer(1, bu, dd->bu[L - 2], dd->layer_size[L - 2]);
+            read_plans();
+        }
         while (lt != lb) {
-            int64_t cand = 0;
             if (td_score <= bu_score) {
                 ++lt;
-                expand(td, dd->td[lt], dd->layer_size[lt], tmp, &cand, nullptr);
+                expand_step(0, td, dd->td[lt], dd->layer_size[lt], tmp, -1, sd ? lt : -1);
                 std::swap(td, tmp);
-                if (sd) {
-                    state_dominance(lt, td);
+                if (!dev_plan) {
+                    td_score = host_score(td, dd->bu[lt]);
+                } else if (lt != lb) {
+                    queue_score(0, td, dd->bu[lt]);
+                    plan_layer(0, td, dd->td[lt + 1], dd->layer_size[lt + 1]);
+                    read_plans();
+                    td_score = score_rb[0];
                 }
-                count_layer(cand, td.total());
-                const LayerCSR &outs = dd->bu[lt];
-                int64_t score = 0;
-                for (int64_t v = 0; v < td.n_nodes; ++v) {
-                    const int64_t deg = outs.off.empty() ? 0 : outs.off[v + 1] - outs.off[v];
-                    score += td.size(v) * deg;
-                }
-                td_score = score;
             } else {
                 --lb;
-                expand(bu, dd->bu[lb], dd->layer_size[lb], tmp, &cand, nullptr);
+                expand_step(1, bu, dd->bu[lb], dd->layer_size[lb], tmp, -1, -1);
                 std::swap(bu, tmp);
-                count_layer(cand, bu.total());
-                const LayerCSR &ins = dd->td[lb];
                 int64_t score = 0;
-                for (int64_t v = 0; v < bu.n_nodes; ++v) {
-                    const int64_t deg = ins.off.empty() ? 0 : ins.off[v + 1] - ins.off[v];
-                    score += bu.size(v) * deg;
+                if (!dev_plan) {
+                    score = host_score(bu, dd->td[lb]);
+                } else if (lt != lb) {
+                    queue_score(1, bu, dd->td[lb]);
+                    plan_layer(1, bu, dd->bu[lb - 1], dd->layer_size[lb - 1]);
+                    read_plans();
+                    score = score_rb[1];
                 }
                 bu_score = static_cast<int64_t>(1.5 * static_cast<double>(score));
             }
         }
         st->layer_coupling = lt;
+        ensure_host(td);
+        ensure_host(bu);
         const double t0 = now_s();
         couple(td, bu, out);
         st->wall_join_s += now_s() - t0;
@@ -2711,6 +3153,7 @@ struct Run {
     void finish() {
         hip_check(hipStreamSynchronize(s), "finish sync");
         flush_late();
+        fold_steps();
         if (d_tests != nullptr) {
             unsigned long long t[4] = {0, 0, 0, 0};
             hip_check(hipMemcpyAsync(t, d_tests, sizeof(t), hipMemcpyDeviceToHost, s),
@@ -2895,6 +3338,9 @@ int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
         hip_check(hipEventCreate(&c->ev_e), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_f), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_g), "hipEventCreate");
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_rb), sizeof(pmdd_k::PlanReadback),
+                                hipHostMallocDefault),
+                  "hipHostMalloc(plan readback)");
         *out = c.release();
         return PMDD_OK;
     });
@@ -2931,6 +3377,9 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
     }
     if (ctx->stream != nullptr) {
         (void)hipStreamDestroy(ctx->stream);
+    }
+    if (ctx->h_rb != nullptr) {
+        (void)hipHostFree(ctx->h_rb);
     }
     delete ctx;
 }

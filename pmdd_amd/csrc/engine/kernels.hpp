@@ -579,7 +579,8 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     const int32_t *__restrict__ which, const ChunkDesc *__restrict__ chunks,
     const int32_t *__restrict__ pts, int64_t stride, uint8_t *alive,
     const int32_t *__restrict__ chunk_nsurv, const int32_t *__restrict__ fv, int64_t fv_stride,
-    const int64_t *__restrict__ fcount, int W, int G, unsigned long long *__restrict__ pair_tests) {
+    const int64_t *__restrict__ fcount, int W, int G, int only_wave,
+    unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
     constexpr int R = CH / kBlock; // rows of points per thread
     constexpr int kU = (kBlock * DP > CH * D) ? kBlock * DP : CH * D;
@@ -591,10 +592,13 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     const int me = which[blockIdx.x / G];
     const int g = static_cast<int>(blockIdx.x % G);
     const ChunkDesc c = chunks[me];
+    const int k_in = me - c.group_first;
+    if (only_wave >= 0 && k_in / W != only_wave) {
+        return; // the list holds every wave's chunks; this launch is one wave
+    }
     const int nS = chunk_nsurv[me];
     const int tid = threadIdx.x;
     volatile uint8_t *valive = alive;
-    const int k_in = me - c.group_first;
     const int wave_first = c.group_first + (k_in / W) * W;
     const int64_t fbase = c.start - static_cast<int64_t>(k_in) * CH;
     const int64_t fn = fcount[c.group_first];
@@ -800,11 +804,15 @@ __global__ __launch_bounds__(kBlock) void k_append_front(
     const int32_t *__restrict__ pts, int64_t stride, const uint8_t *__restrict__ alive,
     const int32_t *__restrict__ chunk_nsurv, int32_t *__restrict__ chunk_nalive,
     int32_t *__restrict__ fv, int64_t fv_stride, const int64_t *__restrict__ fcount_in,
-    int64_t *__restrict__ fcount_out, int W) {
+    int64_t *__restrict__ fcount_out, int W, int only_wave, int d,
+    unsigned long long *__restrict__ bytes) {
     __shared__ int wave_tot[kWaves];
     const int me = which[blockIdx.x];
     const ChunkDesc c = chunks[me];
     const int k_in = me - c.group_first;
+    if (only_wave >= 0 && k_in / W != only_wave) {
+        return;
+    }
     const int wave_first = c.group_first + (k_in / W) * W;
     const int wave_last = min(wave_first + W, c.group_first + c.group_count) - 1;
     const int64_t fbase = c.start - static_cast<int64_t>(k_in) * CH;
@@ -840,6 +848,17 @@ __global__ __launch_bounds__(kBlock) void k_append_front(
         chunk_nalive[me] = written;
         if (me == wave_last) {
             fcount_out[c.group_first] = at + written;
+        }
+        if (bytes != nullptr) {
+            // compulsory bytes of the wave: each point read once, its flag
+            // once, each survivor appended once, and (once per group and
+            // wave, by its last chunk) the front read once
+            unsigned long long b = 4ull * d * (static_cast<unsigned long long>(c.len) + written) +
+                                   static_cast<unsigned long long>(c.len);
+            if (me == wave_last) {
+                b += 4ull * d * static_cast<unsigned long long>(fcount_in[c.group_first]);
+            }
+            atomicAdd(bytes, b);
         }
     }
 }

@@ -1,0 +1,500 @@
+// Device planning of a layer expansion.
+//
+// The chunk filters of kernels.hpp work from tables: a chunk descriptor per
+// chunk (a node of at most CH candidates is one chunk, a bigger node is cut
+// into CH-candidate blocks of its merged, key-ordered parent runs), a list of
+// chunks per filter tier, and for the big nodes their parent-run tables.  These
+// kernels build those tables on the device from the parent frontier's device
+// offsets and the layer's arc CSR, so planning a layer needs no host loop over
+// its arcs and nodes:
+//
+//   k_plan_count  one thread per node: the node-local offset of each arc's
+//                 candidate run (run_off), the node's candidate count (gamma),
+//                 and per workgroup the sums of the node's plan values and the
+//                 largest big node (chunks, arcs)
+//   k_plan_scan   one workgroup: exclusive prefix of the workgroup sums, the
+//                 layer totals into the plan header (the one thing the host
+//                 reads back: it sizes the buffers and picks the launches)
+//   k_plan_write  one thread per node: its prefix of every plan value, then
+//                 its chunk descriptors, tier-list entries and big-node tables
+//
+// After the filters, the survivors are placed without the host as well:
+//   k_block_sums / k_scan_blocks / k_scan_write  exclusive prefix of the
+//                 per-chunk survivor counts (the chunks' output slots)
+//   k_node_offsets  the new frontier's node offsets from the chunk slots
+//
+// The tables are the ones the host planner builds (the same chunk order,
+// lists and run tables), so every filter launch is unchanged.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace pmdd_k {
+
+// Plan values of a node, summed over the layer as exclusive prefixes.
+enum PlanVal {
+    PV_CAND = 0, // candidates (gamma)
+    PV_CHUNKS,   // chunk descriptors
+    PV_TINY,     // 1 if one chunk of at most small_cap candidates
+    PV_MID,      // 1 if one chunk of at most mid_cap (when mid_cap > 0)
+    PV_SMALL,    // 1 if one chunk otherwise
+    PV_BIGN,     // 1 if more than CH candidates (a big node)
+    PV_BIGSZ,    // candidates of a big node
+    PV_RUNS,     // arcs into a big node (its parent runs)
+    kPlanVals
+};
+// Per workgroup of k_plan_count: the kPlanVals sums, then the two maxima.
+constexpr int kPlanPart = kPlanVals + 2;
+
+// What the host reads back after planning a layer.
+struct PlanHeader {
+    int64_t tot[kPlanVals]; // layer totals of the plan values
+    int64_t max_chunks;     // most chunks of one big node (0: none)
+    int64_t max_runs;       // most arcs into one big node (0: none)
+};
+
+// Per-expansion figures accumulated on the device and read once at the end
+// of an enumeration.
+struct StepStat {
+    int64_t surv;      // survivors of the expansion (before state dominance)
+    int64_t dropped;   // labels state dominance removed
+    double ssd_gamma;  // sum over nodes of (candidates - mean)^2
+    double ssd_size;   // sum over nodes of (|F(v)| - mean)^2, after state dominance
+};
+
+// The device block the host reads back in one copy: the plan header of each
+// sweep direction's next layer and each direction's expansion score.
+struct PlanReadback {
+    PlanHeader hdr[2];
+    unsigned long long score[2];
+};
+
+struct PlanShape {
+    int32_t chunk_cap; // CH
+    int32_t small_cap; // tiny tier bound
+    int32_t mid_cap;   // mid tier bound (0: no mid tier)
+    int32_t pad;
+};
+
+// Plan values of one node from its candidate count and in-degree.
+__device__ __forceinline__ void plan_values(int64_t g, int64_t deg, const PlanShape &sh,
+                                            int64_t (&pv)[kPlanVals]) {
+#pragma unroll
+    for (int k = 0; k < kPlanVals; ++k) {
+        pv[k] = 0;
+    }
+    pv[PV_CAND] = g;
+    if (g <= 0) {
+        return;
+    }
+    if (g <= sh.chunk_cap) {
+        pv[PV_CHUNKS] = 1;
+        if (g <= sh.small_cap) {
+            pv[PV_TINY] = 1;
+        } else if (sh.mid_cap > 0 && g <= sh.mid_cap) {
+            pv[PV_MID] = 1;
+        } else {
+            pv[PV_SMALL] = 1;
+        }
+    } else {
+        pv[PV_CHUNKS] = (g + sh.chunk_cap - 1) / sh.chunk_cap;
+        pv[PV_BIGN] = 1;
+        pv[PV_BIGSZ] = g;
+        pv[PV_RUNS] = deg;
+    }
+}
+
+// Wave-wide inclusive prefix sum (64 lanes).
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int64_t u = __shfl_up(v, off);
+        if (lane >= off) {
+            v += u;
+        }
+    }
+    return v;
+}
+
+__device__ __forceinline__ int64_t wave_sum(int64_t v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+    }
+    return v;
+}
+
+__device__ __forceinline__ int64_t wave_max(int64_t v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int64_t u = __shfl_xor(v, off);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+    }
+    return v;
+}
+
+// Exclusive prefix of one int64 per thread over a kBlock-thread workgroup;
+// *total gets the workgroup sum.  Opens with a barrier (safe after any other
+// use of lds) and ends with one (lds may be reused at once).  All threads
+// call it.
+__device__ __forceinline__ int64_t block_exscan64(int64_t v, int64_t *lds, int64_t *total) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int64_t incl = wave_incl_scan(v);
+    __syncthreads();
+    if (lane == kWave - 1) {
+        lds[wave] = incl;
+    }
+    __syncthreads();
+    int64_t before = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const int64_t t = lds[w];
+        before += (w < wave) ? t : 0;
+        sum += t;
+    }
+    __syncthreads();
+    *total = sum;
+    return before + incl - v;
+}
+
+// Workgroup sum of a double (one atomic add per workgroup into *acc).
+__device__ __forceinline__ void block_add_f64(double v, double *lds, double *acc) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    v = wave_sum_f64(v);
+    __syncthreads();
+    if (lane == 0) {
+        lds[wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kWaves; ++w) {
+            s += lds[w];
+        }
+        atomicAdd(acc, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_plan_count: grid ceil(N / kBlock).  run_off[e] for every arc of the
+// layer, gamma[v] for every node, part[b * kPlanPart ..] per workgroup.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_plan_count(
+    int64_t n_nodes, const int64_t *__restrict__ arc_off, const int32_t *__restrict__ arc_src,
+    const int64_t *__restrict__ prev_off, int64_t *__restrict__ run_off,
+    int64_t *__restrict__ gamma, int64_t *__restrict__ part, PlanShape sh) {
+    __shared__ int64_t red[kWaves][kPlanPart];
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    int64_t pv[kPlanVals];
+    int64_t mx_chunks = 0, mx_runs = 0;
+    if (v < n_nodes) {
+        const int64_t e0 = arc_off[v];
+        const int64_t e1 = arc_off[v + 1];
+        int64_t acc = 0;
+        for (int64_t e = e0; e < e1; ++e) {
+            run_off[e] = acc;
+            const int32_t p = arc_src[e];
+            acc += prev_off[p + 1] - prev_off[p];
+        }
+        gamma[v] = acc;
+        plan_values(acc, e1 - e0, sh, pv);
+        if (pv[PV_BIGN] != 0) {
+            mx_chunks = pv[PV_CHUNKS];
+            mx_runs = pv[PV_RUNS];
+        }
+    } else {
+        plan_values(0, 0, sh, pv);
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < kPlanVals; ++k) {
+        const int64_t s = wave_sum(pv[k]);
+        if (lane == 0) {
+            red[wave][k] = s;
+        }
+    }
+    mx_chunks = wave_max(mx_chunks);
+    mx_runs = wave_max(mx_runs);
+    if (lane == 0) {
+        red[wave][kPlanVals] = mx_chunks;
+        red[wave][kPlanVals + 1] = mx_runs;
+    }
+    __syncthreads();
+    if (threadIdx.x < kPlanPart) {
+        const int k = threadIdx.x;
+        int64_t r = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const int64_t x = red[w][k];
+            r = (k < kPlanVals) ? r + x : (x > r ? x : r);
+        }
+        part[static_cast<int64_t>(blockIdx.x) * kPlanPart + k] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_plan_scan: one workgroup.  The nb workgroup sums become exclusive
+// prefixes in place; the totals and maxima go to *hdr.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_plan_scan(int64_t nb, int64_t *__restrict__ part,
+                                                      PlanHeader *__restrict__ hdr) {
+    __shared__ int64_t lds[kWaves];
+    int64_t carry[kPlanVals];
+#pragma unroll
+    for (int k = 0; k < kPlanVals; ++k) {
+        carry[k] = 0;
+    }
+    int64_t mx_chunks = 0, mx_runs = 0;
+    for (int64_t base = 0; base < nb; base += kBlock) {
+        const int64_t b = base + threadIdx.x;
+        int64_t *row = part + b * kPlanPart;
+        const bool ok = b < nb;
+#pragma unroll
+        for (int k = 0; k < kPlanVals; ++k) {
+            const int64_t x = ok ? row[k] : 0;
+            int64_t tot = 0;
+            const int64_t ex = block_exscan64(x, lds, &tot);
+            if (ok) {
+                row[k] = carry[k] + ex;
+            }
+            carry[k] += tot;
+        }
+        if (ok) {
+            mx_chunks = row[kPlanVals] > mx_chunks ? row[kPlanVals] : mx_chunks;
+            mx_runs = row[kPlanVals + 1] > mx_runs ? row[kPlanVals + 1] : mx_runs;
+        }
+    }
+    __shared__ int64_t mx[2][kWaves];
+    mx_chunks = wave_max(mx_chunks);
+    mx_runs = wave_max(mx_runs);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        mx[0][wave] = mx_chunks;
+        mx[1][wave] = mx_runs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kPlanVals; ++k) {
+            hdr->tot[k] = carry[k];
+        }
+        int64_t a = 0, r = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            a = mx[0][w] > a ? mx[0][w] : a;
+            r = mx[1][w] > r ? mx[1][w] : r;
+        }
+        hdr->max_chunks = a;
+        hdr->max_runs = r;
+    }
+}
+
+// Tables k_plan_write fills (sized from the plan header).
+struct PlanTables {
+    ChunkDesc *chunks;   // [tot CHUNKS]
+    int32_t *node_chunk; // [N + 1] first chunk of each node; [N] = tot CHUNKS
+    int32_t *tiny;       // [tot TINY] chunk ids per tier
+    int32_t *mid;        // [tot MID]
+    int32_t *small;      // [tot SMALL]
+    int32_t *big_chunks; // chunks of the big nodes, in chunk order
+    int32_t *big_node;   // [tot BIGN]
+    int64_t *big_base;   // [tot BIGN + 1] first slot of each big node in the run buffer
+    int32_t *grp_run;    // [tot BIGN + 1] first run of each big node
+    int64_t *run_start;  // [tot RUNS + 1] first slot of each run
+    double *ssd_gamma;   // null, or += sum over nodes of (gamma - mean)^2
+};
+
+// ---------------------------------------------------------------------------
+// k_plan_write: grid ceil(N / kBlock), after k_plan_scan.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_plan_write(
+    int64_t n_nodes, const int64_t *__restrict__ arc_off, const int64_t *__restrict__ run_off,
+    const int64_t *__restrict__ gamma, const int64_t *__restrict__ part,
+    const PlanHeader *__restrict__ hdr, PlanTables t, PlanShape sh) {
+    __shared__ int64_t lds[kWaves];
+    __shared__ double flds[kWaves];
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const bool ok = v < n_nodes;
+    const int64_t g = ok ? gamma[v] : 0;
+    const int64_t e0 = ok ? arc_off[v] : 0;
+    const int64_t deg = ok ? arc_off[v + 1] - e0 : 0;
+    int64_t pv[kPlanVals];
+    plan_values(g, deg, sh, pv);
+    const int64_t *bp = part + static_cast<int64_t>(blockIdx.x) * kPlanPart;
+    int64_t at[kPlanVals];
+#pragma unroll
+    for (int k = 0; k < kPlanVals; ++k) {
+        int64_t tot = 0;
+        at[k] = bp[k] + block_exscan64(pv[k], lds, &tot);
+    }
+    if (ok) {
+        const int32_t first = static_cast<int32_t>(at[PV_CHUNKS]);
+        t.node_chunk[v] = first;
+        if (pv[PV_CHUNKS] == 1) {
+            ChunkDesc c;
+            c.start = 0;
+            c.out_base = at[PV_CAND];
+            c.group = static_cast<int32_t>(v);
+            c.len = static_cast<int32_t>(g);
+            c.group_first = first;
+            c.group_count = 1;
+            t.chunks[first] = c;
+            if (pv[PV_TINY] != 0) {
+                t.tiny[at[PV_TINY]] = first;
+            } else if (pv[PV_MID] != 0) {
+                t.mid[at[PV_MID]] = first;
+            } else {
+                t.small[at[PV_SMALL]] = first;
+            }
+        } else if (pv[PV_BIGN] != 0) {
+            const int64_t bi = at[PV_BIGN];
+            const int64_t bpos = at[PV_BIGSZ];
+            t.big_node[bi] = static_cast<int32_t>(v);
+            t.big_base[bi] = bpos;
+            t.grp_run[bi] = static_cast<int32_t>(at[PV_RUNS]);
+            for (int64_t j = 0; j < deg; ++j) {
+                t.run_start[at[PV_RUNS] + j] = bpos + run_off[e0 + j];
+            }
+            const int32_t cnt = static_cast<int32_t>(pv[PV_CHUNKS]);
+            // the big nodes' chunks follow each other in chunk order
+            const int64_t list0 = at[PV_CHUNKS] - (at[PV_TINY] + at[PV_MID] + at[PV_SMALL]);
+            for (int32_t k = 0; k < cnt; ++k) {
+                ChunkDesc c;
+                c.start = bpos + static_cast<int64_t>(k) * sh.chunk_cap;
+                c.out_base = at[PV_CAND] + static_cast<int64_t>(k) * sh.chunk_cap;
+                c.group = static_cast<int32_t>(v);
+                const int64_t rest = g - static_cast<int64_t>(k) * sh.chunk_cap;
+                c.len = static_cast<int32_t>(rest < sh.chunk_cap ? rest : sh.chunk_cap);
+                c.group_first = first;
+                c.group_count = cnt;
+                t.chunks[first + k] = c;
+                t.big_chunks[list0 + k] = first + k;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        t.node_chunk[n_nodes] = static_cast<int32_t>(hdr->tot[PV_CHUNKS]);
+        t.big_base[hdr->tot[PV_BIGN]] = hdr->tot[PV_BIGSZ];
+        t.grp_run[hdr->tot[PV_BIGN]] = static_cast<int32_t>(hdr->tot[PV_RUNS]);
+        t.run_start[hdr->tot[PV_RUNS]] = hdr->tot[PV_BIGSZ];
+    }
+    if (t.ssd_gamma != nullptr) {
+        const double mean = static_cast<double>(hdr->tot[PV_CAND]) / static_cast<double>(n_nodes);
+        const double dv = ok ? static_cast<double>(g) - mean : 0.0;
+        block_add_f64(dv * dv, flds, t.ssd_gamma);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive prefix of n int32 counts into int64 slots: out[i] = base + sum of
+// cnt[0 .. i), out[n] = base + sum of all.  k_block_sums and k_scan_write run
+// on ceil(n / kBlock) workgroups (none when n is 0), k_scan_blocks on one.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_block_sums(int64_t n, const int32_t *__restrict__ cnt,
+                                                       int64_t *__restrict__ part) {
+    __shared__ int64_t lds[kWaves];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    int64_t tot = 0;
+    block_exscan64(i < n ? cnt[i] : 0, lds, &tot);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = tot;
+    }
+}
+
+// The nb workgroup sums become exclusive prefixes; out[n] = base + total and,
+// when total_out is set, *total_out = total.
+__global__ __launch_bounds__(kBlock) void k_scan_blocks(int64_t nb, int64_t *__restrict__ part,
+                                                        int64_t n, int64_t base,
+                                                        int64_t *__restrict__ out,
+                                                        int64_t *__restrict__ total_out) {
+    __shared__ int64_t lds[kWaves];
+    int64_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += kBlock) {
+        const int64_t b = b0 + threadIdx.x;
+        int64_t tot = 0;
+        const int64_t ex = block_exscan64(b < nb ? part[b] : 0, lds, &tot);
+        if (b < nb) {
+            part[b] = carry + ex;
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        out[n] = base + carry;
+        if (total_out != nullptr) {
+            *total_out = carry;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_write(int64_t n, const int32_t *__restrict__ cnt,
+                                                       const int64_t *__restrict__ part,
+                                                       int64_t base, int64_t *__restrict__ out) {
+    __shared__ int64_t lds[kWaves];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    int64_t tot = 0;
+    const int64_t ex = block_exscan64(i < n ? cnt[i] : 0, lds, &tot);
+    if (i < n) {
+        out[i] = base + part[blockIdx.x] + ex;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_node_offsets: grid ceil((N + 1) / kBlock).  off[v] = slot[node_chunk[v]]
+// for v in 0..N (node_chunk[N] = chunk count, slot[chunk count] = the end).
+// With ssd set, adds the sum over nodes of (|F(v)| - mean)^2.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_node_offsets(int64_t n_nodes,
+                                                         const int32_t *__restrict__ node_chunk,
+                                                         const int64_t *__restrict__ slot,
+                                                         int64_t *__restrict__ off,
+                                                         double *__restrict__ ssd) {
+    __shared__ double flds[kWaves];
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (v <= n_nodes) {
+        off[v] = slot[node_chunk[v]];
+    }
+    if (ssd != nullptr) {
+        const double total =
+            static_cast<double>(slot[node_chunk[n_nodes]] - slot[node_chunk[0]]);
+        const double mean = total / static_cast<double>(n_nodes);
+        double dv = 0.0;
+        if (v < n_nodes) {
+            dv = static_cast<double>(slot[node_chunk[v + 1]] - slot[node_chunk[v]]) - mean;
+        }
+        block_add_f64(dv * dv, flds, ssd);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_layer_score: *score += sum over nodes of |F(v)| * (deg_off[v+1] -
+// deg_off[v]) (method 3's expansion score; *score zeroed before).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_layer_score(int64_t n_nodes,
+                                                        const int64_t *__restrict__ off,
+                                                        const int64_t *__restrict__ deg_off,
+                                                        unsigned long long *__restrict__ score) {
+    __shared__ int64_t lds[kWaves];
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    int64_t x = 0;
+    if (v < n_nodes) {
+        x = (off[v + 1] - off[v]) * (deg_off[v + 1] - deg_off[v]);
+    }
+    int64_t tot = 0;
+    block_exscan64(x, lds, &tot);
+    if (threadIdx.x == 0 && tot != 0) {
+        atomicAdd(score, static_cast<unsigned long long>(tot));
+    }
+}
+
+} // namespace pmdd_k
