@@ -240,64 +240,68 @@ __device__ __forceinline__ void load_row(const int32_t *row, int32_t (&q)[D]) {
     }
 }
 
-// q >= m in every objective.
+// Dominance margin of row q over point m: the least over objectives of
+// q[o] - m[o], each a saturating int32 subtraction (so its sign is exact).
+// q >= m in every objective iff the margin is >= 0.  A scan keeps the running
+// maximum of the margins of the rows it meets and tests the sign once: per
+// row that is D subtractions with clamp, a v_min3 chain and a v_max on the
+// VALU, with no lane-mask arithmetic on the scalar unit (a chain of d
+// v_cmp / s_and pairs per row left both units half idle).
 template <int D>
-__device__ __forceinline__ bool ge_vals(const int32_t (&q)[D], const int32_t (&m)[D]) {
-    bool r = q[0] >= m[0];
+__device__ __forceinline__ int32_t margin_vals(const int32_t (&q)[D], const int32_t (&m)[D]) {
+    int32_t r = __builtin_elementwise_sub_sat(q[0], m[0]);
 #pragma unroll
     for (int o = 1; o < D; ++o) {
-        r &= q[o] >= m[o];
+        r = min(r, __builtin_elementwise_sub_sat(q[o], m[o]));
     }
     return r;
 }
 
-// row >= m in every objective; rows are 16-byte aligned, padded to DP ints.
+// The margin of a staged row (16-byte aligned, padded to DP ints).
 template <int D, int DP>
-__device__ __forceinline__ bool ge_all(const int32_t *row, const int32_t (&m)[D]) {
-    const int4 a = *reinterpret_cast<const int4 *>(row);
-    bool r = a.x >= m[0];
-    if (D > 1) r &= a.y >= m[D > 1 ? 1 : 0];
-    if (D > 2) r &= a.z >= m[D > 2 ? 2 : 0];
-    if (D > 3) r &= a.w >= m[D > 3 ? 3 : 0];
-    if (D > 4) {
-        const int4 b = *reinterpret_cast<const int4 *>(row + 4);
-        r &= b.x >= m[D > 4 ? 4 : 0];
-        if (D > 5) r &= b.y >= m[D > 5 ? 5 : 0];
-        if (D > 6) r &= b.z >= m[D > 6 ? 6 : 0];
-        if (D > 7) r &= b.w >= m[D > 7 ? 7 : 0];
-    }
-    return r;
+__device__ __forceinline__ int32_t margin_row(const int32_t *row, const int32_t (&m)[D]) {
+    int32_t q[D];
+    load_row<D, DP>(row, q);
+    return margin_vals<D>(q, m);
 }
 
 // Rows [0, n) of a staged table (DP ints per row, every lane reading the
-// same row), of which a lane only meets rows below its own bound lim <= n,
-// against the lane's point m: returns dead, or'ed with whether some such row
-// is >= m.  Rows go eight at a time so that their LDS reads overlap, and the
-// wave stops once none of its lanes is alive; n is the same in every lane.
-// tests counts the rows a lane met while alive.
-template <int D, int DP>
+// same row) against the lane's point m: returns dead, or'ed with whether
+// some row the lane meets is >= m.  With LIM a lane meets only the rows
+// below its own bound lim <= n (else every row).  Rows go eight at a time so
+// that their LDS reads overlap, and the wave stops once none of its lanes is
+// alive; n is the same in every lane.  tests counts the rows a lane met
+// while alive.
+template <int D, int DP, bool LIM>
 __device__ __forceinline__ bool scan_until_dead(const int32_t *S, int n, int lim,
                                                 const int32_t (&m)[D], bool dead,
                                                 unsigned long long &tests) {
     constexpr int kB = 8;
+    const int bound = LIM ? lim : n;
     int s = 0;
     for (; s + kB <= n; s += kB) {
-        if (!__any(!dead && s < lim)) {
+        if (!__any(!dead && s < bound)) {
             return dead;
         }
         if (!dead) {
-            tests += static_cast<unsigned long long>(max(0, min(kB, lim - s)));
+            tests += static_cast<unsigned long long>(max(0, min(kB, bound - s)));
         }
+        int32_t acc = INT32_MIN;
 #pragma unroll
         for (int k = 0; k < kB; ++k) {
-            int32_t q[D];
-            load_row<D, DP>(&S[(s + k) * DP], q);
-            dead |= (s + k < lim) & ge_vals<D>(q, m);
+            int32_t mg = margin_row<D, DP>(&S[(s + k) * DP], m);
+            if (LIM) {
+                // a row at or past the bound gets a negative margin; below
+                // it the bound term is >= 0 and leaves the sign as it was
+                mg = min(mg, bound - 1 - s - k);
+            }
+            acc = max(acc, mg);
         }
+        dead |= acc >= 0;
     }
-    for (; s < n && s < lim && !dead; ++s) {
+    for (; s < n && s < bound && !dead; ++s) {
         ++tests;
-        dead = ge_all<D, DP>(&S[s * DP], m);
+        dead = margin_row<D, DP>(&S[s * DP], m) >= 0;
     }
     return dead;
 }
@@ -395,11 +399,11 @@ __device__ __forceinline__ void filter_chunk(int ci, const ChunkDesc *__restrict
         __syncthreads();
         bool dead = !valid;
         // against survivors of earlier steps
-        dead = scan_until_dead<D, DP>(S, nS, nS, m, dead, tests);
+        dead = scan_until_dead<D, DP, false>(S, nS, nS, m, dead, tests);
         // against earlier candidates of this step (slots below the lane's own)
         const int wave_end = __builtin_amdgcn_readfirstlane(
             min(tcount, static_cast<int>((tid | (kWave - 1)))));
-        dead = scan_until_dead<D, DP>(S + nS * DP, wave_end, min(tid, tcount), m, dead, tests);
+        dead = scan_until_dead<D, DP, true>(S + nS * DP, wave_end, min(tid, tcount), m, dead, tests);
         // block_rank's opening barrier ends every lane's reads of S before
         // the survivors are written after it
         int total = 0;
@@ -524,6 +528,11 @@ template <int D, int DP, int NR, int R, bool OWN>
 __device__ __forceinline__ void scan_rows(const int32_t *U, const int16_t *TS, int tn,
                                           const int32_t (&m)[R][D], const int (&slot)[R],
                                           bool (&dead)[R]) {
+    int32_t acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        acc[r] = INT32_MIN;
+    }
 #pragma unroll 2
     for (int j = 0; j < tn; ++j) {
         int32_t q[D];
@@ -531,12 +540,16 @@ __device__ __forceinline__ void scan_rows(const int32_t *U, const int16_t *TS, i
         const int sj = OWN ? TS[j] : 0;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            bool t = ge_vals<D>(q, m[r]);
+            int32_t mg = margin_vals<D>(q, m[r]);
             if (OWN) {
-                t &= sj < slot[r];
+                mg = sj < slot[r] ? mg : INT32_MIN;
             }
-            dead[r] |= t;
+            acc[r] = max(acc[r], mg);
         }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        dead[r] |= acc[r] >= 0;
     }
 }
 
@@ -945,7 +958,7 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
                 }
             }
             __syncthreads();
-            dead = scan_until_dead<D, DP>(T, tn, tn, m, dead, tests);
+            dead = scan_until_dead<D, DP, false>(T, tn, tn, m, dead, tests);
         }
     }
     if (valid && dead) {
@@ -1326,16 +1339,18 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
                                             hi - lo - static_cast<int64_t>(tile) * kWave));
         bool dead = false;
         if (holds) {
+            int32_t acc = INT32_MIN;
             if (je - j0 == kSub) {
 #pragma unroll
                 for (int j = 0; j < kSub; ++j) {
-                    dead |= ge_all<D, DP>(&T[(j0 + j) * DP], m);
+                    acc = max(acc, margin_row<D, DP>(&T[(j0 + j) * DP], m));
                 }
             } else {
                 for (int j = j0; j < je; ++j) {
-                    dead |= ge_all<D, DP>(&T[j * DP], m);
+                    acc = max(acc, margin_row<D, DP>(&T[j * DP], m));
                 }
             }
+            dead = acc >= 0;
             tests += static_cast<unsigned long long>(je - j0);
         }
         bool survive = false;
