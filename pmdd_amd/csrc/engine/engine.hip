@@ -35,6 +35,7 @@
 #include <new>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/pmdd_engine.h"
@@ -360,7 +361,7 @@ struct pmdd_ctx {
     DevBuf run_off, which, which_a, which_b, which_c, which_s, which_m;
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
-    DevBuf join_idx, join_x, join_y, join_ctr, front_count, merge_a, merge_b;
+    DevBuf join_x, join_y, join_ctr, front_count, merge_a, merge_b;
     // device-planned layers: per sweep direction the plan of its next layer
     // (arc run offsets, candidate counts, workgroup prefixes); the block the
     // host reads back (headers and scores) and its pinned mirror; the first
@@ -368,6 +369,7 @@ struct pmdd_ctx {
     // per-expansion figures and run counters
     DevBuf plan_run_off[2], plan_gamma[2], plan_part[2];
     DevBuf plan_rb, node_chunk, scan_part, step_stats, run_ctr;
+    DevBuf sd_flag, sd_pos, sd_active, sd_cmp; // device-planned state dominance
     pmdd_k::PlanReadback *h_rb = nullptr;
     std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
     std::vector<hipEvent_t> ev_late; // start/end pairs read after a later wait
@@ -404,11 +406,11 @@ struct pmdd_ctx {
         }
         for (DevBuf *b : {&run_off, &which, &which_a, &which_b, &which_c, &which_s, &which_m,
                           &surv_pts, &alive, &counter, &join_meta, &keep_a, &keep_b, &sort_a,
-                          &sort_b, &big_base, &big_node, &run_start, &grp_run, &join_idx, &join_x,
+                          &sort_b, &big_base, &big_node, &run_start, &grp_run, &join_x,
                           &join_y, &join_ctr, &front_count, &merge_a, &merge_b, &qtier, &qhead,
                           &plan_run_off[0], &plan_run_off[1], &plan_gamma[0], &plan_gamma[1],
                           &plan_part[0], &plan_part[1], &plan_rb, &node_chunk, &scan_part,
-                          &step_stats, &run_ctr}) {
+                          &step_stats, &run_ctr, &sd_flag, &sd_pos, &sd_active, &sd_cmp}) {
             f(*b);
         }
         for (DevFrontier &fr : frontier_pool) {
@@ -492,6 +494,9 @@ struct pmdd_resident {
     std::vector<std::vector<int32_t>> single_arc;    // per layer: arc value of that parent
     std::vector<std::vector<int32_t>> sd_partner;    // per layer, set packing (or empty)
     DevBuf arena;
+    // device copies of the state dominance tables, per layer (null: none)
+    DevBuf sd_arena;
+    std::vector<const int32_t *> d_mw_order, d_single_parent, d_single_arc, d_partner;
 };
 
 namespace {
@@ -786,6 +791,42 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
             }
         }
     }
+    // state dominance tables on the device (the device-planned rule reads them)
+    {
+        r->d_mw_order.assign(L, nullptr);
+        r->d_single_parent.assign(L, nullptr);
+        r->d_single_arc.assign(L, nullptr);
+        r->d_partner.assign(L, nullptr);
+        std::vector<std::pair<const std::vector<int32_t> *, const int32_t **>> parts;
+        for (int l = 0; l < L; ++l) {
+            if (!r->mw_order.empty()) {
+                parts.push_back({&r->mw_order[l], &r->d_mw_order[l]});
+                parts.push_back({&r->single_parent[l], &r->d_single_parent[l]});
+                parts.push_back({&r->single_arc[l], &r->d_single_arc[l]});
+            }
+            if (!r->sd_partner.empty()) {
+                parts.push_back({&r->sd_partner[l], &r->d_partner[l]});
+            }
+        }
+        size_t sbytes = 0;
+        for (const auto &pr : parts) {
+            sbytes += align(pr.first->size() * 4);
+        }
+        if (sbytes > 0) {
+            r->sd_arena.ensure(sbytes, "hipMalloc(state dominance tables)");
+            size_t off = 0;
+            for (const auto &pr : parts) {
+                int32_t *p = reinterpret_cast<int32_t *>(r->sd_arena.as<char>() + off);
+                *pr.second = p;
+                if (!pr.first->empty()) {
+                    hip_check(hipMemcpyAsync(p, pr.first->data(), pr.first->size() * 4,
+                                             hipMemcpyHostToDevice, ctx->stream),
+                              "hipMemcpy(state dominance tables)");
+                }
+                off += align(pr.first->size() * 4);
+            }
+        }
+    }
     hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
     return r.release();
 }
@@ -843,8 +884,8 @@ struct Launch {
                   int32_t *, int64_t, uint8_t *, int32_t *, int32_t *);
     void (*compact)(dim3, hipStream_t, const ChunkDesc *, const int32_t *, const int64_t *,
                     const int32_t *, int64_t, const uint8_t *, int32_t *, int64_t);
-    void (*state_dom)(dim3, hipStream_t, const int32_t *, const int64_t *, const int32_t *,
-                      int64_t, uint8_t *, unsigned long long *);
+    void (*state_dom)(dim3, hipStream_t, const int32_t *, const int64_t *, const int64_t *,
+                      const int32_t *, int64_t, uint8_t *, unsigned long long *);
     void (*compact_nodes)(dim3, hipStream_t, const int64_t *, const int32_t *, int64_t,
                           const uint8_t *, const int64_t *, int32_t *, int64_t);
     void (*gather_runs)(dim3, hipStream_t, const int64_t *, const int32_t *, int32_t,
@@ -889,6 +930,9 @@ constexpr int64_t kRefillBlock = 8192;
 constexpr int64_t kRefillWaves = 8192;
 // Blocks of a batch frontier when it removes the running frontier's points.
 constexpr int64_t kEvictBlock = 2048;
+// Workgroups of a device-planned state dominance launch (each takes targets
+// in turn; about eight per compute unit are resident at once).
+constexpr int64_t kSdGrid = 4096;
 // Largest coupling batch (products) once the first batches have built R.
 constexpr int64_t kJoinBatchCap = int64_t(1) << 25;
 // The same for requests of a server (adaptive batches): their admission
@@ -927,11 +971,11 @@ template <int D> struct LaunchImpl {
         hipLaunchKernelGGL((pmdd_k::k_compact_chunks<D>), g, dim3(kBlock), 0, s, c, ns, out, src,
                            ss, alive, dst, ds);
     }
-    static void state_dom(dim3 g, hipStream_t s, const int32_t *t, const int64_t *off,
-                          const int32_t *pts, int64_t stride, uint8_t *keep,
+    static void state_dom(dim3 g, hipStream_t s, const int32_t *t, const int64_t *count,
+                          const int64_t *off, const int32_t *pts, int64_t stride, uint8_t *keep,
                           unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_state_dominance<D>), g, dim3(kBlock), 0, s, t, off, pts,
-                           stride, keep, pt);
+        hipLaunchKernelGGL((pmdd_k::k_state_dominance<D>), g, dim3(kBlock), 0, s, t, count, off,
+                           pts, stride, keep, pt);
     }
     static void compact_nodes(dim3 g, hipStream_t s, const int64_t *oi, const int32_t *src,
                               int64_t ss, const uint8_t *keep, const int64_t *oo, int32_t *dst,
@@ -2090,8 +2134,8 @@ struct Run {
                            dim3(256), 0, s, ctx->keep_a.as<uint8_t>(), total, uint8_t(1));
         h2d(ctx, ctx->which, targets, "hipMemcpy(dominance targets)");
         K.state_dom(dim3(static_cast<unsigned>(targets.size() / 4)), s, ctx->which.as<int32_t>(),
-                    f.off.as<int64_t>(), f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>(),
-                    t_sd);
+                    nullptr, f.off.as<int64_t>(), f.pts.as<int32_t>(), f.cap,
+                    ctx->keep_a.as<uint8_t>(), t_sd);
         hip_check(hipGetLastError(), "k_state_dominance");
         // kept per node
         ctx->join_meta.ensure(static_cast<size_t>(n) * 8, "hipMalloc(kept counts)");
@@ -2132,6 +2176,112 @@ struct Run {
         f.off.swap(sd_spare.off);
         f.pts.swap(sd_spare.pts);
         std::swap(f.cap, sd_spare.cap);
+    }
+
+    // State dominance on a layer whose offsets are on the device: the rule of
+    // state_dominance with its targets planned by kernels (plan.hpp); f is
+    // compacted into the spare frontier, which then becomes f, and the labels
+    // removed are counted into expansion `step`.  cap bounds f's points (the
+    // layer's candidates).
+    void state_dominance_dev(int l, DevFrontier &f, int step, int64_t cap, bool size_std) {
+        using namespace pmdd_k;
+        const int64_t n = f.n_nodes;
+        const bool knap = opts.problem_type != 2;
+        const int32_t *order = knap ? dd->d_mw_order[l] : nullptr;
+        const int32_t *sp = knap ? dd->d_single_parent[l] : nullptr;
+        const int32_t *sa = knap ? dd->d_single_arc[l] : nullptr;
+        const int32_t *partner = knap ? nullptr : dd->d_partner[l];
+        const int64_t nb = ceil_div(n, kBlock);
+        const int64_t max_targets = n + ceil_div(std::max<int64_t>(cap, 1), kBlock);
+        ctx->sd_flag.ensure(static_cast<size_t>(std::max<int64_t>(n, 1)) * 4, "hipMalloc(sd flags)");
+        ctx->sd_pos.ensure(static_cast<size_t>(n + 1) * 8, "hipMalloc(sd positions)");
+        ctx->sd_active.ensure(static_cast<size_t>(std::max<int64_t>(n, 1)) * 4,
+                              "hipMalloc(sd nodes)");
+        ctx->sd_cmp.ensure(static_cast<size_t>(std::max<int64_t>(n, 1)) * 8, "hipMalloc(sd pairs)");
+        ctx->scan_part.ensure(static_cast<size_t>(std::max<int64_t>(nb, 1)) * 8,
+                              "hipMalloc(scan prefixes)");
+        ctx->which.ensure(static_cast<size_t>(max_targets) * 16, "hipMalloc(dominance targets)");
+        ctx->keep_a.ensure(static_cast<size_t>(std::max<int64_t>(cap, 1)), "hipMalloc(keep flags)");
+        ctx->join_meta.ensure(static_cast<size_t>(std::max<int64_t>(n, 1)) * 8,
+                              "hipMalloc(kept counts)");
+        sd_spare.n_nodes = n;
+        sd_spare.h_off.clear();
+        sd_spare.off.ensure(static_cast<size_t>(n + 1) * 8, "hipMalloc(frontier offsets)");
+        grow_points(sd_spare, d, std::max<int64_t>(cap, 1), 0, s);
+        sample_memory();
+
+        const auto t_dom = late_pair(&st->kernel_dominance_s);
+        const auto t_sdk = late_pair(&st->kernel_state_dominance_s);
+        hip_check(hipEventRecord(t_dom.first, s), "hipEventRecord");
+        hip_check(hipEventRecord(t_sdk.first, s), "hipEventRecord");
+        int32_t *flag = ctx->sd_flag.as<int32_t>();
+        int64_t *pos = ctx->sd_pos.as<int64_t>();
+        int64_t *part = ctx->scan_part.as<int64_t>();
+        const dim3 g(static_cast<unsigned>(nb));
+        // exclusive prefix of cnt[0 .. n) into out[0 .. n]
+        auto prefix = [&](auto *cnt, int64_t *out) {
+            using C = std::remove_const_t<std::remove_pointer_t<decltype(cnt)>>;
+            hipLaunchKernelGGL(k_block_sums<C>, g, dim3(kBlock), 0, s, n, cnt, part);
+            hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kBlock), 0, s, nb, part, n, int64_t(0),
+                               out, static_cast<int64_t *>(nullptr));
+            hipLaunchKernelGGL(k_scan_write<C>, g, dim3(kBlock), 0, s, n, cnt, part, int64_t(0),
+                               out);
+            hip_check(hipGetLastError(), "prefix scan");
+        };
+        // 1. the nodes holding labels, in the rule's order
+        hipLaunchKernelGGL(k_sd_flags, g, dim3(kBlock), 0, s, n, order, f.off.as<int64_t>(), flag);
+        hip_check(hipGetLastError(), "k_sd_flags");
+        prefix(static_cast<const int32_t *>(flag), pos);
+        hipLaunchKernelGGL(k_sd_active, g, dim3(kBlock), 0, s, n, order, flag, pos,
+                           ctx->sd_active.as<int32_t>());
+        hip_check(hipGetLastError(), "k_sd_active");
+        // 2. comparison pairs and targets (the counts reuse the flags)
+        hipLaunchKernelGGL(k_sd_count, g, dim3(kBlock), 0, s, n, pos + n,
+                           ctx->sd_active.as<int32_t>(), f.off.as<int64_t>(), sp, sa, partner,
+                           ctx->sd_cmp.as<int32_t>(), flag);
+        hip_check(hipGetLastError(), "k_sd_count");
+        prefix(static_cast<const int32_t *>(flag), pos);
+        hipLaunchKernelGGL(k_sd_targets, g, dim3(kBlock), 0, s, n, flag, pos,
+                           ctx->sd_active.as<int32_t>(), ctx->sd_cmp.as<int32_t>(),
+                           ctx->which.as<int32_t>());
+        hip_check(hipGetLastError(), "k_sd_targets");
+        // 3. the rule, over the pre-filter labels
+        hipLaunchKernelGGL(k_fill_u8, dim3(static_cast<unsigned>(ceil_div(std::max<int64_t>(cap, 1), 256))),
+                           dim3(256), 0, s, ctx->keep_a.as<uint8_t>(), std::max<int64_t>(cap, 1),
+                           uint8_t(1));
+        K.state_dom(dim3(static_cast<unsigned>(std::min<int64_t>(max_targets, kSdGrid))), s,
+                    ctx->which.as<int32_t>(), pos + n, f.off.as<int64_t>(), f.pts.as<int32_t>(),
+                    f.cap, ctx->keep_a.as<uint8_t>(), t_sd);
+        hip_check(hipGetLastError(), "k_state_dominance");
+        // 4. kept labels per node, the new offsets, the compaction
+        hipLaunchKernelGGL(k_count_kept, dim3(static_cast<unsigned>(n)), dim3(kBlock), 0, s,
+                           f.off.as<int64_t>(), ctx->keep_a.as<uint8_t>(),
+                           ctx->join_meta.as<int64_t>(), n);
+        hip_check(hipGetLastError(), "k_count_kept");
+        prefix(static_cast<const int64_t *>(ctx->join_meta.as<int64_t>()),
+               sd_spare.off.as<int64_t>());
+        hipLaunchKernelGGL(k_sd_dropped, dim3(1), dim3(64), 0, s, f.off.as<int64_t>() + n,
+                           sd_spare.off.as<int64_t>() + n, &step_stat(step)->dropped);
+        hip_check(hipGetLastError(), "k_sd_dropped");
+        K.compact_nodes(dim3(static_cast<unsigned>(n)), s, f.off.as<int64_t>(),
+                        f.pts.as<int32_t>(), f.cap, ctx->keep_a.as<uint8_t>(),
+                        sd_spare.off.as<int64_t>(), sd_spare.pts.as<int32_t>(), sd_spare.cap);
+        hip_check(hipGetLastError(), "k_compact_nodes");
+        if (size_std) {
+            hipLaunchKernelGGL(k_size_ssd, g, dim3(kBlock), 0, s, n, sd_spare.off.as<int64_t>(),
+                               &step_stat(step)->ssd_size);
+            hip_check(hipGetLastError(), "k_size_ssd");
+        }
+        hip_check(hipEventRecord(t_sdk.second, s), "hipEventRecord");
+        hip_check(hipEventRecord(t_dom.second, s), "hipEventRecord");
+        st->state_dominance_launches += 1;
+        // the compacted labels become the layer; the spare keeps the old
+        // buffers for the next layer
+        f.off.swap(sd_spare.off);
+        f.pts.swap(sd_spare.pts);
+        std::swap(f.cap, sd_spare.cap);
+        f.h_off.clear();
+        f.on_host = false;
     }
 
     static double pop_std(const std::vector<int64_t> &x) {
@@ -2345,12 +2495,16 @@ struct Run {
         rec.prev_step = prev.on_host ? -1 : prev.step;
         rec.prev_total = prev.on_host ? prev.total() : 0;
         const bool dev = dev_plan && hdr[side].tot[pmdd_k::PV_CAND] <= opts.max_cand;
+        const int step = static_cast<int>(steps.size());
         if (dev) {
             rec.dev = true;
             rec.cand = hdr[side].tot[pmdd_k::PV_CAND];
-            rec.dev_size_std = rec.layer >= 0 && sd_layer < 0;
-            expand_dev(side, prev, csr, n_nodes, next, static_cast<int>(steps.size()),
-                       rec.layer >= 0, rec.dev_size_std);
+            rec.dev_size_std = rec.layer >= 0;
+            expand_dev(side, prev, csr, n_nodes, next, step, rec.layer >= 0,
+                       rec.layer >= 0 && sd_layer < 0);
+            if (sd_layer >= 0) {
+                state_dominance_dev(sd_layer, next, step, rec.cand, rec.layer >= 0);
+            }
         } else {
             ensure_host(prev);
             std::vector<int64_t> gamma;
@@ -2358,7 +2512,7 @@ struct Run {
             rec.host_surv_pre = next.total();
             rec.host_std_c = pop_std(gamma);
         }
-        if (sd_layer >= 0) {
+        if (sd_layer >= 0 && !dev) {
             ensure_host(next);
             const int64_t before = next.total();
             state_dominance(sd_layer, next);
@@ -2827,7 +2981,6 @@ struct Run {
             st->join_batch_peak_products = std::max(st->join_batch_peak_products, bprod);
             const int32_t Kn = static_cast<int32_t>(pieces.size());
             std::vector<int64_t> meta(4 * static_cast<size_t>(Kn) + 1);
-            std::vector<int64_t> idx;
             int64_t *prod_off = meta.data();
             int64_t *row_base = meta.data() + Kn + 1;
             int64_t *col_base = row_base + Kn;
@@ -2836,27 +2989,18 @@ struct Run {
             int64_t batch_inputs = 0;
             for (int32_t k = 0; k < Kn; ++k) {
                 const Piece &pc = pieces[k];
-                row_base[k] = static_cast<int64_t>(idx.size());
-                for (int64_t a = pc.r0; a < pc.r1; ++a) {
-                    idx.push_back(td.h_off[pc.v] + a);
-                }
-                col_base[k] = static_cast<int64_t>(idx.size());
-                for (int64_t b = pc.c0; b < pc.c1; ++b) {
-                    idx.push_back(bu.h_off[pc.v] + b);
-                }
+                row_base[k] = td.h_off[pc.v] + pc.r0;
+                col_base[k] = bu.h_off[pc.v] + pc.c0;
                 col_size[k] = pc.c1 - pc.c0;
                 prod_off[k + 1] = prod_off[k] + (pc.r1 - pc.r0) * (pc.c1 - pc.c0);
                 batch_inputs += (pc.r1 - pc.r0) + (pc.c1 - pc.c0);
             }
             h2d(ctx, ctx->join_meta, meta, "hipMemcpy(batch table)");
-            h2d(ctx, ctx->join_idx, idx, "hipMemcpy(batch rows)");
             pmdd_k::JoinSrc src;
             src.prod_off = ctx->join_meta.as<int64_t>();
             src.row_base = src.prod_off + Kn + 1;
             src.col_base = src.row_base + Kn;
             src.col_size = src.col_base + Kn;
-            src.row_idx = ctx->join_idx.as<int64_t>();
-            src.col_idx = src.row_idx;
             src.K = Kn;
             src.td_pts = td.pts.as<int32_t>();
             src.td_stride = td.cap;
@@ -3246,7 +3390,9 @@ int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmd
         run.finish();
     }
     // the diagram counts too when it was uploaded outside this call
-    const int64_t diagram = dd->arena.acct == &ctx->acct ? 0 : static_cast<int64_t>(dd->arena.bytes);
+    const int64_t diagram = dd->arena.acct == &ctx->acct
+                                ? 0
+                                : static_cast<int64_t>(dd->arena.bytes + dd->sd_arena.bytes);
     st->ctx_mem_peak_bytes = ctx->acct.peak.load(std::memory_order_relaxed) + diagram;
     st->mem_need_peak_bytes = ctx->acct.need_peak + diagram;
     return PMDD_OK;

@@ -86,17 +86,16 @@ struct ExpandSrc {
     const int64_t *run_off; // [E]
 };
 
-// Candidate source of a coupling batch: for batch node k, the products of its
-// kept TD rows and BU columns, td[row_idx[row_base[k] + a]] +
-// bu[col_idx[col_base[k] + b]], numbered a * col_size[k] + b after
-// prod_off[k].
+// Candidate source of a coupling batch: piece k (a node's TD rows
+// r0 .. r0 + rows times its BU columns c0 .. c0 + col_size[k]) contributes the
+// products td[row_base[k] + a] + bu[col_base[k] + b], numbered
+// a * col_size[k] + b after prod_off[k]; row_base and col_base are the first
+// TD and BU points of the piece (a node's points are contiguous).
 struct JoinSrc {
     const int64_t *prod_off; // [K + 1]
     const int64_t *row_base; // [K]
     const int64_t *col_base; // [K]
     const int64_t *col_size; // [K]
-    const int64_t *row_idx;  // TD point of each kept row
-    const int64_t *col_idx;  // BU point of each kept column
     int32_t K;
     const int32_t *td_pts;
     int64_t td_stride;
@@ -145,8 +144,8 @@ __device__ __forceinline__ void load_candidate(const JoinSrc &s, const ChunkDesc
     const int64_t bs = s.col_size[lo];
     const int64_t a = local / bs;
     const int64_t b = local - a * bs;
-    const int64_t ti = s.row_idx[s.row_base[lo] + a];
-    const int64_t bi = s.col_idx[s.col_base[lo] + b];
+    const int64_t ti = s.row_base[lo] + a;
+    const int64_t bi = s.col_base[lo] + b;
 #pragma unroll
     for (int o = 0; o < D; ++o) {
         p[o] = s.td_pts[o * s.td_stride + ti] + s.bu_pts[o * s.bu_stride + bi];
@@ -910,15 +909,20 @@ __global__ __launch_bounds__(kBlock) void k_compact_chunks(
 // when some label of one of its (up to two) comparison nodes is >= it in every
 // objective.  Reads the unfiltered layer; writes keep flags per point.
 // ---------------------------------------------------------------------------
+// Targets: one per workgroup (count null), or *count of them taken by a
+// fixed grid, workgroup b taking targets b, b + grid, ...
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_state_dominance(
     const int32_t *__restrict__ targets, // [n][4]: node, cmpA, cmpB (-1 = none), first label
+    const int64_t *__restrict__ count,
     const int64_t *__restrict__ off, const int32_t *__restrict__ pts, int64_t stride,
     uint8_t *__restrict__ keep, unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
     __shared__ __align__(16) int32_t T[kBlock * DP];
     unsigned long long tests = 0;
-    const int32_t *tg = targets + 4 * static_cast<int64_t>(blockIdx.x);
+    const int64_t nt = count != nullptr ? *count : static_cast<int64_t>(gridDim.x);
+    for (int64_t ti = blockIdx.x; ti < nt; ti += gridDim.x) {
+    const int32_t *tg = targets + 4 * ti;
     const int node = tg[0];
     const int64_t b0 = off[node];
     const int64_t n = off[node + 1] - b0;
@@ -963,6 +967,8 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
     }
     if (valid && dead) {
         keep[b0 + i] = 0;
+    }
+    __syncthreads(); // every read of T is done before the next target's tiles
     }
     if (pair_tests != nullptr && tests != 0) {
         atomicAdd(pair_tests, tests);

@@ -397,11 +397,12 @@ __global__ __launch_bounds__(kBlock) void k_plan_write(
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive prefix of n int32 counts into int64 slots: out[i] = base + sum of
+// Exclusive prefix of n counts (int32 or int64) into int64 slots: out[i] = base + sum of
 // cnt[0 .. i), out[n] = base + sum of all.  k_block_sums and k_scan_write run
 // on ceil(n / kBlock) workgroups (none when n is 0), k_scan_blocks on one.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_block_sums(int64_t n, const int32_t *__restrict__ cnt,
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_block_sums(int64_t n, const C *__restrict__ cnt,
                                                        int64_t *__restrict__ part) {
     __shared__ int64_t lds[kWaves];
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -437,7 +438,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_blocks(int64_t nb, int64_t *__r
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_write(int64_t n, const int32_t *__restrict__ cnt,
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_scan_write(int64_t n, const C *__restrict__ cnt,
                                                        const int64_t *__restrict__ part,
                                                        int64_t base, int64_t *__restrict__ out) {
     __shared__ int64_t lds[kWaves];
@@ -495,6 +497,120 @@ __global__ __launch_bounds__(kBlock) void k_layer_score(int64_t n_nodes,
     if (threadIdx.x == 0 && tot != 0) {
         atomicAdd(score, static_cast<unsigned long long>(tot));
     }
+}
+
+// ---------------------------------------------------------------------------
+// State dominance planned on the device: the comparison targets of a layer,
+// one per 256 labels of a node that has comparison nodes, as the host rule
+// builds them.  Knapsack: the nodes in descending min_weight order that hold
+// labels, each against the next two of them, skipping a pair of
+// single-parent nodes whose parent is the same and whose arcs differ.  Set
+// packing: every node that holds labels against its partner, when the
+// partner holds labels.  Nodes are taken through an order (null: index
+// order); the ones holding labels are compacted first (k_sd_flags, a prefix
+// of the flags, k_sd_active), then counted (k_sd_count), the counts prefixed
+// and the targets written (k_sd_targets).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_sd_flags(int64_t n, const int32_t *__restrict__ order,
+                                                     const int64_t *__restrict__ off,
+                                                     int32_t *__restrict__ flag) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) {
+        const int64_t v = order != nullptr ? order[i] : i;
+        flag[i] = off[v + 1] > off[v] ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sd_active(int64_t n, const int32_t *__restrict__ order,
+                                                      const int32_t *__restrict__ flag,
+                                                      const int64_t *__restrict__ apos,
+                                                      int32_t *__restrict__ active) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n && flag[i] != 0) {
+        active[apos[i]] = order != nullptr ? order[i] : static_cast<int32_t>(i);
+    }
+}
+
+// Per active index a (a < n; active count apos_end[0]): the comparison nodes
+// (cmp[2a], cmp[2a + 1], -1 for none) and the number of 256-label targets.
+__global__ __launch_bounds__(kBlock) void k_sd_count(
+    int64_t n, const int64_t *__restrict__ n_active, const int32_t *__restrict__ active,
+    const int64_t *__restrict__ off, const int32_t *__restrict__ single_parent,
+    const int32_t *__restrict__ single_arc, const int32_t *__restrict__ partner,
+    int32_t *__restrict__ cmp, int32_t *__restrict__ cnt) {
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (a >= n) {
+        return;
+    }
+    const int64_t na = *n_active;
+    if (a >= na) {
+        cnt[a] = 0;
+        return;
+    }
+    const int32_t n1 = active[a];
+    int32_t c[2] = {-1, -1};
+    int k = 0;
+    if (partner != nullptr) {
+        const int32_t n2 = partner[n1];
+        if (n2 >= 0 && off[n2 + 1] > off[n2]) {
+            c[k++] = n2;
+        }
+    } else {
+        const int64_t end = a + 3 < na ? a + 3 : na;
+        for (int64_t j = a + 1; j < end; ++j) {
+            const int32_t n2 = active[j];
+            const int32_t p1 = single_parent[n1], p2 = single_parent[n2];
+            if (p1 >= 0 && p2 >= 0 && p1 == p2 && single_arc[n1] != single_arc[n2]) {
+                continue;
+            }
+            c[k++] = n2;
+        }
+    }
+    const int64_t size = off[n1 + 1] - off[n1];
+    cmp[2 * a] = c[0];
+    cmp[2 * a + 1] = c[1];
+    cnt[a] = k > 0 ? static_cast<int32_t>((size + kBlock - 1) / kBlock) : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sd_targets(int64_t n, const int32_t *__restrict__ cnt,
+                                                       const int64_t *__restrict__ tpos,
+                                                       const int32_t *__restrict__ active,
+                                                       const int32_t *__restrict__ cmp,
+                                                       int32_t *__restrict__ targets) {
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (a >= n) {
+        return;
+    }
+    const int32_t k = cnt[a];
+    for (int32_t b = 0; b < k; ++b) {
+        int32_t *t = targets + 4 * (tpos[a] + b);
+        t[0] = active[a];
+        t[1] = cmp[2 * a];
+        t[2] = cmp[2 * a + 1];
+        t[3] = b * kBlock;
+    }
+}
+
+// *acc += *old_end - *new_end (labels a compaction removed).
+__global__ void k_sd_dropped(const int64_t *__restrict__ old_end,
+                             const int64_t *__restrict__ new_end, int64_t *__restrict__ acc) {
+    if (threadIdx.x == 0) {
+        *acc += *old_end - *new_end;
+    }
+}
+
+// *ssd += sum over nodes of (|F(v)| - mean)^2, from node offsets.
+__global__ __launch_bounds__(kBlock) void k_size_ssd(int64_t n_nodes,
+                                                     const int64_t *__restrict__ off,
+                                                     double *__restrict__ ssd) {
+    __shared__ double flds[kWaves];
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const double mean = static_cast<double>(off[n_nodes] - off[0]) / static_cast<double>(n_nodes);
+    double dv = 0.0;
+    if (v < n_nodes) {
+        dv = static_cast<double>(off[v + 1] - off[v]) - mean;
+    }
+    block_add_f64(dv * dv, flds, ssd);
 }
 
 } // namespace pmdd_k
