@@ -930,9 +930,6 @@ constexpr int64_t kRefillBlock = 8192;
 constexpr int64_t kRefillWaves = 8192;
 // Blocks of a batch frontier when it removes the running frontier's points.
 constexpr int64_t kEvictBlock = 2048;
-// Workgroups of a device-planned state dominance launch (each takes targets
-// in turn; about eight per compute unit are resident at once).
-constexpr int64_t kSdGrid = 4096;
 // Largest coupling batch (products) once the first batches have built R.
 constexpr int64_t kJoinBatchCap = int64_t(1) << 25;
 // The same for requests of a server (adaptive batches): their admission
@@ -2213,7 +2210,6 @@ struct Run {
         const auto t_dom = late_pair(&st->kernel_dominance_s);
         const auto t_sdk = late_pair(&st->kernel_state_dominance_s);
         hip_check(hipEventRecord(t_dom.first, s), "hipEventRecord");
-        hip_check(hipEventRecord(t_sdk.first, s), "hipEventRecord");
         int32_t *flag = ctx->sd_flag.as<int32_t>();
         int64_t *pos = ctx->sd_pos.as<int64_t>();
         int64_t *part = ctx->scan_part.as<int64_t>();
@@ -2249,10 +2245,17 @@ struct Run {
         hipLaunchKernelGGL(k_fill_u8, dim3(static_cast<unsigned>(ceil_div(std::max<int64_t>(cap, 1), 256))),
                            dim3(256), 0, s, ctx->keep_a.as<uint8_t>(), std::max<int64_t>(cap, 1),
                            uint8_t(1));
-        K.state_dom(dim3(static_cast<unsigned>(std::min<int64_t>(max_targets, kSdGrid))), s,
+        // one workgroup per target, the grid sized by the bound on the
+        // targets (the ones past the device count exit at once): hardware
+        // dispatch of one target per workgroup measured faster than a
+        // fixed grid taking targets from a queue (the rule: 42 ms against
+        // 55-65 ms per bench step)
+        hip_check(hipEventRecord(t_sdk.first, s), "hipEventRecord");
+        K.state_dom(dim3(static_cast<unsigned>(max_targets)), s,
                     ctx->which.as<int32_t>(), pos + n, f.off.as<int64_t>(), f.pts.as<int32_t>(),
                     f.cap, ctx->keep_a.as<uint8_t>(), t_sd);
         hip_check(hipGetLastError(), "k_state_dominance");
+        hip_check(hipEventRecord(t_sdk.second, s), "hipEventRecord");
         // 4. kept labels per node, the new offsets, the compaction
         hipLaunchKernelGGL(k_count_kept, dim3(static_cast<unsigned>(n)), dim3(kBlock), 0, s,
                            f.off.as<int64_t>(), ctx->keep_a.as<uint8_t>(),
@@ -2272,7 +2275,6 @@ struct Run {
                                &step_stat(step)->ssd_size);
             hip_check(hipGetLastError(), "k_size_ssd");
         }
-        hip_check(hipEventRecord(t_sdk.second, s), "hipEventRecord");
         hip_check(hipEventRecord(t_dom.second, s), "hipEventRecord");
         st->state_dominance_launches += 1;
         // the compacted labels become the layer; the spare keeps the old

@@ -909,8 +909,8 @@ __global__ __launch_bounds__(kBlock) void k_compact_chunks(
 // when some label of one of its (up to two) comparison nodes is >= it in every
 // objective.  Reads the unfiltered layer; writes keep flags per point.
 // ---------------------------------------------------------------------------
-// Targets: one per workgroup (count null), or *count of them taken by a
-// fixed grid, workgroup b taking targets b, b + grid, ...
+// One target per workgroup: every workgroup's (count null), or the first
+// *count (the grid may be larger; the rest exit at once).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_state_dominance(
     const int32_t *__restrict__ targets, // [n][4]: node, cmpA, cmpB (-1 = none), first label
@@ -920,8 +920,10 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
     constexpr int DP = Pad<D>::DP;
     __shared__ __align__(16) int32_t T[kBlock * DP];
     unsigned long long tests = 0;
-    const int64_t nt = count != nullptr ? *count : static_cast<int64_t>(gridDim.x);
-    for (int64_t ti = blockIdx.x; ti < nt; ti += gridDim.x) {
+    const int64_t ti = blockIdx.x;
+    if (count != nullptr && ti >= *count) {
+        return;
+    }
     const int32_t *tg = targets + 4 * ti;
     const int node = tg[0];
     const int64_t b0 = off[node];
@@ -967,8 +969,6 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
     }
     if (valid && dead) {
         keep[b0 + i] = 0;
-    }
-    __syncthreads(); // every read of T is done before the next target's tiles
     }
     if (pair_tests != nullptr && tests != 0) {
         atomicAdd(pair_tests, tests);
