@@ -46,6 +46,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace pmdd_k {
 
@@ -310,11 +311,16 @@ __device__ __forceinline__ bool scan_until_dead(const int32_t *S, int n, int lim
 // shared by k_filter_chunks (one workgroup per listed chunk) and
 // k_filter_queue (persistent workgroups taking chunks from a queue).
 // ---------------------------------------------------------------------------
+// Nodes of at most this many arcs are put in key order by merging their
+// parent runs (below); others by the bitonic network.
+constexpr int kMergeRuns = 16;
+
 template <int D, int CH, int NT> struct FilterLds {
     int32_t cs[D][CH];
     uint64_t sk[CH];
     alignas(16) int32_t S[CH * Pad<D>::DP];
     int wave_tot[NT / kWave];
+    int rs[kMergeRuns + 1]; // run starts of a merged node
 };
 
 // Filters chunk ci; adds the pair tests it ran to `tests`.  All threads of
@@ -357,8 +363,59 @@ __device__ __forceinline__ void filter_chunk(int ci, const ChunkDesc *__restrict
     }
     __syncthreads();
 
+    // Key order.  A node's candidates are its arcs' parent runs, each already
+    // in descending key order (a parent frontier is, and an arc adds the same
+    // weight to all of its points), so within a run the sort words are
+    // strictly descending (equal keys: slots ascending).  With few runs, a
+    // candidate's place is its index in its own run plus, for every other
+    // run, the number of that run's words above its own (a binary search):
+    // the order the bitonic network gives, at a fraction of its passes.
+    bool merged = false;
+    if constexpr (std::is_same<Src, ExpandSrc>::value && !PRESORTED) {
+        // an ExpandSrc chunk is a whole node (c.start is 0, c.len its count)
+        const int64_t e0 = src.arc_off[c.group];
+        const int nr = static_cast<int>(src.arc_off[c.group + 1] - e0);
+        if (nr <= kMergeRuns) {
+            merged = true;
+            auto &rs = L.rs;
+            if (tid <= nr) {
+                rs[tid] = tid < nr ? static_cast<int>(src.run_off[e0 + tid]) : n;
+            }
+            __syncthreads();
+            uint64_t *out = reinterpret_cast<uint64_t *>(&S[0]); // free until the skyline
+            for (int t = tid; t < n; t += NT) {
+                const uint64_t w = sk[t];
+                int a = 0;
+                while (a + 1 < nr && rs[a + 1] <= t) {
+                    ++a; // last run starting at or before t (runs may be empty)
+                }
+                int rank = t - rs[a];
+                for (int b = 0; b < nr; ++b) {
+                    if (b == a) {
+                        continue;
+                    }
+                    int lo = rs[b], hi = rs[b + 1];
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (sk[mid] > w) {
+                            lo = mid + 1;
+                        } else {
+                            hi = mid;
+                        }
+                    }
+                    rank += lo - rs[b];
+                }
+                out[rank] = w;
+            }
+            __syncthreads();
+            for (int t = tid; t < n; t += NT) {
+                sk[t] = out[t];
+            }
+            __syncthreads();
+        }
+    }
     // bitonic sort, descending (skipped when the source is in key order)
-    for (int k = 2; !PRESORTED && k <= npad; k <<= 1) {
+    for (int k = 2; !PRESORTED && !merged && k <= npad; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int t = tid; t < (npad >> 1); t += NT) {
                 const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
