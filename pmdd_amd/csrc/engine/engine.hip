@@ -1512,24 +1512,33 @@ struct Run {
     } xh;
 
     // Timings read after a later synchronisation, so the host need not wait
-    // for the work they bracket: pair i of ctx->ev_late adds to late_acc[i].
-    std::vector<double *> late_acc;
-    std::pair<hipEvent_t, hipEvent_t> late_pair(double *acc) {
-        const size_t i = late_acc.size();
-        while (ctx->ev_late.size() < 2 * (i + 1)) {
+    // for the work they bracket: mark() records event i of ctx->ev_late on
+    // the stream, span(a, b, acc) adds the time between marks a and b to
+    // *acc when flush_late runs.  Spans of one layer share its marks.
+    size_t n_marks = 0;
+    struct Span {
+        size_t a, b;
+        double *acc;
+    };
+    std::vector<Span> spans;
+    size_t mark() {
+        const size_t i = n_marks++;
+        while (ctx->ev_late.size() <= i) {
             hipEvent_t e = nullptr;
             hip_check(hipEventCreate(&e), "hipEventCreate");
             ctx->ev_late.push_back(e);
         }
-        late_acc.push_back(acc);
-        return {ctx->ev_late[2 * i], ctx->ev_late[2 * i + 1]};
+        hip_check(hipEventRecord(ctx->ev_late[i], s), "hipEventRecord");
+        return i;
     }
-    // Only once the stream has passed every recorded pair.
+    void span(size_t a, size_t b, double *acc) { spans.push_back({a, b, acc}); }
+    // Only once the stream has passed every mark.
     void flush_late() {
-        for (size_t i = 0; i < late_acc.size(); ++i) {
-            add_time(ctx->ev_late[2 * i], ctx->ev_late[2 * i + 1], late_acc[i]);
+        for (const Span &x : spans) {
+            add_time(ctx->ev_late[x.a], ctx->ev_late[x.b], x.acc);
         }
-        late_acc.clear();
+        spans.clear();
+        n_marks = 0;
     }
 
     float elapsed_ms() {
@@ -2009,12 +2018,11 @@ struct Run {
             }
             grow_points(next, d, at, next_total, s);
             // no wait: the next pass or layer is planned while this runs
-            const auto tc = late_pair(&st->kernel_expand_td_s);
-            hip_check(hipEventRecord(tc.first, s), "hipEventRecord");
+            const size_t tc = mark();
             run_compact(cb, static_cast<int64_t>(chunks.size()), chunk_out,
                         ctx->surv_pts.as<int32_t>(), pass_c, ctx->alive.as<uint8_t>(),
                         next.pts.as<int32_t>(), next.cap);
-            hip_check(hipEventRecord(tc.second, s), "hipEventRecord");
+            span(tc, mark(), &st->kernel_expand_td_s);
             next_total = at;
             sample_memory();
         }
@@ -2207,9 +2215,7 @@ struct Run {
         grow_points(sd_spare, d, std::max<int64_t>(cap, 1), 0, s);
         sample_memory();
 
-        const auto t_dom = late_pair(&st->kernel_dominance_s);
-        const auto t_sdk = late_pair(&st->kernel_state_dominance_s);
-        hip_check(hipEventRecord(t_dom.first, s), "hipEventRecord");
+        const size_t s0 = mark();
         int32_t *flag = ctx->sd_flag.as<int32_t>();
         int64_t *pos = ctx->sd_pos.as<int64_t>();
         int64_t *part = ctx->scan_part.as<int64_t>();
@@ -2250,12 +2256,12 @@ struct Run {
         // dispatch of one target per workgroup measured faster than a
         // fixed grid taking targets from a queue (the rule: 42 ms against
         // 55-65 ms per bench step)
-        hip_check(hipEventRecord(t_sdk.first, s), "hipEventRecord");
+        const size_t s1 = mark();
         K.state_dom(dim3(static_cast<unsigned>(max_targets)), s,
                     ctx->which.as<int32_t>(), pos + n, f.off.as<int64_t>(), f.pts.as<int32_t>(),
                     f.cap, ctx->keep_a.as<uint8_t>(), t_sd);
         hip_check(hipGetLastError(), "k_state_dominance");
-        hip_check(hipEventRecord(t_sdk.second, s), "hipEventRecord");
+        span(s1, mark(), &st->kernel_state_dominance_s);
         // 4. kept labels per node, the new offsets, the compaction
         hipLaunchKernelGGL(k_count_kept, dim3(static_cast<unsigned>(n)), dim3(kBlock), 0, s,
                            f.off.as<int64_t>(), ctx->keep_a.as<uint8_t>(),
@@ -2275,7 +2281,7 @@ struct Run {
                                &step_stat(step)->ssd_size);
             hip_check(hipGetLastError(), "k_size_ssd");
         }
-        hip_check(hipEventRecord(t_dom.second, s), "hipEventRecord");
+        span(s0, mark(), &st->kernel_dominance_s);
         st->state_dominance_launches += 1;
         // the compacted labels become the layer; the spare keeps the old
         // buffers for the next layer
@@ -2429,8 +2435,7 @@ struct Run {
     // header goes to the readback block of sweep direction `side`).
     void plan_layer(int side, const DevFrontier &prev, const LayerCSR &csr, int64_t n) {
         const int64_t nb = ceil_div(n, kBlock);
-        const auto tp = late_pair(&st->kernel_expand_td_s);
-        hip_check(hipEventRecord(tp.first, s), "hipEventRecord");
+        const size_t p0 = mark();
         if (nb > 0) {
             hipLaunchKernelGGL(pmdd_k::k_plan_count, dim3(static_cast<unsigned>(nb)), dim3(kBlock),
                                0, s, n, csr.d_off, csr.d_src, prev.off.as<int64_t>(),
@@ -2442,7 +2447,7 @@ struct Run {
         hipLaunchKernelGGL(pmdd_k::k_plan_scan, dim3(1), dim3(kBlock), 0, s, nb,
                            ctx->plan_part[side].as<int64_t>(), &rb()->hdr[side]);
         hip_check(hipGetLastError(), "k_plan_scan");
-        hip_check(hipEventRecord(tp.second, s), "hipEventRecord");
+        span(p0, mark(), &st->kernel_expand_td_s);
     }
 
     // Reads the headers and scores back (waits for the stream).
@@ -2575,8 +2580,7 @@ struct Run {
                               "hipMalloc(scan prefixes)");
         sample_memory();
 
-        const auto t_exp = late_pair(&st->kernel_expand_td_s);
-        hip_check(hipEventRecord(t_exp.first, s), "hipEventRecord");
+        const size_t m0 = mark();
         PlanTables t;
         t.chunks = cb.chunks.as<ChunkDesc>();
         t.node_chunk = ctx->node_chunk.as<int32_t>();
@@ -2632,11 +2636,8 @@ struct Run {
                     cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>());
             hip_check(hipGetLastError(), "k_stage_chunks");
         }
-        hip_check(hipEventRecord(t_exp.second, s), "hipEventRecord");
-        const auto t_fil = late_pair(&st->kernel_filter_s);
-        const auto t_dom = late_pair(&st->kernel_dominance_s);
-        hip_check(hipEventRecord(t_fil.first, s), "hipEventRecord");
-        hip_check(hipEventRecord(t_dom.first, s), "hipEventRecord");
+        const size_t m1 = mark();
+        span(m0, m1, &st->kernel_expand_td_s);
         const pmdd_tuning &T = ctx->tuning[d];
         const int64_t counts[3] = {n_tiny, n_mid, n_small};
         const DevBuf *lists[3] = {&ctx->which_s, &ctx->which_m, &ctx->which_a};
@@ -2652,7 +2653,8 @@ struct Run {
             hip_check(hipGetLastError(), "k_filter_chunks");
             ++filters;
         }
-        hip_check(hipEventRecord(t_fil.second, s), "hipEventRecord");
+        const size_t m2 = mark();
+        span(m1, m2, &st->kernel_filter_s);
         if (n_bign > 0) {
             // every wave's chunks in one list; a launch per wave takes its own
             const int waves = static_cast<int>(ceil_div(h.max_chunks, kCrossWave));
@@ -2667,13 +2669,10 @@ struct Run {
                 int64_t *fin = fc + (w % 2) * static_cast<int64_t>(nc1);
                 int64_t *fout = fc + ((w + 1) % 2) * static_cast<int64_t>(nc1);
                 const int only = waves > 1 ? w : -1;
-                const auto t_x = late_pair(&st->kernel_cross_s);
-                hip_check(hipEventRecord(t_x.first, s), "hipEventRecord");
                 K.cross(dim3(static_cast<unsigned>(n_bigch * G)), s, ctx->which_b.as<int32_t>(),
                         cb.chunks.as<ChunkDesc>(), pts, C, alive, cb.nsurv.as<int32_t>(), sorted,
                         bigpos, fin, kCrossWave, G, only, t_cross);
                 hip_check(hipGetLastError(), "k_cross_chunks");
-                hip_check(hipEventRecord(t_x.second, s), "hipEventRecord");
                 K.append_front(dim3(static_cast<unsigned>(n_bigch)), s, ctx->which_b.as<int32_t>(),
                                cb.chunks.as<ChunkDesc>(), pts, C, alive, cb.nsurv.as<int32_t>(),
                                cb.nalive.as<int32_t>(), sorted, bigpos, fin, fout, kCrossWave,
@@ -2682,13 +2681,15 @@ struct Run {
             }
             st->cross_launches += waves;
         }
-        hip_check(hipEventRecord(t_dom.second, s), "hipEventRecord");
+        const size_t m3 = mark();
+        span(m1, m3, &st->kernel_dominance_s);
+        if (n_bign > 0) {
+            span(m2, m3, &st->kernel_cross_s); // the cross waves with their appends
+        }
         st->filter_launches += filters;
         st->filter_candidates += C;
 
         // placement: chunk output slots, node offsets, then the survivors
-        const auto t_pl = late_pair(&st->kernel_expand_td_s);
-        hip_check(hipEventRecord(t_pl.first, s), "hipEventRecord");
         int64_t *slot = cb.out.as<int64_t>();
         if (nbc > 0) {
             hipLaunchKernelGGL(k_block_sums, dim3(static_cast<unsigned>(nbc)), dim3(kBlock), 0, s,
@@ -2716,7 +2717,7 @@ struct Run {
                       next.cap);
             hip_check(hipGetLastError(), "k_compact_chunks");
         }
-        hip_check(hipEventRecord(t_pl.second, s), "hipEventRecord");
+        span(m3, mark(), &st->kernel_expand_td_s);
     }
 
     // Expansion records into the statistics (the stream is idle).
