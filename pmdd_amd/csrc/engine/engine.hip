@@ -2223,9 +2223,8 @@ struct Run {
         // exclusive prefix of cnt[0 .. n) into out[0 .. n]
         auto prefix = [&](auto *cnt, int64_t *out) {
             using C = std::remove_const_t<std::remove_pointer_t<decltype(cnt)>>;
-            hipLaunchKernelGGL(k_block_sums<C>, g, dim3(kBlock), 0, s, n, cnt, part);
-            hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kBlock), 0, s, nb, part, n, int64_t(0),
-                               out, static_cast<int64_t *>(nullptr));
+            hipLaunchKernelGGL(k_block_sums<C>, g, dim3(kBlock), 0, s, n, cnt, part, arrivals(),
+                               int64_t(0), out, static_cast<int64_t *>(nullptr));
             hipLaunchKernelGGL(k_scan_write<C>, g, dim3(kBlock), 0, s, n, cnt, part, int64_t(0),
                                out);
             hip_check(hipGetLastError(), "prefix scan");
@@ -2390,6 +2389,10 @@ struct Run {
         return sh;
     }
     pmdd_k::PlanReadback *rb() const { return ctx->plan_rb.as<pmdd_k::PlanReadback>(); }
+    // Arrival counter of the prefix kernels whose last workgroup finishes
+    // the prefix (zero between launches; launches on the stream follow each
+    // other).  Bytes 8..11 of the run counters.
+    unsigned *arrivals() const { return reinterpret_cast<unsigned *>(ctx->run_ctr.as<char>() + 8); }
     pmdd_k::StepStat *step_stat(int i) const {
         return ctx->step_stats.as<pmdd_k::StepStat>() + i;
     }
@@ -2436,17 +2439,12 @@ struct Run {
     void plan_layer(int side, const DevFrontier &prev, const LayerCSR &csr, int64_t n) {
         const int64_t nb = ceil_div(n, kBlock);
         const size_t p0 = mark();
-        if (nb > 0) {
-            hipLaunchKernelGGL(pmdd_k::k_plan_count, dim3(static_cast<unsigned>(nb)), dim3(kBlock),
-                               0, s, n, csr.d_off, csr.d_src, prev.off.as<int64_t>(),
-                               ctx->plan_run_off[side].as<int64_t>(),
-                               ctx->plan_gamma[side].as<int64_t>(),
-                               ctx->plan_part[side].as<int64_t>(), plan_shape());
-            hip_check(hipGetLastError(), "k_plan_count");
-        }
-        hipLaunchKernelGGL(pmdd_k::k_plan_scan, dim3(1), dim3(kBlock), 0, s, nb,
-                           ctx->plan_part[side].as<int64_t>(), &rb()->hdr[side]);
-        hip_check(hipGetLastError(), "k_plan_scan");
+        hipLaunchKernelGGL(pmdd_k::k_plan_count, dim3(static_cast<unsigned>(std::max<int64_t>(nb, 1))),
+                           dim3(kBlock), 0, s, n, csr.d_off, csr.d_src, prev.off.as<int64_t>(),
+                           ctx->plan_run_off[side].as<int64_t>(),
+                           ctx->plan_gamma[side].as<int64_t>(), ctx->plan_part[side].as<int64_t>(),
+                           plan_shape(), arrivals(), &rb()->hdr[side]);
+        hip_check(hipGetLastError(), "k_plan_count");
         span(p0, mark(), &st->kernel_expand_td_s);
     }
 
@@ -2691,15 +2689,12 @@ struct Run {
 
         // placement: chunk output slots, node offsets, then the survivors
         int64_t *slot = cb.out.as<int64_t>();
-        if (nbc > 0) {
-            hipLaunchKernelGGL(k_block_sums, dim3(static_cast<unsigned>(nbc)), dim3(kBlock), 0, s,
-                               nch, cb.nalive.as<int32_t>(), ctx->scan_part.as<int64_t>());
-            hip_check(hipGetLastError(), "k_block_sums");
-        }
-        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kBlock), 0, s, nbc,
-                           ctx->scan_part.as<int64_t>(), nch, int64_t(0), slot,
+        hipLaunchKernelGGL(k_block_sums<int32_t>,
+                           dim3(static_cast<unsigned>(std::max<int64_t>(nbc, 1))), dim3(kBlock), 0,
+                           s, nch, static_cast<const int32_t *>(cb.nalive.as<int32_t>()),
+                           ctx->scan_part.as<int64_t>(), arrivals(), int64_t(0), slot,
                            &step_stat(step)->surv);
-        hip_check(hipGetLastError(), "k_scan_blocks");
+        hip_check(hipGetLastError(), "k_block_sums");
         if (nbc > 0) {
             hipLaunchKernelGGL(k_scan_write, dim3(static_cast<unsigned>(nbc)), dim3(kBlock), 0, s,
                                nch, cb.nalive.as<int32_t>(), ctx->scan_part.as<int64_t>(),

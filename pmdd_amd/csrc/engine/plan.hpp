@@ -11,16 +11,16 @@
 //   k_plan_count  one thread per node: the node-local offset of each arc's
 //                 candidate run (run_off), the node's candidate count (gamma),
 //                 and per workgroup the sums of the node's plan values and the
-//                 largest big node (chunks, arcs)
-//   k_plan_scan   one workgroup: exclusive prefix of the workgroup sums, the
-//                 layer totals into the plan header (the one thing the host
-//                 reads back: it sizes the buffers and picks the launches)
+//                 largest big node (chunks, arcs); the last workgroup to finish
+//                 turns the sums into exclusive prefixes and writes the layer
+//                 totals into the plan header (the one thing the host reads
+//                 back: it sizes the buffers and picks the launches)
 //   k_plan_write  one thread per node: its prefix of every plan value, then
 //                 its chunk descriptors, tier-list entries and big-node tables
 //
 // After the filters, the survivors are placed without the host as well:
-//   k_block_sums / k_scan_blocks / k_scan_write  exclusive prefix of the
-//                 per-chunk survivor counts (the chunks' output slots)
+//   k_block_sums / k_scan_write  exclusive prefix of the per-chunk
+//                 survivor counts (the chunks' output slots)
 //   k_node_offsets  the new frontier's node offsets from the chunk slots
 //
 // The tables are the ones the host planner builds (the same chunk order,
@@ -186,6 +186,83 @@ __device__ __forceinline__ void block_add_f64(double v, double *lds, double *acc
     }
 }
 
+// Called by every thread of every workgroup once the workgroup has written
+// its partial results: true in the workgroup that arrives last, whose reads
+// after the call see every workgroup's partials (the arrival counter is an
+// agent-scope atomic with acquire-release order: each workgroup's writes are
+// released to the device before its arrival, the last one acquires them).
+// The last workgroup puts the counter back to zero for the next launch.  No
+// workgroup waits for another.
+__device__ __forceinline__ bool last_to_arrive(unsigned *counter) {
+    __shared__ bool last;
+    __syncthreads(); // the workgroup's partials are written before its arrival
+    if (threadIdx.x == 0) {
+        const unsigned prev =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = prev == gridDim.x - 1;
+        if (last) {
+            __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+    return last;
+}
+
+// Exclusive prefix, in place, of nb rows of kPlanPart values (the first
+// kPlanVals summed, the last two maxima) into the plan header's totals and
+// maxima; one workgroup.
+__device__ __forceinline__ void plan_scan_rows(int64_t nb, int64_t *part, PlanHeader *hdr) {
+    __shared__ int64_t lds[kWaves];
+    __shared__ int64_t mx[2][kWaves];
+    int64_t carry[kPlanVals];
+#pragma unroll
+    for (int k = 0; k < kPlanVals; ++k) {
+        carry[k] = 0;
+    }
+    int64_t mx_chunks = 0, mx_runs = 0;
+    for (int64_t base = 0; base < nb; base += kBlock) {
+        const int64_t b = base + threadIdx.x;
+        int64_t *row = part + b * kPlanPart;
+        const bool ok = b < nb;
+#pragma unroll
+        for (int k = 0; k < kPlanVals; ++k) {
+            const int64_t x = ok ? row[k] : 0;
+            int64_t tot = 0;
+            const int64_t ex = block_exscan64(x, lds, &tot);
+            if (ok) {
+                row[k] = carry[k] + ex;
+            }
+            carry[k] += tot;
+        }
+        if (ok) {
+            mx_chunks = row[kPlanVals] > mx_chunks ? row[kPlanVals] : mx_chunks;
+            mx_runs = row[kPlanVals + 1] > mx_runs ? row[kPlanVals + 1] : mx_runs;
+        }
+    }
+    mx_chunks = wave_max(mx_chunks);
+    mx_runs = wave_max(mx_runs);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        mx[0][wave] = mx_chunks;
+        mx[1][wave] = mx_runs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kPlanVals; ++k) {
+            hdr->tot[k] = carry[k];
+        }
+        int64_t a = 0, r = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            a = mx[0][w] > a ? mx[0][w] : a;
+            r = mx[1][w] > r ? mx[1][w] : r;
+        }
+        hdr->max_chunks = a;
+        hdr->max_runs = r;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_plan_count: grid ceil(N / kBlock).  run_off[e] for every arc of the
 // layer, gamma[v] for every node, part[b * kPlanPart ..] per workgroup.
@@ -193,7 +270,8 @@ __device__ __forceinline__ void block_add_f64(double v, double *lds, double *acc
 __global__ __launch_bounds__(kBlock) void k_plan_count(
     int64_t n_nodes, const int64_t *__restrict__ arc_off, const int32_t *__restrict__ arc_src,
     const int64_t *__restrict__ prev_off, int64_t *__restrict__ run_off,
-    int64_t *__restrict__ gamma, int64_t *__restrict__ part, PlanShape sh) {
+    int64_t *__restrict__ gamma, int64_t *__restrict__ part, PlanShape sh,
+    unsigned *__restrict__ counter, PlanHeader *__restrict__ hdr) {
     __shared__ int64_t red[kWaves][kPlanPart];
     const int64_t v = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     int64_t pv[kPlanVals];
@@ -241,62 +319,9 @@ __global__ __launch_bounds__(kBlock) void k_plan_count(
         }
         part[static_cast<int64_t>(blockIdx.x) * kPlanPart + k] = r;
     }
-}
-
-// ---------------------------------------------------------------------------
-// k_plan_scan: one workgroup.  The nb workgroup sums become exclusive
-// prefixes in place; the totals and maxima go to *hdr.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_plan_scan(int64_t nb, int64_t *__restrict__ part,
-                                                      PlanHeader *__restrict__ hdr) {
-    __shared__ int64_t lds[kWaves];
-    int64_t carry[kPlanVals];
-#pragma unroll
-    for (int k = 0; k < kPlanVals; ++k) {
-        carry[k] = 0;
-    }
-    int64_t mx_chunks = 0, mx_runs = 0;
-    for (int64_t base = 0; base < nb; base += kBlock) {
-        const int64_t b = base + threadIdx.x;
-        int64_t *row = part + b * kPlanPart;
-        const bool ok = b < nb;
-#pragma unroll
-        for (int k = 0; k < kPlanVals; ++k) {
-            const int64_t x = ok ? row[k] : 0;
-            int64_t tot = 0;
-            const int64_t ex = block_exscan64(x, lds, &tot);
-            if (ok) {
-                row[k] = carry[k] + ex;
-            }
-            carry[k] += tot;
-        }
-        if (ok) {
-            mx_chunks = row[kPlanVals] > mx_chunks ? row[kPlanVals] : mx_chunks;
-            mx_runs = row[kPlanVals + 1] > mx_runs ? row[kPlanVals + 1] : mx_runs;
-        }
-    }
-    __shared__ int64_t mx[2][kWaves];
-    mx_chunks = wave_max(mx_chunks);
-    mx_runs = wave_max(mx_runs);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    if (lane == 0) {
-        mx[0][wave] = mx_chunks;
-        mx[1][wave] = mx_runs;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < kPlanVals; ++k) {
-            hdr->tot[k] = carry[k];
-        }
-        int64_t a = 0, r = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            a = mx[0][w] > a ? mx[0][w] : a;
-            r = mx[1][w] > r ? mx[1][w] : r;
-        }
-        hdr->max_chunks = a;
-        hdr->max_runs = r;
+    // the last workgroup turns the sums into prefixes and writes the header
+    if (last_to_arrive(counter)) {
+        plan_scan_rows(gridDim.x, part, hdr);
     }
 }
 
@@ -316,7 +341,7 @@ struct PlanTables {
 };
 
 // ---------------------------------------------------------------------------
-// k_plan_write: grid ceil(N / kBlock), after k_plan_scan.
+// k_plan_write: grid ceil(N / kBlock), after k_plan_count.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_plan_write(
     int64_t n_nodes, const int64_t *__restrict__ arc_off, const int64_t *__restrict__ run_off,
@@ -397,13 +422,18 @@ __global__ __launch_bounds__(kBlock) void k_plan_write(
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive prefix of n counts (int32 or int64) into int64 slots: out[i] = base + sum of
-// cnt[0 .. i), out[n] = base + sum of all.  k_block_sums and k_scan_write run
-// on ceil(n / kBlock) workgroups (none when n is 0), k_scan_blocks on one.
-// ---------------------------------------------------------------------------
+// Exclusive prefix of n counts (int32 or int64) into int64 slots: out[i] =
+// base + sum of cnt[0 .. i), out[n] = base + sum of all.  k_block_sums (on
+// ceil(n / kBlock) workgroups, at least one) writes the workgroup sums, and
+// its last workgroup to finish turns them into exclusive prefixes, writes
+// out[n] and, when total_out is set, *total_out = the sum; k_scan_write (on
+// ceil(n / kBlock) workgroups, none when n is 0) writes out[0 .. n).
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_block_sums(int64_t n, const C *__restrict__ cnt,
-                                                       int64_t *__restrict__ part) {
+                                                       int64_t *__restrict__ part,
+                                                       unsigned *__restrict__ counter,
+                                                       int64_t base, int64_t *__restrict__ out,
+                                                       int64_t *__restrict__ total_out) {
     __shared__ int64_t lds[kWaves];
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     int64_t tot = 0;
@@ -411,24 +441,19 @@ __global__ __launch_bounds__(kBlock) void k_block_sums(int64_t n, const C *__res
     if (threadIdx.x == 0) {
         part[blockIdx.x] = tot;
     }
-}
-
-// The nb workgroup sums become exclusive prefixes; out[n] = base + total and,
-// when total_out is set, *total_out = total.
-__global__ __launch_bounds__(kBlock) void k_scan_blocks(int64_t nb, int64_t *__restrict__ part,
-                                                        int64_t n, int64_t base,
-                                                        int64_t *__restrict__ out,
-                                                        int64_t *__restrict__ total_out) {
-    __shared__ int64_t lds[kWaves];
+    if (!last_to_arrive(counter)) {
+        return;
+    }
+    const int64_t nb = gridDim.x;
     int64_t carry = 0;
     for (int64_t b0 = 0; b0 < nb; b0 += kBlock) {
         const int64_t b = b0 + threadIdx.x;
-        int64_t tot = 0;
-        const int64_t ex = block_exscan64(b < nb ? part[b] : 0, lds, &tot);
+        int64_t t = 0;
+        const int64_t ex = block_exscan64(b < nb ? part[b] : 0, lds, &t);
         if (b < nb) {
             part[b] = carry + ex;
         }
-        carry += tot;
+        carry += t;
     }
     if (threadIdx.x == 0) {
         out[n] = base + carry;
