@@ -370,6 +370,7 @@ struct pmdd_ctx {
     DevBuf plan_run_off[2], plan_gamma[2], plan_part[2];
     DevBuf plan_rb, node_chunk, scan_part, step_stats, run_ctr;
     DevBuf sd_flag, sd_pos, sd_active, sd_cmp; // device-planned state dominance
+    DevBuf batch_ctr;                          // a coupling batch's device counts
     pmdd_k::PlanReadback *h_rb = nullptr;
     std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
     std::vector<hipEvent_t> ev_late; // start/end pairs read after a later wait
@@ -410,7 +411,8 @@ struct pmdd_ctx {
                           &join_y, &join_ctr, &front_count, &merge_a, &merge_b, &qtier, &qhead,
                           &plan_run_off[0], &plan_run_off[1], &plan_gamma[0], &plan_gamma[1],
                           &plan_part[0], &plan_part[1], &plan_rb, &node_chunk, &scan_part,
-                          &step_stats, &run_ctr, &sd_flag, &sd_pos, &sd_active, &sd_cmp}) {
+                          &step_stats, &run_ctr, &sd_flag, &sd_pos, &sd_active, &sd_cmp,
+                          &batch_ctr}) {
             f(*b);
         }
         for (DevFrontier &fr : frontier_pool) {
@@ -899,7 +901,7 @@ struct Launch {
                       unsigned long long *, unsigned long long *);
     void (*refill_flags)(dim3, hipStream_t, const int32_t *, int64_t,
                          const unsigned long long *, const int32_t *, int64_t, int64_t, int64_t,
-                         uint8_t *, unsigned long long *);
+                         const int64_t *, uint8_t *, unsigned long long *);
     // persistent chunk filters of a queued pass, one per tier (nodes of at
     // most kSmallNode candidates, of at most mid_cap, of at most CH), with
     // the grid that keeps every compute unit full for each
@@ -928,8 +930,6 @@ constexpr int kFilterThreadsBig = 512;
 // no faster).
 constexpr int64_t kRefillBlock = 8192;
 constexpr int64_t kRefillWaves = 8192;
-// Blocks of a batch frontier when it removes the running frontier's points.
-constexpr int64_t kEvictBlock = 2048;
 // Largest coupling batch (products) once the first batches have built R.
 constexpr int64_t kJoinBatchCap = int64_t(1) << 25;
 // The same for requests of a server (adaptive batches): their admission
@@ -1000,13 +1000,15 @@ template <int D> struct LaunchImpl {
                           int64_t hi, int32_t *Y, int64_t ys, unsigned long long *n_out,
                           unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D, false>), g, dim3(pmdd_k::kWave), 0, s,
-                           X, xs, n_in, R, rs, lo, hi, Y, ys, n_out, nullptr, pt);
+                           X, xs, n_in, R, rs, lo, hi, static_cast<const int64_t *>(nullptr), Y,
+                           ys, n_out, nullptr, pt);
     }
     static void refill_flags(dim3 g, hipStream_t s, const int32_t *X, int64_t xs,
                              const unsigned long long *n_in, const int32_t *R, int64_t rs,
-                             int64_t lo, int64_t hi, uint8_t *flag, unsigned long long *pt) {
+                             int64_t lo, int64_t hi, const int64_t *hi_dev, uint8_t *flag,
+                             unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D, true>), g, dim3(pmdd_k::kWave), 0, s, X,
-                           xs, n_in, R, rs, lo, hi, nullptr, 0, nullptr, flag, pt);
+                           xs, n_in, R, rs, lo, hi, hi_dev, nullptr, 0, nullptr, flag, pt);
     }
     static void stage(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       pmdd_k::SortedSrc src, int32_t *out, int64_t stride, uint8_t *alive,
@@ -2393,6 +2395,24 @@ struct Run {
     // the prefix (zero between launches; launches on the stream follow each
     // other).  Bytes 8..11 of the run counters.
     unsigned *arrivals() const { return reinterpret_cast<unsigned *>(ctx->run_ctr.as<char>() + 8); }
+    // out[0 .. n] = exclusive prefix of cnt[0 .. n) (out[n] the sum), and
+    // *total = the sum when total is set; two launches, no wait.
+    template <class C> void device_prefix(const C *cnt, int64_t n, int64_t *out, int64_t *total) {
+        const int64_t nb = ceil_div(n, kBlock);
+        ctx->scan_part.ensure(static_cast<size_t>(std::max<int64_t>(nb, 1)) * 8,
+                              "hipMalloc(scan prefixes)");
+        hipLaunchKernelGGL(pmdd_k::k_block_sums<C>,
+                           dim3(static_cast<unsigned>(std::max<int64_t>(nb, 1))), dim3(kBlock), 0, s,
+                           n, cnt, ctx->scan_part.as<int64_t>(), arrivals(), int64_t(0), out,
+                           total);
+        hip_check(hipGetLastError(), "k_block_sums");
+        if (nb > 0) {
+            hipLaunchKernelGGL(pmdd_k::k_scan_write<C>, dim3(static_cast<unsigned>(nb)),
+                               dim3(kBlock), 0, s, n, cnt, ctx->scan_part.as<int64_t>(),
+                               int64_t(0), out);
+            hip_check(hipGetLastError(), "k_scan_write");
+        }
+    }
     pmdd_k::StepStat *step_stat(int i) const {
         return ctx->step_stats.as<pmdd_k::StepStat>() + i;
     }
@@ -2884,12 +2904,15 @@ struct Run {
     // Cutset coupling (couple_cutsets_cuda, couple.cu:340-498): the result is
     // the nondominated set of the union over cutset nodes of TD(v) + BU(v).
     // Nodes with both frontiers non-empty, largest |TD|+|BU| first, go in
-    // batches of at most max_prod products (a single node may exceed it).  Per
-    // batch: products filtered chunk by chunk, the survivors merged into key
-    // order, then in descending-key blocks filtered locally, against the
-    // running frontier R (kept in key order) and against earlier blocks,
-    // giving the batch frontier B; R loses the points B strictly dominates
-    // and B is merged into R.
+    // batches of at most max_prod products (a node with more is split into
+    // pieces).  Per batch: the products are checked against the running
+    // frontier R (kept in key order); the survivors are filtered chunk by
+    // chunk and merged into key order, then in descending-key blocks filtered
+    // locally and against earlier blocks, giving the batch frontier B; R
+    // loses the points B strictly dominates and B is merged into R.  The host
+    // waits twice per batch: for the products past R (they size the rest),
+    // and at the end for the batch's counts; everything between is counted
+    // and placed on the device.
     void couple(const DevFrontier &td, const DevFrontier &bu, pmdd_frontier *out) {
         const int CH = K.chunk_cap;
         const int64_t n = td.n_nodes;
@@ -2911,10 +2934,7 @@ struct Run {
         DevFrontier &R = lr.f, &B = lbt.f;
         int64_t r_count = 0;
         size_t i = 0;
-        std::vector<ChunkDesc> chunks, rchunks;
-        std::vector<int32_t> nalive, rnalive;
-        std::vector<int64_t> outpos, run_start, routpos;
-        std::vector<int32_t> grp_run;
+        std::vector<ChunkDesc> rchunks;
         ChunkBufs &cb = ctx->cb_main;
         ChunkBufs &cr = ctx->cb_aux;
         const std::vector<int32_t> none;
@@ -3005,22 +3025,13 @@ struct Run {
             src.bu_pts = bu.pts.as<int32_t>();
             src.bu_stride = bu.cap;
 
-            // 1. the batch's products, densely, in product order
+            // 1. the batch's products, densely, in product order; then the
+            // check against the running frontier R (descending key order) in
+            // blocks of growing length: most products meet a dominator among
+            // R's first points, so the later blocks see few of them
             const size_t pbytes = static_cast<size_t>(bprod) * d * 4;
             ctx->join_x.ensure(pbytes, "hipMalloc(product buffer)");
             ctx->join_y.ensure(pbytes, "hipMalloc(product buffer)");
-            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            K.join_products(dim3(static_cast<unsigned>(
-                                std::min<int64_t>(ceil_div(bprod, pmdd_k::kBlock), 16384))),
-                            s, src, bprod, ctx->join_x.as<int32_t>(), bprod);
-            hip_check(hipGetLastError(), "k_join_products");
-            hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-            // 2. against the running frontier R (descending key order), in
-            // blocks of growing length: most products meet a dominator among
-            // R's first points, so the later blocks see few of them
-            int32_t *F = ctx->join_x.as<int32_t>();
-            int64_t ytotal = bprod;
-            std::vector<unsigned long long> ctr;
             std::vector<int64_t> bounds;
             if (r_count > 0) {
                 bounds.assign(1, 0);
@@ -3033,43 +3044,52 @@ struct Run {
                     bounds.push_back(bounds.back() + kRefillBlock);
                 }
                 bounds.push_back(r_count);
-                const int nbk = static_cast<int>(bounds.size()) - 1;
-                // ctr[0]: products; ctr[1 + b]: survivors of block b
-                ctr.assign(1 + static_cast<size_t>(nbk), 0);
-                ctr[0] = static_cast<unsigned long long>(bprod);
-                h2d(ctx, ctx->join_ctr, ctr, "hipMemcpy(join counters)");
-                unsigned long long *dc = ctx->join_ctr.as<unsigned long long>();
+            }
+            const int nbk = std::max<int>(0, static_cast<int>(bounds.size()) - 1);
+            // counters: [0] products, [1 + b] survivors of R block b
+            std::vector<unsigned long long> ctr(1 + static_cast<size_t>(nbk), 0);
+            ctr[0] = static_cast<unsigned long long>(bprod);
+            h2d(ctx, ctx->join_ctr, ctr, "hipMemcpy(join counters)");
+            unsigned long long *dc = ctx->join_ctr.as<unsigned long long>();
+            const size_t m0 = mark();
+            K.join_products(dim3(static_cast<unsigned>(
+                                std::min<int64_t>(ceil_div(bprod, pmdd_k::kBlock), 16384))),
+                            s, src, bprod, ctx->join_x.as<int32_t>(), bprod);
+            hip_check(hipGetLastError(), "k_join_products");
+            const size_t m1 = mark();
+            span(m0, m1, &st->kernel_expand_td_s);
+            int32_t *F = ctx->join_x.as<int32_t>();
+            if (nbk > 0) {
                 const unsigned grid = static_cast<unsigned>(std::min<int64_t>(
                     ceil_div(bprod, pmdd_k::kWave), ctx->tuning[d].refill_waves));
                 int32_t *in = ctx->join_x.as<int32_t>();
                 int32_t *outb = ctx->join_y.as<int32_t>();
-                hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
                 for (int b = 0; b < nbk; ++b) {
                     K.refill_vs(dim3(grid), s, in, bprod, dc + b, R.pts.as<int32_t>(), R.cap,
                                 bounds[b], bounds[b + 1], outb, bprod, dc + 1 + b, t_check);
                     hip_check(hipGetLastError(), "k_refill_vs_block");
                     std::swap(in, outb);
                 }
-                hip_check(hipEventRecord(ctx->ev_f, s), "hipEventRecord");
-                d2h(ctx, ctr, ctx->join_ctr, ctr.size(), "join counters");
                 F = in;
-                ytotal = static_cast<int64_t>(ctr[nbk]);
-                add_time(ctx->ev_e, ctx->ev_f, &st->kernel_frontier_check_s);
-                add_time(ctx->ev_e, ctx->ev_f, &st->kernel_dominance_s);
+                const size_t m2 = mark();
+                span(m1, m2, &st->kernel_frontier_check_s);
+                span(m1, m2, &st->kernel_dominance_s);
                 st->frontier_check_launches += nbk;
-                for (int b = 0; b < nbk; ++b) {
-                    const int64_t n_in = static_cast<int64_t>(ctr[b]);
-                    const int64_t n_out = static_cast<int64_t>(ctr[b + 1]);
-                    st->frontier_check_bytes +=
-                        static_cast<int64_t>(d) * 4 * (n_in + n_out + bounds[b + 1] - bounds[b]);
-                }
-            } else {
-                hip_check(hipStreamSynchronize(s), "product sync");
             }
-            add_time(ctx->ev_a, ctx->ev_c, &st->kernel_expand_td_s);
+            // the first of the batch's two waits: the products past R size
+            // the batch's own filter
+            d2h(ctx, ctr, ctx->join_ctr, ctr.size(), "join counters");
+            flush_late();
+            const int64_t ytotal = static_cast<int64_t>(ctr[nbk]);
+            for (int b = 0; b < nbk; ++b) {
+                const int64_t n_in = static_cast<int64_t>(ctr[b]);
+                const int64_t n_out = static_cast<int64_t>(ctr[b + 1]);
+                st->frontier_check_bytes +=
+                    static_cast<int64_t>(d) * 4 * (n_in + n_out + bounds[b + 1] - bounds[b]);
+            }
             if (debug_join) {
-                std::fprintf(stderr, "join batch: nodes=%d products=%lld after_R=%lld",
-                             Kn, static_cast<long long>(bprod), static_cast<long long>(ytotal));
+                std::fprintf(stderr, "join batch: nodes=%d products=%lld after_R=%lld", Kn,
+                             static_cast<long long>(bprod), static_cast<long long>(ytotal));
             }
             if (ytotal == 0) {
                 if (debug_join) {
@@ -3077,214 +3097,216 @@ struct Run {
                 }
                 continue;
             }
-            // 3. the survivors: filtered chunk by chunk (presorted in LDS),
-            // then merged into key order
-            chunks.clear();
-            const int32_t cnt = static_cast<int32_t>(ceil_div(ytotal, CH));
-            for (int32_t k = 0; k < cnt; ++k) {
-                ChunkDesc c;
-                c.start = static_cast<int64_t>(k) * CH;
-                c.len = static_cast<int32_t>(std::min<int64_t>(CH, ytotal - c.start));
-                c.out_base = c.start;
-                c.group = 0;
-                c.group_first = 0;
-                c.group_count = 1;
-                chunks.push_back(c);
-            }
-            upload_table(cb, chunks);
+            // From here on every size is on the device (counted into bc:
+            // [0] products past R, [1] their local survivors, [2] the batch
+            // frontier B, [3] R's points B leaves, [4] the new R), every
+            // table is sized by the products past R, and the batch ends with
+            // its second wait.
+            const int64_t cnt = ceil_div(ytotal, CH);
+            ctx->batch_ctr.ensure(64, "hipMalloc(batch counts)");
+            int64_t *bc = ctx->batch_ctr.as<int64_t>();
+            cb.chunks.ensure(static_cast<size_t>(cnt) * sizeof(ChunkDesc), "hipMalloc(chunk table)");
+            cb.nsurv.ensure(static_cast<size_t>(cnt) * 4, "hipMalloc(chunk counts)");
+            cb.nalive.ensure(static_cast<size_t>(cnt) * 4, "hipMalloc(chunk counts)");
+            cb.out.ensure(static_cast<size_t>(cnt + 1) * 8, "hipMalloc(chunk outputs)");
             ctx->surv_pts.ensure(static_cast<size_t>(ytotal) * d * 4, "hipMalloc(survivor buffer)");
             ctx->alive.ensure(static_cast<size_t>(ytotal), "hipMalloc(alive flags)");
+            ctx->sort_a.ensure(static_cast<size_t>(ytotal) * d * 4, "hipMalloc(sort buffer)");
+            ctx->sort_b.ensure(static_cast<size_t>(ytotal) * d * 4, "hipMalloc(sort buffer)");
+            ctx->grp_run.ensure(2 * sizeof(int32_t), "hipMalloc(group table)");
+            ctx->run_start.ensure(3 * sizeof(int64_t), "hipMalloc(run table)");
+            grow_points(B, d, ytotal, 0, s);
+            const dim3 cgrid(static_cast<unsigned>(ceil_div(cnt, kBlock)));
+            int32_t *pts = ctx->surv_pts.as<int32_t>();
+            uint8_t *alive = ctx->alive.as<uint8_t>();
+            // 2. the survivors filtered chunk by chunk (presorted in LDS),
+            // written in key order per chunk and merged into one key order
+            hipLaunchKernelGGL(pmdd_k::k_count_to_i64, dim3(1), dim3(64), 0, s, dc + nbk, bc + 0);
+            hipLaunchKernelGGL(pmdd_k::k_uniform_chunks, cgrid, dim3(kBlock), 0, s, bc + 0, cnt, CH,
+                               false, cb.chunks.as<ChunkDesc>(), cb.nsurv.as<int32_t>(),
+                               cb.nalive.as<int32_t>());
+            hip_check(hipGetLastError(), "k_uniform_chunks");
+            const size_t m3 = mark();
             pmdd_k::SortedSrc fsrc{F, bprod};
-            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            const int dense = run_filter(K.filter_dense, cb, ctx->which_a, none, cnt, &fsrc,
-                                         ctx->surv_pts.as<int32_t>(), ytotal,
-                                         ctx->alive.as<uint8_t>());
-            hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-            d2h(ctx, nalive, cb.nalive, chunks.size(), "product survivor counts");
-            add_time(ctx->ev_a, ctx->ev_c, &st->kernel_filter_s);
-            add_time(ctx->ev_a, ctx->ev_c, &st->kernel_dominance_s);
-            st->filter_launches += dense;
-            st->filter_candidates += ytotal;
-            outpos.resize(chunks.size());
-            run_start.clear();
-            int64_t xtotal = 0;
-            for (size_t c = 0; c < chunks.size(); ++c) {
-                outpos[c] = xtotal;
-                run_start.push_back(xtotal);
-                xtotal += nalive[c];
+            const int dense = run_filter(K.filter_dense, cb, ctx->which_a, none, cnt, &fsrc, pts,
+                                         ytotal, alive);
+            const size_t m4 = mark();
+            span(m3, m4, &st->kernel_filter_s);
+            span(m3, m4, &st->kernel_dominance_s);
+            int64_t *slot = cb.out.as<int64_t>();
+            device_prefix(cb.nalive.as<int32_t>(), cnt, slot, bc + 1);
+            K.compact(dim3(static_cast<unsigned>(cnt)), s, cb.chunks.as<ChunkDesc>(),
+                      cb.nsurv.as<int32_t>(), slot, pts, ytotal, alive, ctx->sort_a.as<int32_t>(),
+                      ytotal);
+            hip_check(hipGetLastError(), "k_compact_chunks");
+            // runs: the chunks' survivors (slot prefix, its end the count)
+            h2d(ctx, ctx->grp_run, std::vector<int32_t>{0, static_cast<int32_t>(cnt)},
+                "hipMemcpy(group table)");
+            int32_t *ys = ctx->sort_a.as<int32_t>();
+            {
+                int32_t *oth = ctx->sort_b.as<int32_t>();
+                const int passes = ceil_log2(cnt);
+                const dim3 grid(static_cast<unsigned>(ceil_div(ytotal, kBlock)));
+                for (int p = 0; p < passes; ++p) {
+                    K.merge_pass(grid, s, ys, oth, ytotal, slot, ctx->grp_run.as<int32_t>(), 1, p,
+                                 ytotal);
+                    hip_check(hipGetLastError(), "k_merge_pass");
+                    std::swap(ys, oth);
+                }
             }
-            run_start.push_back(xtotal);
-            grp_run.assign({0, cnt});
-            ctx->sort_a.ensure(static_cast<size_t>(xtotal) * d * 4, "hipMalloc(sort buffer)");
-            ctx->sort_b.ensure(static_cast<size_t>(xtotal) * d * 4, "hipMalloc(sort buffer)");
-            hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
-            run_compact(cb, cnt, outpos, ctx->surv_pts.as<int32_t>(), ytotal,
-                        ctx->alive.as<uint8_t>(), ctx->sort_a.as<int32_t>(), xtotal);
-            int32_t *ys = merge_runs(ctx->sort_a, ctx->sort_b, xtotal, run_start, grp_run, cnt);
-            ytotal = xtotal;
-            // 4. descending-key blocks of the rest: local filter, earlier blocks
-            int64_t btotal = 0;
-            chunks.clear();
-            const int32_t nb = static_cast<int32_t>(ceil_div(ytotal, CH));
-            for (int32_t k = 0; k < nb; ++k) {
-                ChunkDesc c;
-                c.start = static_cast<int64_t>(k) * CH;
-                c.len = static_cast<int32_t>(std::min<int64_t>(CH, ytotal - c.start));
-                c.out_base = c.start;
-                c.group = 0;
-                c.group_first = 0;
-                c.group_count = nb;
-                chunks.push_back(c);
-            }
-            int sorted_filters = 0;
-            if (nb > 0) {
-                upload_table(cb, chunks);
-                hip_check(hipEventRecord(ctx->ev_g, s), "hipEventRecord");
-                pmdd_k::SortedSrc ss{ys, ytotal};
-                if (nb == 1) {
-                    sorted_filters = run_filter(K.filter_sorted, cb, ctx->which_a, none, nb, &ss,
-                                                ctx->surv_pts.as<int32_t>(), ytotal,
-                                                ctx->alive.as<uint8_t>());
-                } else {
-                    std::vector<int32_t> all(nb);
-                    for (int32_t k = 0; k < nb; ++k) {
-                        all[k] = k;
-                    }
-                    run_stage(cb, ctx->which_a, all, ss, ctx->surv_pts.as<int32_t>(), ytotal,
-                              ctx->alive.as<uint8_t>());
-                }
-                hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
-                const int cross_waves = run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(),
-                                                        ytotal, ctx->alive.as<uint8_t>(), ys,
-                                                        ytotal);
-                d2h(ctx, nalive, cb.nalive, chunks.size(), "batch survivor counts");
-                if (cross_waves > 0) {
-                    st->kernel_cross_s += pool_seconds(cross_waves);
-                    st->cross_launches += cross_waves;
-                    st->cross_bytes += cross_bytes(chunks, nalive);
-                }
-                outpos.resize(chunks.size());
-                for (size_t c = 0; c < chunks.size(); ++c) {
-                    outpos[c] = btotal;
-                    btotal += nalive[c];
-                }
+            const size_t m5 = mark();
+            span(m4, m5, &st->kernel_expand_td_s);
+            // 3. descending-key blocks of the merged survivors: a block's
+            // local filter and the check against the earlier blocks (the
+            // group's front), giving the batch frontier B
+            hipLaunchKernelGGL(pmdd_k::k_uniform_chunks, cgrid, dim3(kBlock), 0, s, bc + 1, cnt, CH,
+                               true, cb.chunks.as<ChunkDesc>(), cb.nsurv.as<int32_t>(),
+                               cb.nalive.as<int32_t>());
+            hip_check(hipGetLastError(), "k_uniform_chunks");
+            pmdd_k::SortedSrc ss{ys, ytotal};
+            int sorted_filters = 0, waves = 0;
+            if (cnt == 1) {
+                sorted_filters = run_filter(K.filter_sorted, cb, ctx->which_a, none, 1, &ss, pts,
+                                            ytotal, alive);
             } else {
-                hip_check(hipEventRecord(ctx->ev_g, s), "hipEventRecord");
-                hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+                K.stage(dim3(static_cast<unsigned>(cnt)), s, nullptr, cb.chunks.as<ChunkDesc>(), ss,
+                        pts, ytotal, alive, cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>());
+                hip_check(hipGetLastError(), "k_stage_chunks");
+                waves = static_cast<int>(ceil_div(cnt, kCrossWave));
+                ctx->front_count.ensure(2 * static_cast<size_t>(cnt) * sizeof(int64_t),
+                                        "hipMalloc(front counts)");
+                fill_bytes(ctx->front_count.p, 2 * cnt * static_cast<int64_t>(sizeof(int64_t)));
+                int64_t *fc = ctx->front_count.as<int64_t>();
+                const pmdd_tuning &T = ctx->tuning[d];
+                const int64_t per_wave = ceil_div(cnt, waves);
+                const int G = static_cast<int>(std::min<int64_t>(
+                    T.cross_split_max, std::max<int64_t>(1, ceil_div(T.cross_workgroups, per_wave))));
+                for (int w = 0; w < waves; ++w) {
+                    int64_t *fin = fc + (w % 2) * cnt;
+                    int64_t *fout = fc + ((w + 1) % 2) * cnt;
+                    const int only = waves > 1 ? w : -1;
+                    K.cross(dim3(static_cast<unsigned>(cnt * G)), s, nullptr,
+                            cb.chunks.as<ChunkDesc>(), pts, ytotal, alive, cb.nsurv.as<int32_t>(),
+                            ys, ytotal, fin, kCrossWave, G, only, t_cross);
+                    hip_check(hipGetLastError(), "k_cross_chunks");
+                    K.append_front(dim3(static_cast<unsigned>(cnt)), s, nullptr,
+                                   cb.chunks.as<ChunkDesc>(), pts, ytotal, alive,
+                                   cb.nsurv.as<int32_t>(), cb.nalive.as<int32_t>(), ys, ytotal, fin,
+                                   fout, kCrossWave, only, ctx->run_ctr.as<unsigned long long>());
+                    hip_check(hipGetLastError(), "k_append_front");
+                }
+                st->cross_launches += waves;
             }
-            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-            hip_check(hipStreamSynchronize(s), "batch filter sync");
-            if (debug_join) {
-                std::fprintf(stderr, " batch_frontier=%lld R=%lld\n",
-                             static_cast<long long>(btotal), static_cast<long long>(r_count));
-            }
-            // ev_g .. ev_c (inside the dominance time): one k_filter_chunks
-            // launch, or the staging of a batch of several chunks
-            if (sorted_filters > 0) {
-                add_time(ctx->ev_g, ctx->ev_c, &st->kernel_filter_s);
-            }
-            add_time(ctx->ev_d, ctx->ev_b, &st->kernel_dominance_s);
-            st->filter_launches += sorted_filters;
-            st->filter_candidates += ytotal;
-            st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + btotal);
-            st->join_bytes += 4 * static_cast<int64_t>(d) * batch_inputs;
-            if (btotal == 0) {
-                continue;
-            }
-            grow_points(B, d, btotal, 0, s);
-            run_compact(cb, nb, outpos, ctx->surv_pts.as<int32_t>(), ytotal,
-                        ctx->alive.as<uint8_t>(), B.pts.as<int32_t>(), B.cap);
+            const size_t m6 = mark();
+            span(m5, m6, &st->kernel_dominance_s);
+            span(m5, m6, waves > 0 ? &st->kernel_cross_s : &st->kernel_filter_s);
+            // 4. B, dense in key order
+            device_prefix(cb.nalive.as<int32_t>(), cnt, slot, bc + 2);
+            K.compact(dim3(static_cast<unsigned>(cnt)), s, cb.chunks.as<ChunkDesc>(),
+                      cb.nsurv.as<int32_t>(), slot, pts, ytotal, alive, B.pts.as<int32_t>(), B.cap);
+            hip_check(hipGetLastError(), "k_compact_chunks");
+            const size_t m7 = mark();
+            span(m6, m7, &st->kernel_expand_td_s);
+            const int64_t r_prev = r_count;
             if (r_count == 0) {
-                hip_check(hipStreamSynchronize(s), "batch sync");
+                // the first batch frontier is R
                 std::swap(R.pts, B.pts);
                 std::swap(R.cap, B.cap);
-                r_count = btotal;
-                continue;
-            }
-            // 5. R minus what B strictly dominates, then R and B merged by key
-            rchunks.clear();
-            const int32_t rn = static_cast<int32_t>(ceil_div(r_count, CH));
-            std::vector<int32_t> rlen(rn);
-            for (int32_t k = 0; k < rn; ++k) {
-                ChunkDesc c;
-                c.start = static_cast<int64_t>(k) * CH;
-                c.len = static_cast<int32_t>(std::min<int64_t>(CH, r_count - c.start));
-                c.out_base = c.start;
-                c.group = 0;
-                c.group_first = 0;
-                c.group_count = 1;
-                rchunks.push_back(c);
-                rlen[k] = c.len;
-            }
-            upload_table(cr, rchunks);
-            h2d(ctx, cr.nsurv, rlen, "hipMemcpy(frontier chunk sizes)");
-            ctx->keep_b.ensure(static_cast<size_t>(r_count), "hipMalloc(frontier flags)");
-            hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(r_count, 256))),
-                               dim3(256), 0, s, ctx->keep_b.as<uint8_t>(), r_count, uint8_t(1));
-            // B passed the check against this R, so no point of B equals one of
-            // R, and "strictly dominated by B" is "weakly dominated by B":
-            // the refill check in flag mode, R in place, B in key-ordered blocks
-            h2d(ctx, ctx->join_ctr, std::vector<unsigned long long>{
-                                        static_cast<unsigned long long>(r_count)},
-                "hipMemcpy(frontier count)");
-            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            const unsigned rgrid = static_cast<unsigned>(std::min<int64_t>(
-                ceil_div(r_count, pmdd_k::kWave), ctx->tuning[d].refill_waves));
-            hip_check(hipEventRecord(ctx->ev_e, s), "hipEventRecord");
-            for (int64_t lo = 0; lo < btotal; lo += kEvictBlock) {
-                const int64_t hi = std::min<int64_t>(btotal, lo + kEvictBlock);
+            } else {
+                // 5. R minus what B strictly dominates (B passed the check
+                // against this R, so no point of B equals one of R, and
+                // "strictly dominated" is "weakly dominated": the refill
+                // check in flag mode, R in place against the whole of B),
+                // then R and B merged by key
+                rchunks.clear();
+                const int32_t rn = static_cast<int32_t>(ceil_div(r_count, CH));
+                std::vector<int32_t> rlen(rn);
+                for (int32_t k = 0; k < rn; ++k) {
+                    ChunkDesc c;
+                    c.start = static_cast<int64_t>(k) * CH;
+                    c.len = static_cast<int32_t>(std::min<int64_t>(CH, r_count - c.start));
+                    c.out_base = c.start;
+                    c.group = 0;
+                    c.group_first = 0;
+                    c.group_count = 1;
+                    rchunks.push_back(c);
+                    rlen[k] = c.len;
+                }
+                upload_table(cr, rchunks);
+                cr.out.ensure(static_cast<size_t>(rn + 1) * 8, "hipMalloc(chunk outputs)");
+                h2d(ctx, cr.nsurv, rlen, "hipMemcpy(frontier chunk sizes)");
+                ctx->keep_b.ensure(static_cast<size_t>(r_count), "hipMalloc(frontier flags)");
+                hipLaunchKernelGGL(pmdd_k::k_fill_u8,
+                                   dim3(static_cast<unsigned>(ceil_div(r_count, 256))), dim3(256), 0,
+                                   s, ctx->keep_b.as<uint8_t>(), r_count, uint8_t(1));
+                h2d(ctx, ctx->join_ctr,
+                    std::vector<unsigned long long>{static_cast<unsigned long long>(r_count)},
+                    "hipMemcpy(frontier count)");
+                const int64_t SM = r_count + ytotal; // merge stride: R's and B's bound
+                DevBuf &M = ctx->merge_a;
+                DevBuf &Rn = ctx->merge_b;
+                M.ensure(static_cast<size_t>(SM) * d * 4, "hipMalloc(frontier merge)");
+                Rn.ensure(static_cast<size_t>(SM) * d * 4, "hipMalloc(frontier merge)");
+                const size_t e0 = mark();
+                const unsigned rgrid = static_cast<unsigned>(std::min<int64_t>(
+                    ceil_div(r_count, pmdd_k::kWave), ctx->tuning[d].refill_waves));
                 K.refill_flags(dim3(rgrid), s, R.pts.as<int32_t>(), R.cap,
                                ctx->join_ctr.as<unsigned long long>(), B.pts.as<int32_t>(), B.cap,
-                               lo, hi, ctx->keep_b.as<uint8_t>(), t_check);
+                               0, 0, bc + 2, ctx->keep_b.as<uint8_t>(), t_check);
                 hip_check(hipGetLastError(), "k_refill_vs_block");
-                // R and the block of B read once, R's flags read and written
+                const size_t e1 = mark();
+                span(e0, e1, &st->kernel_frontier_check_s);
+                span(e0, e1, &st->kernel_dominance_s);
                 st->frontier_check_launches += 1;
-                st->frontier_check_bytes += static_cast<int64_t>(d) * 4 * (r_count + hi - lo) +
-                                            2 * r_count;
+                hipLaunchKernelGGL(pmdd_k::k_count_chunks, dim3(static_cast<unsigned>(rn)),
+                                   dim3(kBlock), 0, s, cr.chunks.as<ChunkDesc>(),
+                                   cr.nsurv.as<int32_t>(), ctx->keep_b.as<uint8_t>(),
+                                   cr.nalive.as<int32_t>());
+                hip_check(hipGetLastError(), "k_count_chunks");
+                device_prefix(cr.nalive.as<int32_t>(), rn, cr.out.as<int64_t>(), bc + 3);
+                K.compact(dim3(static_cast<unsigned>(rn)), s, cr.chunks.as<ChunkDesc>(),
+                          cr.nsurv.as<int32_t>(), cr.out.as<int64_t>(), R.pts.as<int32_t>(), R.cap,
+                          ctx->keep_b.as<uint8_t>(), M.as<int32_t>(), SM);
+                hip_check(hipGetLastError(), "k_compact_chunks");
+                hipLaunchKernelGGL(pmdd_k::k_copy_at,
+                                   dim3(static_cast<unsigned>(
+                                       std::min<int64_t>(ceil_div(ytotal, kBlock), 4096))),
+                                   dim3(kBlock), 0, s, B.pts.as<int32_t>(), B.cap, bc + 2, d,
+                                   M.as<int32_t>(), SM, bc + 3);
+                hip_check(hipGetLastError(), "k_copy_at");
+                hipLaunchKernelGGL(pmdd_k::k_two_runs, dim3(1), dim3(64), 0, s, bc + 3, bc + 2,
+                                   ctx->run_start.as<int64_t>(), ctx->grp_run.as<int32_t>(),
+                                   bc + 4);
+                hip_check(hipGetLastError(), "k_two_runs");
+                K.merge_pass(dim3(static_cast<unsigned>(ceil_div(SM, kBlock))), s,
+                             M.as<int32_t>(), Rn.as<int32_t>(), SM, ctx->run_start.as<int64_t>(),
+                             ctx->grp_run.as<int32_t>(), 1, 0, SM);
+                hip_check(hipGetLastError(), "k_merge_pass");
+                span(e1, mark(), &st->kernel_expand_td_s);
+                // R takes the merged buffer; its old buffer is kept for the next batch
+                R.pts.swap(Rn);
+                R.cap = SM;
             }
-            hip_check(hipEventRecord(ctx->ev_f, s), "hipEventRecord");
-            hipLaunchKernelGGL(pmdd_k::k_count_chunks, dim3(static_cast<unsigned>(rn)),
-                               dim3(kBlock), 0, s, cr.chunks.as<ChunkDesc>(),
-                               cr.nsurv.as<int32_t>(), ctx->keep_b.as<uint8_t>(),
-                               cr.nalive.as<int32_t>());
-            hip_check(hipGetLastError(), "k_count_chunks");
-            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-            d2h(ctx, rnalive, cr.nalive, rchunks.size(), "frontier survivor counts");
-            add_time(ctx->ev_a, ctx->ev_b, &st->kernel_dominance_s);
-            add_time(ctx->ev_e, ctx->ev_f, &st->kernel_frontier_check_s);
-            routpos.resize(rn);
-            int64_t ralive = 0;
-            for (int32_t k = 0; k < rn; ++k) {
-                routpos[k] = ralive;
-                ralive += rnalive[k];
+            // the batch's second wait: its counts
+            std::vector<int64_t> h;
+            d2h(ctx, h, ctx->batch_ctr, 5, "batch counts");
+            flush_late();
+            const int64_t xtotal = h[1], btotal = h[2];
+            r_count = r_prev == 0 ? btotal : h[4];
+            st->filter_launches += dense + sorted_filters;
+            st->filter_candidates += ytotal + xtotal;
+            st->algorithmic_bytes += 4 * static_cast<int64_t>(d) * (batch_inputs + btotal);
+            st->join_bytes += 4 * static_cast<int64_t>(d) * batch_inputs;
+            if (r_prev > 0) {
+                // R and B read once, R's flags read and written
+                st->frontier_check_bytes += static_cast<int64_t>(d) * 4 * (r_prev + btotal) +
+                                            2 * r_prev;
             }
-            const int64_t total_new = ralive + btotal;
-            // two reused buffers, stride total_new (the merge pass's stride)
-            DevBuf &M = ctx->merge_a;
-            DevBuf &Rn = ctx->merge_b;
-            M.ensure(static_cast<size_t>(total_new) * d * 4, "hipMalloc(frontier merge)");
-            Rn.ensure(static_cast<size_t>(total_new) * d * 4, "hipMalloc(frontier merge)");
-            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
-            run_compact(cr, rn, routpos, R.pts.as<int32_t>(), R.cap, ctx->keep_b.as<uint8_t>(),
-                        M.as<int32_t>(), total_new);
-            for (int o = 0; o < d; ++o) {
-                hip_check(hipMemcpyAsync(M.as<int32_t>() + o * total_new + ralive,
-                                         B.pts.as<int32_t>() + o * B.cap,
-                                         static_cast<size_t>(btotal) * 4, hipMemcpyDeviceToDevice,
-                                         s),
-                          "hipMemcpy(batch frontier)");
+            if (debug_join) {
+                std::fprintf(stderr, " local=%lld batch_frontier=%lld R=%lld\n",
+                             static_cast<long long>(xtotal), static_cast<long long>(btotal),
+                             static_cast<long long>(r_count));
             }
-            std::vector<int64_t> rs = {0, ralive, total_new};
-            std::vector<int32_t> gr = {0, 2};
-            int32_t *merged = merge_runs(M, Rn, total_new, rs, gr, 2);
-            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
-            hip_check(hipStreamSynchronize(s), "frontier merge sync");
-            add_time(ctx->ev_a, ctx->ev_b, &st->kernel_expand_td_s);
-            // R takes the merged buffer; its old buffer is kept for the next batch
-            DevBuf &won = (merged == M.as<int32_t>()) ? M : Rn;
-            R.pts.swap(won);
-            R.cap = total_new;
-            r_count = total_new;
             sample_memory();
         }
         st->join_bytes += 4 * static_cast<int64_t>(d) * r_count;

@@ -658,12 +658,14 @@ __global__ __launch_bounds__(kBlock) void k_cross_chunks(
     __shared__ int16_t TS[kBlock];          // slots of a tile of the chunk itself
     __shared__ int wave_tot[kWaves];
 
-    const int me = which[blockIdx.x / G];
+    const int me = which != nullptr ? which[blockIdx.x / G] : static_cast<int>(blockIdx.x / G);
     const int g = static_cast<int>(blockIdx.x % G);
     const ChunkDesc c = chunks[me];
     const int k_in = me - c.group_first;
-    if (only_wave >= 0 && k_in / W != only_wave) {
-        return; // the list holds every wave's chunks; this launch is one wave
+    if (k_in >= c.group_count || (only_wave >= 0 && k_in / W != only_wave)) {
+        // a chunk past its group's count (a table sized by a bound), or of
+        // another wave (the list holds every wave's chunks)
+        return;
     }
     const int nS = chunk_nsurv[me];
     const int tid = threadIdx.x;
@@ -844,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_chunks(const int32_t *__restri
                                                          int64_t stride, uint8_t *__restrict__ alive,
                                                          int32_t *__restrict__ chunk_nsurv,
                                                          int32_t *__restrict__ chunk_nalive) {
-    const int me = which[blockIdx.x];
+    const int me = which != nullptr ? which[blockIdx.x] : static_cast<int>(blockIdx.x);
     const ChunkDesc c = chunks[me];
     for (int s = threadIdx.x; s < c.len; s += kBlock) {
 #pragma unroll
@@ -876,10 +878,10 @@ __global__ __launch_bounds__(kBlock) void k_append_front(
     int64_t *__restrict__ fcount_out, int W, int only_wave, int d,
     unsigned long long *__restrict__ bytes) {
     __shared__ int wave_tot[kWaves];
-    const int me = which[blockIdx.x];
+    const int me = which != nullptr ? which[blockIdx.x] : static_cast<int>(blockIdx.x);
     const ChunkDesc c = chunks[me];
     const int k_in = me - c.group_first;
-    if (only_wave >= 0 && k_in / W != only_wave) {
+    if (k_in >= c.group_count || (only_wave >= 0 && k_in / W != only_wave)) {
         return;
     }
     const int wave_first = c.group_first + (k_in / W) * W;
@@ -1156,7 +1158,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_pass(
     const int64_t *__restrict__ run_start, const int32_t *__restrict__ grp_run, int32_t G,
     int pass, int64_t total) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (q >= total) {
+    // elements end at the table's sentinel run_start[grp_run[G]]; `total`
+    // bounds the grid (it may exceed the elements when they are counted on
+    // the device)
+    if (q >= total || q >= run_start[grp_run[G]]) {
         return;
     }
     // group of q: last g with run_start[grp_run[g]] <= q
@@ -1263,11 +1268,15 @@ __global__ __launch_bounds__(kBlock) void k_join_products(JoinSrc src, int64_t n
 // ---------------------------------------------------------------------------
 //
 // FLAGS mode: X keeps its order; a point that dies gets flag[q] = 0 (points
-// whose flag is 0 already are skipped), and nothing is appended.
+// whose flag is 0 already are skipped), and nothing is appended.  In FLAGS
+// mode an empty block leaves every point as it is; otherwise a launch over an
+// empty block appends nothing, so its caller must not rely on the survivors
+// (the engine never launches one there).
 template <int D, bool FLAGS>
 __global__ __launch_bounds__(kWave) void k_refill_vs_block(
     const int32_t *__restrict__ X, int64_t xstride, const unsigned long long *__restrict__ n_in,
-    const int32_t *__restrict__ R, int64_t rstride, int64_t lo, int64_t hi,
+    const int32_t *__restrict__ R, int64_t rstride, int64_t lo, int64_t hi_arg,
+    const int64_t *__restrict__ hi_dev,
     int32_t *__restrict__ Y, int64_t ystride, unsigned long long *__restrict__ n_out,
     uint8_t *__restrict__ flag, unsigned long long *__restrict__ pair_tests) {
     constexpr int DP = Pad<D>::DP;
@@ -1281,8 +1290,11 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
     const int64_t n = static_cast<int64_t>(*n_in);
     const int64_t G = gridDim.x;
     const int64_t w = blockIdx.x;
-    if (w * kWave >= n) {
-        return;
+    // the block ends at hi_arg, or at *hi_dev when that is set (a block whose
+    // length is known on the device only)
+    const int64_t hi = hi_dev != nullptr ? *hi_dev : hi_arg;
+    if (w * kWave >= n || hi <= lo) {
+        return; // an empty block kills nothing (flags stay, nothing appended)
     }
     const int64_t len = hi - lo;
     const int nsub = static_cast<int>((len + kSub - 1) / kSub);

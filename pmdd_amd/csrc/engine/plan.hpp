@@ -638,4 +638,77 @@ __global__ __launch_bounds__(kBlock) void k_size_ssd(int64_t n_nodes,
     block_add_f64(dv * dv, flds, ssd);
 }
 
+// ---------------------------------------------------------------------------
+// Coupling batches counted on the device.
+// ---------------------------------------------------------------------------
+// k_uniform_chunks: n_max chunk descriptors cutting *total points (dense,
+// from 0) into CH-point blocks: chunk k < ceil(*total / CH) covers
+// [k CH, min((k + 1) CH, *total)); the rest are empty (len 0 at the end, no
+// survivors).  One group of all the non-empty chunks (group_count = their
+// number) when `one_group`, else each chunk its own group.
+__global__ __launch_bounds__(kBlock) void k_uniform_chunks(const int64_t *__restrict__ total,
+                                                           int64_t n_max, int32_t CH,
+                                                           bool one_group,
+                                                           ChunkDesc *__restrict__ chunks,
+                                                           int32_t *__restrict__ nsurv,
+                                                           int32_t *__restrict__ nalive) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (k >= n_max) {
+        return;
+    }
+    const int64_t t = *total;
+    const int64_t cnt = (t + CH - 1) / CH;
+    ChunkDesc c;
+    c.start = k < cnt ? k * CH : t;
+    c.out_base = c.start;
+    c.group = 0;
+    const int64_t rest = t - c.start;
+    c.len = static_cast<int32_t>(k < cnt ? (rest < CH ? rest : CH) : 0);
+    c.group_first = one_group ? 0 : static_cast<int32_t>(k);
+    c.group_count = one_group ? static_cast<int32_t>(cnt) : 1;
+    chunks[k] = c;
+    nsurv[k] = 0;
+    nalive[k] = 0;
+}
+
+// *dst = (int64) *src (a device count of another type).
+__global__ void k_count_to_i64(const unsigned long long *__restrict__ src,
+                               int64_t *__restrict__ dst) {
+    if (threadIdx.x == 0) {
+        *dst = static_cast<int64_t>(*src);
+    }
+}
+
+// The run table of two dense runs, [0, *a) and [*a, *a + *b), one group:
+// run_start = {0, a, a + b}, grp_run = {0, 2}; *sum = a + b.
+__global__ void k_two_runs(const int64_t *__restrict__ a, const int64_t *__restrict__ b,
+                           int64_t *__restrict__ run_start, int32_t *__restrict__ grp_run,
+                           int64_t *__restrict__ sum) {
+    if (threadIdx.x == 0) {
+        run_start[0] = 0;
+        run_start[1] = *a;
+        run_start[2] = *a + *b;
+        grp_run[0] = 0;
+        grp_run[1] = 2;
+        *sum = *a + *b;
+    }
+}
+
+// dst[o * dst_stride + *at + i] = src[o * src_stride + i] for i < *n, every
+// objective o < d (grid-stride).
+__global__ __launch_bounds__(kBlock) void k_copy_at(const int32_t *__restrict__ src,
+                                                    int64_t src_stride,
+                                                    const int64_t *__restrict__ n, int d,
+                                                    int32_t *__restrict__ dst, int64_t dst_stride,
+                                                    const int64_t *__restrict__ at) {
+    const int64_t cnt = *n;
+    const int64_t base = *at;
+    const int64_t step = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < cnt; i += step) {
+        for (int o = 0; o < d; ++o) {
+            dst[o * dst_stride + base + i] = src[o * src_stride + i];
+        }
+    }
+}
+
 } // namespace pmdd_k
