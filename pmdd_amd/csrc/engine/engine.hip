@@ -371,7 +371,11 @@ struct pmdd_ctx {
     DevBuf plan_rb, node_chunk, scan_part, step_stats, run_ctr;
     DevBuf sd_flag, sd_pos, sd_active, sd_cmp; // device-planned state dominance
     DevBuf batch_ctr;                          // a coupling batch's device counts
+    // the readback block in host memory the device writes (mapped,
+    // coherent), its device address, and the last sequence number published
     pmdd_k::PlanReadback *h_rb = nullptr;
+    pmdd_k::PlanReadback *d_rb_host = nullptr;
+    unsigned long long rb_seq = 0;
     std::vector<hipEvent_t> ev_pool; // start/end pairs around single launches
     std::vector<hipEvent_t> ev_late; // start/end pairs read after a later wait
     Staging staging;
@@ -2470,12 +2474,28 @@ struct Run {
 
     // Reads the headers and scores back (waits for the stream).
     void read_plans() {
-        hip_check(hipMemcpyAsync(ctx->h_rb, ctx->plan_rb.p, sizeof(pmdd_k::PlanReadback),
-                                 hipMemcpyDeviceToHost, s),
-                  "hipMemcpy(plan header)");
-        hip_check(hipStreamSynchronize(s), "plan header sync");
-        ctx->staging.reset();
-        flush_late();
+        // the device writes the block into host memory and then its
+        // sequence number; the host waits for the number (polling the stream
+        // now and then so that a failed stream ends the wait)
+        const unsigned long long want = ++ctx->rb_seq;
+        hipLaunchKernelGGL(pmdd_k::k_publish, dim3(1), dim3(64), 0, s,
+                           static_cast<const pmdd_k::PlanReadback *>(rb()), ctx->d_rb_host, want);
+        hip_check(hipGetLastError(), "k_publish");
+        for (uint64_t spin = 1;; ++spin) {
+            if (__atomic_load_n(&ctx->h_rb->seq, __ATOMIC_ACQUIRE) == want) {
+                break;
+            }
+            if ((spin & 1023) == 0) {
+                const hipError_t q = hipStreamQuery(s);
+                if (q != hipSuccess && q != hipErrorNotReady) {
+                    hip_check(q, "plan header wait");
+                }
+                if (q == hipSuccess && __atomic_load_n(&ctx->h_rb->seq, __ATOMIC_ACQUIRE) != want) {
+                    fail(PMDD_E_HIP, "plan header wait: the stream is idle but the header "
+                                     "was not published");
+                }
+            }
+        }
         for (int k = 0; k < 2; ++k) {
             hdr[k] = ctx->h_rb->hdr[k];
             score_rb[k] = static_cast<int64_t>(ctx->h_rb->score[k]);
@@ -3505,8 +3525,11 @@ int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
         hip_check(hipEventCreate(&c->ev_f), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_g), "hipEventCreate");
         hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_rb), sizeof(pmdd_k::PlanReadback),
-                                hipHostMallocDefault),
+                                hipHostMallocMapped | hipHostMallocCoherent),
                   "hipHostMalloc(plan readback)");
+        std::memset(c->h_rb, 0, sizeof(pmdd_k::PlanReadback));
+        hip_check(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_rb_host), c->h_rb, 0),
+                  "hipHostGetDevicePointer(plan readback)");
         *out = c.release();
         return PMDD_OK;
     });

@@ -29,6 +29,8 @@
 
 #include "kernels.hpp"
 
+#include <cstddef>
+
 namespace pmdd_k {
 
 // Plan values of a node, summed over the layer as exclusive prefixes.
@@ -67,7 +69,24 @@ struct StepStat {
 struct PlanReadback {
     PlanHeader hdr[2];
     unsigned long long score[2];
+    unsigned long long seq; // the host copy's sequence number (k_publish)
 };
+
+// k_publish: one thread copies the readback block into host memory (mapped,
+// coherent) and then, after a system-scope release, its sequence number;
+// the host waits for the number instead of copying and synchronizing.
+__global__ void k_publish(const PlanReadback *__restrict__ src, PlanReadback *dst,
+                          unsigned long long seq) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const unsigned long long *a = reinterpret_cast<const unsigned long long *>(src);
+        unsigned long long *b = reinterpret_cast<unsigned long long *>(dst);
+        constexpr int kWords = static_cast<int>(offsetof(PlanReadback, seq) / 8);
+        for (int i = 0; i < kWords; ++i) {
+            b[i] = a[i];
+        }
+        __hip_atomic_store(&dst->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
 
 struct PlanShape {
     int32_t chunk_cap; // CH
