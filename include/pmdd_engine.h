@@ -97,6 +97,12 @@ typedef struct pmdd_opts {
                               * is split over several batches, so no batch exceeds it);
                               * PMDD_MAX_PROD_ADAPTIVE (0): batches start at 32K products
                               * and double up to 32M */
+    int32_t phase_timing;    /* 1 = time the phases of every layer with HIP events
+                              * (kernel_*_s below); 0 = record no per-layer event (the
+                              * events cost about 6% of a run's wall time).  The named
+                              * entry points set it when given stats, as the reference
+                              * times its kernels whenever it is given stats. */
+    int32_t reserved;
 } pmdd_opts;
 
 #define PMDD_DEFAULT_MAX_CAND 20000000LL
@@ -119,6 +125,7 @@ typedef struct pmdd_frontier {
  * writes min(std_capacity, num_layers) entries (methods 1 and 2; method 3
  * leaves them at 0, as the reference clears them, enum.cu:610-613). */
 typedef struct pmdd_stats {
+    /* Every kernel_*_s field is 0 unless opts.phase_timing is set. */
     double wall_pack_transfer_s;
     double wall_join_s;
     double kernel_expand_td_s;

@@ -61,6 +61,8 @@ class Opts(C.Structure):
         ("count_pair_tests", C.c_int32),
         ("max_cand", C.c_int64),
         ("max_prod", C.c_int64),
+        ("phase_timing", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -1437,6 +1437,9 @@ struct Run {
                        *t_sd = nullptr;
     FrontierLease sd_lease;
     DevFrontier &sd_spare; // state dominance output, swapped with the layer's frontier
+    // per-phase device times (opts.phase_timing): HIP events bracket the
+    // phases of every layer; off, no event is recorded
+    const bool timing;
     // PMDD_DEBUG_JOIN=1 prints per-batch coupling counts to stderr
     const bool debug_join = std::getenv("PMDD_DEBUG_JOIN") != nullptr;
 
@@ -1446,7 +1449,7 @@ struct Run {
     // serving packer's packed layers)
     Run(pmdd_ctx *c, pmdd_resident *r, int d_, const pmdd_opts &o, pmdd_stats *stats)
         : ctx(c), dd(r), opts(o), st(stats), K(launch_for(d_)), d(d_), s(c->stream),
-          sd_lease(c, d_), sd_spare(sd_lease.f) {
+          sd_lease(c, d_), sd_spare(sd_lease.f), timing(o.phase_timing != 0) {
         if (opts.max_cand <= 0) {
             opts.max_cand = PMDD_DEFAULT_MAX_CAND;
         }
@@ -1499,6 +1502,9 @@ struct Run {
     // the stream must have passed them.
     double pool_seconds(int pairs) {
         double t = 0.0;
+        if (!timing) {
+            return t;
+        }
         for (int i = 0; i < pairs; ++i) {
             float ms = 0.f;
             hip_check(hipEventElapsedTime(&ms, ctx->ev_pool[2 * i], ctx->ev_pool[2 * i + 1]),
@@ -1527,17 +1533,29 @@ struct Run {
         double *acc;
     };
     std::vector<Span> spans;
+    void record(hipEvent_t e) {
+        if (timing) {
+            hip_check(hipEventRecord(e, s), "hipEventRecord");
+        }
+    }
     size_t mark() {
+        if (!timing) {
+            return 0;
+        }
         const size_t i = n_marks++;
         while (ctx->ev_late.size() <= i) {
             hipEvent_t e = nullptr;
             hip_check(hipEventCreate(&e), "hipEventCreate");
             ctx->ev_late.push_back(e);
         }
-        hip_check(hipEventRecord(ctx->ev_late[i], s), "hipEventRecord");
+        record(ctx->ev_late[i]);
         return i;
     }
-    void span(size_t a, size_t b, double *acc) { spans.push_back({a, b, acc}); }
+    void span(size_t a, size_t b, double *acc) {
+        if (timing) {
+            spans.push_back({a, b, acc});
+        }
+    }
     // Only once the stream has passed every mark.
     void flush_late() {
         for (const Span &x : spans) {
@@ -1549,6 +1567,9 @@ struct Run {
 
     float elapsed_ms() {
         float ms = 0.f;
+        if (!timing) {
+            return ms;
+        }
         hip_check(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b), "hipEventElapsedTime");
         return ms;
     }
@@ -1732,12 +1753,12 @@ struct Run {
             const pmdd_tuning &T = ctx->tuning[d];
             const int G = static_cast<int>(std::min<int64_t>(
                 T.cross_split_max, std::max<int64_t>(1, ceil_div(T.cross_workgroups, n))));
-            hip_check(hipEventRecord(pool_event(2 * launches), s), "hipEventRecord");
+            record(pool_event(2 * launches));
             K.cross(dim3(static_cast<unsigned>(n * G)), s, wl + start[w], cb.chunks.as<ChunkDesc>(),
                     pts, stride, alive, cb.nsurv.as<int32_t>(), fv, fv_stride, fin, kCrossWave, G,
                     -1, t_cross);
             hip_check(hipGetLastError(), "k_cross_chunks");
-            hip_check(hipEventRecord(pool_event(2 * launches + 1), s), "hipEventRecord");
+            record(pool_event(2 * launches + 1));
             K.append_front(dim3(static_cast<unsigned>(n)), s, wl + start[w],
                            cb.chunks.as<ChunkDesc>(), pts, stride, alive, cb.nsurv.as<int32_t>(),
                            cb.nalive.as<int32_t>(), fv, fv_stride, fin, fout, kCrossWave, -1,
@@ -1939,7 +1960,7 @@ struct Run {
                 }
                 continue;
             }
-            hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+            record(ctx->ev_a);
             int32_t *sorted = nullptr;
             if (!big_node.empty()) {
                 grp_run.push_back(static_cast<int32_t>(run_start.size()));
@@ -1970,7 +1991,7 @@ struct Run {
                           ctx->alive.as<uint8_t>());
             }
             // only k_filter_chunks launches between ev_d and ev_c
-            hip_check(hipEventRecord(ctx->ev_d, s), "hipEventRecord");
+            record(ctx->ev_d);
             int filters = 0;
             if (queued) {
                 filters = run_queue_pass(cb, src, which_tiny, which_mid, which_small,
@@ -1990,13 +2011,13 @@ struct Run {
                                       ctx->surv_pts.as<int32_t>(), pass_c,
                                       ctx->alive.as<uint8_t>());
             }
-            hip_check(hipEventRecord(ctx->ev_c, s), "hipEventRecord");
+            record(ctx->ev_c);
             int cross_waves = 0;
             if (!which_cross.empty()) {
                 cross_waves = run_cross_waves(cb, chunks, ctx->surv_pts.as<int32_t>(), pass_c,
                                               ctx->alive.as<uint8_t>(), sorted, bigpos);
             }
-            hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+            record(ctx->ev_b);
             d2h(ctx, nalive, cb.nalive, chunks.size(), "chunk survivor counts");
             flush_late();
             if (cross_waves > 0) {
@@ -2045,6 +2066,9 @@ struct Run {
     // dominance time also count in kernel_total_s (the filter, frontier-check
     // and cross figures are parts of the dominance time).
     void add_time(hipEvent_t a, hipEvent_t b, double *acc) {
+        if (!timing) {
+            return;
+        }
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
         *acc += ms * 1e-3;
@@ -2140,7 +2164,7 @@ struct Run {
         }
         const int64_t total = f.total();
         ctx->keep_a.ensure(std::max<int64_t>(total, 1), "hipMalloc(keep flags)");
-        hip_check(hipEventRecord(ctx->ev_a, s), "hipEventRecord");
+        record(ctx->ev_a);
         hipLaunchKernelGGL(pmdd_k::k_fill_u8, dim3(static_cast<unsigned>(ceil_div(total, 256))),
                            dim3(256), 0, s, ctx->keep_a.as<uint8_t>(), total, uint8_t(1));
         h2d(ctx, ctx->which, targets, "hipMemcpy(dominance targets)");
@@ -2154,7 +2178,7 @@ struct Run {
                            s, f.off.as<int64_t>(), ctx->keep_a.as<uint8_t>(),
                            ctx->join_meta.as<int64_t>(), n);
         hip_check(hipGetLastError(), "k_count_kept");
-        hip_check(hipEventRecord(ctx->ev_b, s), "hipEventRecord");
+        record(ctx->ev_b);
         std::vector<int64_t> kept;
         d2h(ctx, kept, ctx->join_meta, n, "kept counts");
         const double ms = elapsed_ms();
@@ -3706,6 +3730,9 @@ static int named_call(pmdd_ctx *ctx, const pmdd_dd_view *view, int kind, int met
     o.method = method;
     o.problem_type = problem_type;
     o.state_dominance = sd;
+    // the reference times its kernels whenever it is given stats
+    // (topdown.cu:428-507 passes a timer only when stats != NULL)
+    o.phase_timing = stats != nullptr ? 1 : 0;
     o.max_cand = max_cand;
     o.max_prod = max_prod;
     return pmdd_enumerate(ctx, view, &o, out, stats, err, errlen);
@@ -3960,10 +3987,12 @@ struct Packer {
     }
 };
 
-// Whether a request goes to a packer: method 1 without state dominance, on a
-// diagram small enough to share packed layers with others.
+// Whether a request goes to a packer: method 1 without state dominance or
+// per-request diagnostics, on a diagram small enough to share packed layers
+// with others.
 bool packable(const pmdd_dd_view &v, const pmdd_opts &o) {
-    if (o.method != 1 || o.state_dominance != 0 || o.count_pair_tests != 0) {
+    if (o.method != 1 || o.state_dominance != 0 || o.count_pair_tests != 0 ||
+        o.phase_timing != 0) {
         return false;
     }
     int64_t widest = 0;

@@ -75,7 +75,8 @@ class Engine:
     def version() -> str:
         return abi.engine_lib().pmdd_engine_version().decode()
 
-    def _opts(self, method, problem_type, state_dominance, max_cand, max_prod, count_pair_tests):
+    def _opts(self, method, problem_type, state_dominance, max_cand, max_prod, count_pair_tests,
+              phase_timing=False):
         o = abi.Opts()
         o.method = int(method)
         o.problem_type = int(problem_type)
@@ -83,13 +84,17 @@ class Engine:
         o.count_pair_tests = 1 if count_pair_tests else 0
         o.max_cand = int(max_cand)
         o.max_prod = int(max_prod)
+        o.phase_timing = 1 if phase_timing else 0
         return o
 
     def enumerate(self, dd, method: int = 1, state_dominance: int = 0,
                   max_cand: int = abi.DEFAULT_MAX_CAND, max_prod: int = abi.MAX_PROD_ADAPTIVE,
-                  count_pair_tests: bool = False, sort: bool = True) -> EnumerationResult:
+                  count_pair_tests: bool = False, sort: bool = True,
+                  phase_timing: bool = False) -> EnumerationResult:
+        """phase_timing=True fills the kernel_*_s statistics (HIP events around
+        the phases of every layer); they are 0 otherwise."""
         o = self._opts(method, dd.problem_type, state_dominance, max_cand, max_prod,
-                       count_pair_tests)
+                       count_pair_tests, phase_timing)
         fr = abi.Frontier()
         st = abi.Stats()
         nl = dd.num_layers
@@ -166,10 +171,11 @@ class ResidentDD:
 
     def enumerate(self, method: int = 1, state_dominance: int = 0,
                   max_cand: int = abi.DEFAULT_MAX_CAND, max_prod: int = abi.MAX_PROD_ADAPTIVE,
-                  count_pair_tests: bool = False, sort: bool = True) -> EnumerationResult:
+                  count_pair_tests: bool = False, sort: bool = True,
+                  phase_timing: bool = False) -> EnumerationResult:
         eng = self.engine
         o = eng._opts(method, self.problem_type, state_dominance, max_cand, max_prod,
-                      count_pair_tests)
+                      count_pair_tests, phase_timing)
         fr = abi.Frontier()
         st = abi.Stats()
         err = C.create_string_buffer(1024)

@@ -94,7 +94,7 @@ class Measure:
             fam = {"filter": 0.0, "cross": 0.0, "check": 0.0, "kernel": 0.0}
             for name, r, methods in self.res:
                 for m in methods:
-                    out = r.enumerate(method=m)
+                    out = r.enumerate(method=m, phase_timing=True)
                     key = (name, m)
                     dg = digest(out.points)
                     if self.ref.setdefault(key, dg) != dg:

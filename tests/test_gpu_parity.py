@@ -278,17 +278,58 @@ def test_engine_reuse_across_objective_counts(engine, tmp_path):
             assert _same(engine.enumerate(dd, method=method).points, ref), (k, method)
 
 
+TIMES = ("kernel_expand_td_s", "kernel_dominance_s", "kernel_total_s", "kernel_filter_s",
+         "kernel_frontier_check_s", "kernel_cross_s", "kernel_state_dominance_s")
+
+
 def test_kernel_time_breakdown_is_consistent(engine):
-    """kernel_total_s is expansion plus dominance time; the filter, frontier
-    check and cross figures are parts of the dominance time."""
+    """With phase timing, kernel_total_s is expansion plus dominance time; the
+    filter, frontier check and cross figures are parts of the dominance time.
+    Without it every kernel time is 0 and the frontier and counts are the
+    same."""
     dd = _dd("KP_p-3_n-30_ins-2.dat")
     for method in (1, 3):
-        s = engine.enumerate(dd, method=method, state_dominance=1).stats
+        r = engine.enumerate(dd, method=method, state_dominance=1, phase_timing=True)
+        s = r.stats
+        assert s["kernel_total_s"] > 0
         assert s["kernel_total_s"] == pytest.approx(
             s["kernel_expand_td_s"] + s["kernel_dominance_s"], rel=1e-9, abs=1e-12)
         parts = s["kernel_filter_s"] + s["kernel_frontier_check_s"] + s["kernel_cross_s"]
         assert parts <= s["kernel_dominance_s"] * (1 + 1e-6) + 1e-9
         assert s["filter_launches"] > 0
+        q = engine.enumerate(dd, method=method, state_dominance=1)
+        assert all(q.stats[k] == 0.0 for k in TIMES), {k: q.stats[k] for k in TIMES}
+        assert _same(q.points, r.points)
+        for k in ("work_candidates_total", "work_frontier_survivors_total",
+                  "dominance_filtered_total", "filter_launches", "cross_launches"):
+            assert q.stats[k] == s[k], k
+
+
+def test_named_entry_point_times_its_kernels(engine):
+    """pmdd_topdown_enumerate and pmdd_coupled_bdd_enumerate, the entry points
+    that replace the reference's topdown_cuda_enumerate and
+    coupled_bdd_cuda_enumerate, fill the kernel times whenever they are given
+    stats, as the reference does (topdown.cu:428-507), and give the oracle's
+    frontier."""
+    import ctypes as C
+    from pmdd_amd.engine import _take_frontier
+    dd = _dd("KP_p-3_n-30_ins-2.dat")
+    ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
+    lib = engine._lib
+    for name in ("pmdd_topdown_enumerate", "pmdd_coupled_bdd_enumerate"):
+        print(f"case {name}", flush=True)
+        fr = abi.Frontier()
+        st = abi.Stats()
+        err = C.create_string_buffer(1024)
+        args = [engine._h, dd.view_ptr, 1, 0, abi.DEFAULT_MAX_CAND]
+        if name == "pmdd_coupled_bdd_enumerate":
+            args.append(abi.DEFAULT_MAX_PROD)
+        rc = getattr(lib, name)(*args, C.byref(fr), C.byref(st), err, len(err))
+        assert rc == abi.PMDD_OK, err.value.decode()
+        assert _same(_take_frontier(lib, fr, True), ref)
+        assert st.kernel_total_s > 0 and st.kernel_dominance_s > 0
+        assert st.kernel_total_s == pytest.approx(st.kernel_expand_td_s + st.kernel_dominance_s,
+                                                  rel=1e-9, abs=1e-12)
 
 
 def test_tsp15_5obj_seed13095_known_count(engine):
