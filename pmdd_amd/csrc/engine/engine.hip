@@ -499,9 +499,8 @@ struct pmdd_resident {
     std::vector<std::vector<int32_t>> single_parent; // per layer: parent index or -1
     std::vector<std::vector<int32_t>> single_arc;    // per layer: arc value of that parent
     std::vector<std::vector<int32_t>> sd_partner;    // per layer, set packing (or empty)
-    DevBuf arena;
+    DevBuf arena; // CSR arrays of both directions, then the state dominance tables
     // device copies of the state dominance tables, per layer (null: none)
-    DevBuf sd_arena;
     std::vector<const int32_t *> d_mw_order, d_single_parent, d_single_arc, d_partner;
 };
 
@@ -511,7 +510,8 @@ namespace {
 // Upload: pack the view into per-layer CSR in both directions
 // (pack_bdd_layers enum.cu:438-593, pack_mdd_layers enum.cu:85-158).
 // ---------------------------------------------------------------------------
-void validate_view(const pmdd_dd_view *v) {
+// The checks of a view that are linear in its layers and nodes.
+void validate_nodes(const pmdd_dd_view *v) {
     if (v == nullptr || v->layer_off == nullptr || v->in_off == nullptr) {
         fail(PMDD_E_INVALID, "decision diagram view is NULL or incomplete");
     }
@@ -556,18 +556,6 @@ void validate_view(const pmdd_dd_view *v) {
     if (A > 0 && (v->arc_src == nullptr || v->arc_w == nullptr || v->arc_val == nullptr)) {
         fail(PMDD_E_INVALID, "arc arrays missing");
     }
-    for (int l = 1; l < L; ++l) {
-        const int64_t prev_n = v->layer_off[l] - v->layer_off[l - 1];
-        for (int64_t a = v->in_off[v->layer_off[l]]; a < v->in_off[v->layer_off[l + 1]]; ++a) {
-            if (v->arc_src[a] < 0 || v->arc_src[a] >= prev_n) {
-                fail(PMDD_E_INVALID, "arc source out of range in layer " + std::to_string(l));
-            }
-            // a BDD arc is a 0-arc or a 1-arc (the bottom-up CSR is grouped so)
-            if (v->kind == PMDD_DD_BDD && v->arc_val[a] != 0 && v->arc_val[a] != 1) {
-                fail(PMDD_E_INVALID, "BDD arc value other than 0 or 1 in layer " + std::to_string(l));
-            }
-        }
-    }
     // A set-packing state-dominance partner is -1 or another node of the same
     // layer (a node's own labels would all drop against themselves).
     if (v->sd_partner != nullptr) {
@@ -582,17 +570,49 @@ void validate_view(const pmdd_dd_view *v) {
             }
         }
     }
-    // Every path sum stays inside int32: per objective, the sum over layers of
-    // the largest |arc weight| entering the layer bounds any partial or full
-    // path sum (the reference adds unchecked, bottomup.cu:124, couple.cu:93).
-    for (int o = 0; o < v->d; ++o) {
+}
+
+// The arcs into layer l (1..L-1): sources in the previous layer, BDD arc
+// values 0 or 1 (the bottom-up CSR is grouped so).  maxw[o] receives the
+// largest |weight| of objective o over these arcs.
+void validate_layer(const pmdd_dd_view *v, int l, int64_t *maxw) {
+    const int64_t prev_n = v->layer_off[l] - v->layer_off[l - 1];
+    const int64_t a0 = v->in_off[v->layer_off[l]], a1 = v->in_off[v->layer_off[l + 1]];
+    const int d = v->d;
+    const int32_t *src = v->arc_src;
+    const int32_t *val = v->arc_val;
+    const bool bdd = v->kind == PMDD_DD_BDD;
+    bool bad_src = false, bad_val = false;
+    int64_t mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t a = a0; a < a1; ++a) {
+        bad_src |= src[a] < 0 || src[a] >= prev_n;
+        bad_val |= bdd && (val[a] & ~1) != 0;
+        const int32_t *w = v->arc_w + a * d;
+        for (int o = 0; o < d; ++o) {
+            mx[o] = std::max<int64_t>(mx[o], std::llabs(static_cast<long long>(w[o])));
+        }
+    }
+    if (bad_src) {
+        fail(PMDD_E_INVALID, "arc source out of range in layer " + std::to_string(l));
+    }
+    if (bad_val) {
+        fail(PMDD_E_INVALID, "BDD arc value other than 0 or 1 in layer " + std::to_string(l));
+    }
+    for (int o = 0; o < d; ++o) {
+        maxw[o] = mx[o];
+    }
+}
+
+// Every path sum stays inside int32: per objective, the sum over layers of
+// the largest |arc weight| entering the layer bounds any partial or full path
+// sum (the reference adds unchecked, bottomup.cu:124, couple.cu:93).  maxw
+// holds validate_layer's figures, d per layer.
+void check_path_bound(const pmdd_dd_view *v, const std::vector<int64_t> &maxw) {
+    const int d = v->d;
+    for (int o = 0; o < d; ++o) {
         int64_t bound = 0;
-        for (int l = 1; l < L; ++l) {
-            int64_t mx = 0;
-            for (int64_t a = v->in_off[v->layer_off[l]]; a < v->in_off[v->layer_off[l + 1]]; ++a) {
-                mx = std::max<int64_t>(mx, std::llabs(static_cast<long long>(v->arc_w[a * v->d + o])));
-            }
-            bound += mx;
+        for (int l = 1; l < v->num_layers; ++l) {
+            bound += maxw[static_cast<size_t>(l) * d + o];
         }
         if (bound > INT32_MAX) {
             fail(PMDD_E_OVERFLOW, "objective " + std::to_string(o) +
@@ -602,8 +622,30 @@ void validate_view(const pmdd_dd_view *v) {
     }
 }
 
+void validate_view(const pmdd_dd_view *v) {
+    validate_nodes(v);
+    std::vector<int64_t> maxw(static_cast<size_t>(v->num_layers) * v->d, 0);
+    for (int l = 1; l < v->num_layers; ++l) {
+        validate_layer(v, l, &maxw[static_cast<size_t>(l) * v->d]);
+    }
+    check_path_bound(v, maxw);
+}
+
+inline void *staging_take(pmdd_ctx *ctx, size_t n);
+
+// Diagrams whose device arena is larger than this are packed into pageable
+// host memory; smaller ones into the context's pinned staging, which keeps
+// that size for later uploads.
+constexpr size_t kPinnedUploadMax = size_t(256) << 20;
+
+// The view packed into one device arena, laid out in host memory first and
+// copied in one transfer: per layer l its top-down CSR (arcs into l, l >= 1:
+// offsets, sources, weights SoA [d][E]) and its bottom-up CSR (arcs out of
+// l, l <= L-2, grouped by tail, "src" = head in l+1), then the state
+// dominance tables of every layer.  Each layer is checked and packed by one
+// of a few host threads.
 pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
-    validate_view(v);
+    validate_nodes(v);
     std::unique_ptr<pmdd_resident> r(new pmdd_resident());
     const int L = v->num_layers;
     const int d = v->d;
@@ -614,127 +656,170 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
     r->root = v->root;
     r->terminal = v->terminal;
     r->layer_size.resize(L);
+    std::vector<int64_t> E(L + 1, 0); // E[l]: arcs into layer l
     for (int l = 0; l < L; ++l) {
         r->layer_size[l] = v->layer_off[l + 1] - v->layer_off[l];
+        E[l] = v->in_off[v->layer_off[l + 1]] - v->in_off[v->layer_off[l]];
     }
     r->td.resize(L);
     r->bu.resize(L);
-    std::vector<std::vector<int32_t>> td_w(L), bu_w(L);
-    // layer l: its top-down CSR, and the bottom-up CSR of layer l-1 (the same
-    // arcs grouped by tail); layers are independent, so a few host threads
-    // take them in turn
-    auto pack_layer = [&](int l) {
-        const int64_t n = r->layer_size[l];
-        const int64_t g0 = v->layer_off[l];
-        const int64_t a0 = v->in_off[g0];
-        const int64_t E = v->in_off[g0 + n] - a0;
-        LayerCSR &c = r->td[l];
-        c.n_arcs = E;
-        c.off.resize(n + 1);
-        for (int64_t i = 0; i <= n; ++i) {
-            c.off[i] = v->in_off[g0 + i] - a0;
-        }
-        c.src.assign(v->arc_src + a0, v->arc_src + a0 + E);
-        std::vector<int32_t> &tw = td_w[l];
-        tw.resize(static_cast<size_t>(d) * E);
-        for (int64_t e = 0; e < E; ++e) {
-            for (int o = 0; o < d; ++o) {
-                tw[o * E + e] = v->arc_w[(a0 + e) * d + o];
-            }
-        }
-        // out-arcs grouped by tail in the reference merge order: BDD 1-arcs
-        // then 0-arcs, MDD by arc value, each in in-arc order
-        LayerCSR &b = r->bu[l - 1];
-        const int64_t np = r->layer_size[l - 1];
-        b.n_arcs = E;
-        b.off.assign(np + 1, 0);
-        std::vector<int32_t> head(E);
-        for (int64_t i = 0; i < n; ++i) {
-            for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) {
-                head[e] = static_cast<int32_t>(i);
-            }
-        }
-        for (int64_t e = 0; e < E; ++e) {
-            b.off[c.src[e] + 1]++;
-        }
-        for (int64_t i = 0; i < np; ++i) {
-            b.off[i + 1] += b.off[i];
-        }
-        std::vector<int64_t> order(E);
-        std::vector<int64_t> pos(b.off.begin(), b.off.end() - 1);
-        const int32_t *val = v->arc_val + a0;
-        if (v->kind == PMDD_DD_BDD) {
-            for (int t = 1; t >= 0; --t) {
-                for (int64_t e = 0; e < E; ++e) {
-                    if (val[e] == t) {
-                        order[pos[c.src[e]]++] = e;
-                    }
-                }
-            }
-        } else {
-            for (int64_t e = 0; e < E; ++e) {
-                order[pos[c.src[e]]++] = e;
-            }
-            for (int64_t i = 0; i < np; ++i) {
-                std::stable_sort(order.begin() + b.off[i], order.begin() + b.off[i + 1],
-                                 [&](int64_t x, int64_t y) { return val[x] < val[y]; });
-            }
-        }
-        b.src.resize(E);
-        std::vector<int32_t> &bw = bu_w[l - 1];
-        bw.resize(static_cast<size_t>(d) * E);
-        for (int64_t k = 0; k < E; ++k) {
-            const int64_t e = order[k];
-            b.src[k] = head[e];
-            for (int o = 0; o < d; ++o) {
-                bw[o * E + k] = tw[o * E + e];
-            }
-        }
+    const bool knap = v->min_weight != nullptr;
+    const bool part = v->sd_partner != nullptr;
+
+    // arena layout
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    struct Place {
+        size_t off = 0, src = 0, w = 0;
     };
-    {
-        const int workers = static_cast<int>(std::min<unsigned>(
-            8, std::max(1u, std::thread::hardware_concurrency())));
-        std::vector<std::thread> pool;
-        std::atomic<int> next_layer(1);
-        std::exception_ptr failure;
-        std::mutex failure_mu;
-        for (int w = 0; w < workers; ++w) {
-            pool.emplace_back([&] {
-                try {
-                    for (int l = next_layer++; l < L; l = next_layer++) {
-                        pack_layer(l);
-                    }
-                } catch (...) {
-                    std::lock_guard<std::mutex> g(failure_mu);
-                    failure = std::current_exception();
-                }
-            });
+    std::vector<Place> td_at(L), bu_at(L);
+    std::vector<size_t> sd_at(L, 0); // first table of layer l
+    size_t bytes = 0;
+    auto place = [&](Place &p, int64_t n, int64_t e) {
+        p.off = bytes;
+        bytes += align(static_cast<size_t>(n + 1) * 8);
+        p.src = bytes;
+        bytes += align(static_cast<size_t>(e) * 4);
+        p.w = bytes;
+        bytes += align(static_cast<size_t>(d) * e * 4);
+    };
+    for (int l = 0; l < L; ++l) {
+        if (l >= 1) {
+            place(td_at[l], r->layer_size[l], E[l]);
         }
-        for (std::thread &t : pool) {
-            t.join();
-        }
-        if (failure) {
-            std::rethrow_exception(failure);
+        if (l <= L - 2) {
+            place(bu_at[l], r->layer_size[l], E[l + 1]);
         }
     }
-    // knapsack metadata for state dominance (pack_bdd_layers enum.cu:496-540)
-    if (v->min_weight != nullptr) {
+    const int tables = (knap ? 3 : 0) + (part ? 1 : 0);
+    for (int l = 0; l < L; ++l) {
+        sd_at[l] = bytes;
+        bytes += static_cast<size_t>(tables) * align(static_cast<size_t>(r->layer_size[l]) * 4);
+    }
+    bytes = std::max<size_t>(bytes, 256);
+
+    std::unique_ptr<char[]> pageable; // not cleared: every byte read is written
+    char *host = nullptr;
+    if (bytes <= kPinnedUploadMax) {
+        host = static_cast<char *>(staging_take(ctx, bytes));
+    }
+    if (host == nullptr) {
+        pageable.reset(new char[bytes]);
+        host = pageable.get();
+    }
+    if (knap) {
         r->min_weight.resize(L);
         r->mw_order.resize(L);
         r->single_parent.resize(L);
         r->single_arc.resize(L);
-        for (int l = 0; l < L; ++l) {
-            const int64_t n = r->layer_size[l];
-            const int64_t g0 = v->layer_off[l];
+    }
+    if (part) {
+        r->sd_partner.resize(L);
+    }
+    std::vector<int64_t> maxw(static_cast<size_t>(L) * d, 0);
+
+    // layer l: its arcs checked, its top-down CSR, the bottom-up CSR of layer
+    // l-1 (the same arcs grouped by tail), its state dominance tables
+    auto pack_layer = [&](int l) {
+        const int64_t n = r->layer_size[l];
+        const int64_t g0 = v->layer_off[l];
+        if (l >= 1) {
+            validate_layer(v, l, &maxw[static_cast<size_t>(l) * d]);
+            const int64_t a0 = v->in_off[g0];
+            const int64_t e_n = E[l];
+            LayerCSR &c = r->td[l];
+            c.n_arcs = e_n;
+            c.off.resize(n + 1);
+            int64_t *hoff = reinterpret_cast<int64_t *>(host + td_at[l].off);
+            for (int64_t i = 0; i <= n; ++i) {
+                c.off[i] = v->in_off[g0 + i] - a0;
+                hoff[i] = c.off[i];
+            }
+            c.src.assign(v->arc_src + a0, v->arc_src + a0 + e_n);
+            if (e_n > 0) {
+                std::memcpy(host + td_at[l].src, c.src.data(), static_cast<size_t>(e_n) * 4);
+            }
+            int32_t *tw = reinterpret_cast<int32_t *>(host + td_at[l].w);
+            for (int64_t e = 0; e < e_n; ++e) {
+                const int32_t *w = v->arc_w + (a0 + e) * d;
+                for (int o = 0; o < d; ++o) {
+                    tw[o * e_n + e] = w[o];
+                }
+            }
+            // out-arcs grouped by tail in the reference merge order: BDD
+            // 1-arcs then 0-arcs, MDD by arc value, each in in-arc order
+            LayerCSR &b = r->bu[l - 1];
+            const int64_t np = r->layer_size[l - 1];
+            b.n_arcs = e_n;
+            b.off.assign(np + 1, 0);
+            for (int64_t e = 0; e < e_n; ++e) {
+                b.off[c.src[e] + 1]++;
+            }
+            for (int64_t i = 0; i < np; ++i) {
+                b.off[i + 1] += b.off[i];
+            }
+            std::vector<int64_t> order(e_n);
+            std::vector<int64_t> pos(b.off.begin(), b.off.end() - 1);
+            const int32_t *val = v->arc_val + a0;
+            if (v->kind == PMDD_DD_BDD) {
+                for (int t = 1; t >= 0; --t) {
+                    for (int64_t e = 0; e < e_n; ++e) {
+                        if (val[e] == t) {
+                            order[pos[c.src[e]]++] = e;
+                        }
+                    }
+                }
+            } else {
+                for (int64_t e = 0; e < e_n; ++e) {
+                    order[pos[c.src[e]]++] = e;
+                }
+                for (int64_t i = 0; i < np; ++i) {
+                    std::stable_sort(order.begin() + b.off[i], order.begin() + b.off[i + 1],
+                                     [&](int64_t x, int64_t y) { return val[x] < val[y]; });
+                }
+            }
+            std::vector<int32_t> head(e_n);
+            for (int64_t i = 0; i < n; ++i) {
+                for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) {
+                    head[e] = static_cast<int32_t>(i);
+                }
+            }
+            std::memcpy(host + bu_at[l - 1].off, b.off.data(), static_cast<size_t>(np + 1) * 8);
+            b.src.resize(e_n);
+            int32_t *bw = reinterpret_cast<int32_t *>(host + bu_at[l - 1].w);
+            for (int64_t k = 0; k < e_n; ++k) {
+                const int64_t e = order[k];
+                b.src[k] = head[e];
+                for (int o = 0; o < d; ++o) {
+                    bw[o * e_n + k] = tw[o * e_n + e];
+                }
+            }
+            if (e_n > 0) {
+                std::memcpy(host + bu_at[l - 1].src, b.src.data(), static_cast<size_t>(e_n) * 4);
+            }
+        }
+        // state dominance tables: knapsack nodes by descending min_weight
+        // (stable) with their single parent and its arc value
+        // (pack_bdd_layers enum.cu:496-540); set-packing partners
+        size_t at = sd_at[l];
+        auto put = [&](const std::vector<int32_t> &t) {
+            std::memcpy(host + at, t.data(), t.size() * 4);
+            at += align(t.size() * 4);
+        };
+        if (knap) {
             r->min_weight[l].assign(v->min_weight + g0, v->min_weight + g0 + n);
             const std::vector<int32_t> &mw = r->min_weight[l];
+            // keys (descending min_weight, ascending index): a stable order
+            std::vector<uint64_t> key(n);
+            for (int64_t i = 0; i < n; ++i) {
+                const uint32_t up = static_cast<uint32_t>(mw[i]) ^ 0x80000000u;
+                key[i] = (static_cast<uint64_t>(~up) << 32) | static_cast<uint64_t>(i);
+            }
+            std::sort(key.begin(), key.end());
             std::vector<int32_t> &ord = r->mw_order[l];
             ord.resize(n);
             for (int64_t i = 0; i < n; ++i) {
-                ord[i] = static_cast<int32_t>(i);
+                ord[i] = static_cast<int32_t>(key[i] & 0xffffffffu);
             }
-            std::stable_sort(ord.begin(), ord.end(),
-                             [&](int32_t a, int32_t b) { return mw[a] > mw[b]; });
             r->single_parent[l].assign(n, -1);
             r->single_arc[l].assign(n, -1);
             for (int64_t i = 0; i < n; ++i) {
@@ -744,95 +829,83 @@ pmdd_resident *upload(pmdd_ctx *ctx, const pmdd_dd_view *v) {
                     r->single_arc[l][i] = v->arc_val[a0];
                 }
             }
+            put(ord);
+            put(r->single_parent[l]);
+            put(r->single_arc[l]);
         }
-    }
-
-    if (v->sd_partner != nullptr) {
-        r->sd_partner.resize(L);
-        for (int l = 0; l < L; ++l) {
-            const int64_t g0 = v->layer_off[l];
-            r->sd_partner[l].assign(v->sd_partner + g0, v->sd_partner + g0 + r->layer_size[l]);
+        if (part) {
+            r->sd_partner[l].assign(v->sd_partner + g0, v->sd_partner + g0 + n);
+            put(r->sd_partner[l]);
         }
-    }
-
-    // one device arena for all CSR arrays
-    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-    size_t bytes = 0;
-    for (int l = 0; l < L; ++l) {
-        for (LayerCSR *c : {&r->td[l], &r->bu[l]}) {
-            if (c->off.empty()) {
-                continue;
+    };
+    {
+        // a thread per 16K arcs, at most 8; the first failure in layer order
+        // is the one reported
+        const int64_t arcs = v->in_off[v->layer_off[L]];
+        const int workers = static_cast<int>(std::min<int64_t>(
+            {8, std::max<int64_t>(1, std::thread::hardware_concurrency()),
+             std::max<int64_t>(1, arcs >> 14), L}));
+        std::vector<std::exception_ptr> failure(L);
+        std::atomic<int> next_layer(0);
+        auto work = [&] {
+            for (int l = next_layer++; l < L; l = next_layer++) {
+                try {
+                    pack_layer(l);
+                } catch (...) {
+                    failure[l] = std::current_exception();
+                }
             }
-            bytes += align(c->off.size() * 8) + align(c->src.size() * 4) +
-                     align(static_cast<size_t>(d) * c->n_arcs * 4);
+        };
+        std::vector<std::thread> pool;
+        for (int w = 1; w < workers; ++w) {
+            pool.emplace_back(work);
+        }
+        work();
+        for (std::thread &t : pool) {
+            t.join();
+        }
+        for (const std::exception_ptr &e : failure) {
+            if (e) {
+                std::rethrow_exception(e);
+            }
         }
     }
+    check_path_bound(v, maxw);
+
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     r->arena.ensure(bytes, "hipMalloc(diagram arena)");
     char *base = r->arena.as<char>();
-    size_t at = 0;
     for (int l = 0; l < L; ++l) {
-        for (int dir = 0; dir < 2; ++dir) {
-            LayerCSR *c = dir == 0 ? &r->td[l] : &r->bu[l];
-            const std::vector<int32_t> &w = dir == 0 ? td_w[l] : bu_w[l];
-            if (c->off.empty()) {
-                continue;
-            }
-            c->d_off = reinterpret_cast<int64_t *>(base + at);
-            at += align(c->off.size() * 8);
-            c->d_src = reinterpret_cast<int32_t *>(base + at);
-            at += align(c->src.size() * 4);
-            c->d_w = reinterpret_cast<int32_t *>(base + at);
-            at += align(static_cast<size_t>(d) * c->n_arcs * 4);
-            hip_check(hipMemcpyAsync(c->d_off, c->off.data(), c->off.size() * 8,
-                                     hipMemcpyHostToDevice, ctx->stream),
-                      "hipMemcpy(csr offsets)");
-            if (c->n_arcs > 0) {
-                hip_check(hipMemcpyAsync(c->d_src, c->src.data(), c->src.size() * 4,
-                                         hipMemcpyHostToDevice, ctx->stream),
-                          "hipMemcpy(csr sources)");
-                hip_check(hipMemcpyAsync(c->d_w, w.data(), w.size() * 4, hipMemcpyHostToDevice,
-                                         ctx->stream),
-                          "hipMemcpy(csr weights)");
-            }
+        if (l >= 1) {
+            r->td[l].d_off = reinterpret_cast<int64_t *>(base + td_at[l].off);
+            r->td[l].d_src = reinterpret_cast<int32_t *>(base + td_at[l].src);
+            r->td[l].d_w = reinterpret_cast<int32_t *>(base + td_at[l].w);
+        }
+        if (l <= L - 2) {
+            r->bu[l].d_off = reinterpret_cast<int64_t *>(base + bu_at[l].off);
+            r->bu[l].d_src = reinterpret_cast<int32_t *>(base + bu_at[l].src);
+            r->bu[l].d_w = reinterpret_cast<int32_t *>(base + bu_at[l].w);
         }
     }
-    // state dominance tables on the device (the device-planned rule reads them)
-    {
-        r->d_mw_order.assign(L, nullptr);
-        r->d_single_parent.assign(L, nullptr);
-        r->d_single_arc.assign(L, nullptr);
-        r->d_partner.assign(L, nullptr);
-        std::vector<std::pair<const std::vector<int32_t> *, const int32_t **>> parts;
-        for (int l = 0; l < L; ++l) {
-            if (!r->mw_order.empty()) {
-                parts.push_back({&r->mw_order[l], &r->d_mw_order[l]});
-                parts.push_back({&r->single_parent[l], &r->d_single_parent[l]});
-                parts.push_back({&r->single_arc[l], &r->d_single_arc[l]});
-            }
-            if (!r->sd_partner.empty()) {
-                parts.push_back({&r->sd_partner[l], &r->d_partner[l]});
-            }
+    r->d_mw_order.assign(L, nullptr);
+    r->d_single_parent.assign(L, nullptr);
+    r->d_single_arc.assign(L, nullptr);
+    r->d_partner.assign(L, nullptr);
+    for (int l = 0; l < L; ++l) {
+        const size_t t = align(static_cast<size_t>(r->layer_size[l]) * 4);
+        size_t at = sd_at[l];
+        if (knap) {
+            r->d_mw_order[l] = reinterpret_cast<const int32_t *>(base + at);
+            r->d_single_parent[l] = reinterpret_cast<const int32_t *>(base + at + t);
+            r->d_single_arc[l] = reinterpret_cast<const int32_t *>(base + at + 2 * t);
+            at += 3 * t;
         }
-        size_t sbytes = 0;
-        for (const auto &pr : parts) {
-            sbytes += align(pr.first->size() * 4);
-        }
-        if (sbytes > 0) {
-            r->sd_arena.ensure(sbytes, "hipMalloc(state dominance tables)");
-            size_t off = 0;
-            for (const auto &pr : parts) {
-                int32_t *p = reinterpret_cast<int32_t *>(r->sd_arena.as<char>() + off);
-                *pr.second = p;
-                if (!pr.first->empty()) {
-                    hip_check(hipMemcpyAsync(p, pr.first->data(), pr.first->size() * 4,
-                                             hipMemcpyHostToDevice, ctx->stream),
-                              "hipMemcpy(state dominance tables)");
-                }
-                off += align(pr.first->size() * 4);
-            }
+        if (part) {
+            r->d_partner[l] = reinterpret_cast<const int32_t *>(base + at);
         }
     }
+    hip_check(hipMemcpyAsync(base, host, bytes, hipMemcpyHostToDevice, ctx->stream),
+              "hipMemcpy(diagram arena)");
     hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
     return r.release();
 }
@@ -3456,7 +3529,7 @@ int run_enumeration(pmdd_ctx *ctx, pmdd_resident *dd, const pmdd_opts *opts, pmd
     // the diagram counts too when it was uploaded outside this call
     const int64_t diagram = dd->arena.acct == &ctx->acct
                                 ? 0
-                                : static_cast<int64_t>(dd->arena.bytes + dd->sd_arena.bytes);
+                                : static_cast<int64_t>(dd->arena.bytes);
     st->ctx_mem_peak_bytes = ctx->acct.peak.load(std::memory_order_relaxed) + diagram;
     st->mem_need_peak_bytes = ctx->acct.need_peak + diagram;
     return PMDD_OK;
