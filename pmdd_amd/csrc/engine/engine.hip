@@ -993,6 +993,9 @@ struct Launch {
 // that many of their workgroups fit one compute unit at once.
 constexpr int kSmallNode = 256;
 constexpr int kMidNode = 1024; // a middle tier where a full chunk holds 2048 points
+// Layers of fewer nodes than this send every node above kSmallNode
+// candidates to the big-node path (several workgroups per chunk).
+constexpr int64_t kNarrowLayer = 256;
 // Threads of the chunk filter: 128 for nodes of at most 256 candidates (most
 // hold far fewer, so half the waves of a 256-thread workgroup would idle);
 // for chunks of 1024 or more candidates: their
@@ -2483,12 +2486,18 @@ struct Run {
     std::vector<StepRec> steps;
     bool want_std = false;
 
-    pmdd_k::PlanShape plan_shape() const {
+    // Tiers of a layer of n nodes.  A node of at most CH candidates is
+    // normally filtered by one workgroup; in a layer of fewer nodes than the
+    // chip has compute units those workgroups would run alone (a 2048-
+    // candidate node takes about 110 us on one), so there every node above
+    // the tiny tier takes the big-node path, whose cross filter spreads a
+    // chunk over several workgroups.
+    pmdd_k::PlanShape plan_shape(int64_t n) const {
         pmdd_k::PlanShape sh;
         sh.chunk_cap = K.chunk_cap;
         sh.small_cap = kSmallNode;
         sh.mid_cap = K.mid_cap;
-        sh.pad = 0;
+        sh.big_min = n < kNarrowLayer ? kSmallNode : K.chunk_cap;
         return sh;
     }
     pmdd_k::PlanReadback *rb() const { return ctx->plan_rb.as<pmdd_k::PlanReadback>(); }
@@ -2564,7 +2573,7 @@ struct Run {
                            dim3(kBlock), 0, s, n, csr.d_off, csr.d_src, prev.off.as<int64_t>(),
                            ctx->plan_run_off[side].as<int64_t>(),
                            ctx->plan_gamma[side].as<int64_t>(), ctx->plan_part[side].as<int64_t>(),
-                           plan_shape(), arrivals(), &rb()->hdr[side]);
+                           plan_shape(n), arrivals(), &rb()->hdr[side]);
         hip_check(hipGetLastError(), "k_plan_count");
         span(p0, mark(), &st->kernel_expand_td_s);
     }
@@ -2734,7 +2743,7 @@ struct Run {
                                n_nodes, csr.d_off, ctx->plan_run_off[side].as<int64_t>(),
                                ctx->plan_gamma[side].as<int64_t>(),
                                ctx->plan_part[side].as<int64_t>(), &rb()->hdr[side], t,
-                               plan_shape());
+                               plan_shape(n_nodes));
             hip_check(hipGetLastError(), "k_plan_write");
         }
         ExpandSrc src;

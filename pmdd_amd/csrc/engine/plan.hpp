@@ -92,7 +92,7 @@ struct PlanShape {
     int32_t chunk_cap; // CH
     int32_t small_cap; // tiny tier bound
     int32_t mid_cap;   // mid tier bound (0: no mid tier)
-    int32_t pad;
+    int32_t big_min;   // nodes of more candidates take the big-node path (<= CH)
 };
 
 // Plan values of one node from its candidate count and in-degree.
@@ -106,7 +106,7 @@ __device__ __forceinline__ void plan_values(int64_t g, int64_t deg, const PlanSh
     if (g <= 0) {
         return;
     }
-    if (g <= sh.chunk_cap) {
+    if (g <= sh.big_min) {
         pv[PV_CHUNKS] = 1;
         if (g <= sh.small_cap) {
             pv[PV_TINY] = 1;
@@ -385,7 +385,9 @@ __global__ __launch_bounds__(kBlock) void k_plan_write(
     if (ok) {
         const int32_t first = static_cast<int32_t>(at[PV_CHUNKS]);
         t.node_chunk[v] = first;
-        if (pv[PV_CHUNKS] == 1) {
+        // a node in a tier (one chunk, filtered by one workgroup); a big node
+        // may also have a single chunk (a narrow layer's)
+        if (pv[PV_BIGN] == 0 && pv[PV_CHUNKS] == 1) {
             ChunkDesc c;
             c.start = 0;
             c.out_base = at[PV_CAND];
