@@ -101,3 +101,26 @@ def test_big_nodes_in_several_waves(engine):
             # one launch per wave: more launches than expansions means some
             # layer ran several waves
             assert dev.stats["cross_launches"] > dd.num_layers
+
+
+def test_narrow_layers_take_the_big_node_path(engine):
+    """KP_p-3_n-30 (d=3, 2048-candidate chunks): no node exceeds a chunk,
+    but the narrow layers near the root and the terminal (fewer than 256
+    nodes) send every node above 256 candidates to the big-node path, one
+    chunk each, whose cross filter spreads it over several workgroups.  The
+    cross filter therefore runs in the top-down sweep, and the frontier is
+    the oracle's in every method."""
+    name = "KP_p-3_n-30_ins-2.dat"
+    dd = DecisionDiagram.from_file(instance_path(name), 1)
+    ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
+    for method in (1, 2, 3):
+        print(f"case {name} method={method}", flush=True)
+        dev = _run(engine, dd, False, method=method)
+        assert dev.points.shape == ref.shape and (dev.points == ref).all()
+        if method == 1:
+            # the top-down sweep meets such nodes (a probe counted 8 cross
+            # launches; with the rule switched off, none)
+            assert dev.stats["cross_launches"] > 0
+        host = _run(engine, dd, True, method=method)
+        for k in EXACT:
+            assert dev.stats[k] == host.stats[k], (k, dev.stats[k], host.stats[k])
