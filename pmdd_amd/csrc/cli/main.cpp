@@ -127,6 +127,9 @@ int main(int argc, char *argv[]) {
         eo.state_dominance = opts.state_dominance;
         eo.max_cand = opts.max_cand;
         eo.max_prod = opts.max_prod;
+        // the stats line carries the kernel times, as the reference's does
+        // (it times its kernels whenever it is given stats, topdown.cu:428-507)
+        eo.phase_timing = 1;
         rc = pmdd_enumerate(ctx, pmdd_dd_get_view(dd), &eo, &fr, &es, err, sizeof(err));
     }
     if (rc != PMDD_OK) {

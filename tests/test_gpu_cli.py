@@ -49,6 +49,11 @@ def test_cli_outputs(tmp_path, name, ptype, method, sd):
     for key in ("timing", "memory", "work", "dominance", "structure", "metrics", "status"):
         assert key in rec
     assert rec["status"]["status_state"] == "ok"
+    # the kernel times are filled, as the reference fills them whenever it
+    # is given stats (its total is expansion plus dominance time)
+    k = rec["timing"]["kernel"]
+    assert k["kernel_total_s"] > 0
+    assert abs(k["kernel_total_s"] - (k["kernel_expand_td_s"] + k["kernel_dominance_s"])) < 3e-6
     assert rec["structure"]["is_tsp_branch"] == (ptype == "3")
     if method == "3":
         assert rec["structure"]["layer_coupling"] > 0
