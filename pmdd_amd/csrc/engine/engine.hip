@@ -354,6 +354,11 @@ struct Staging {
 struct pmdd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for the small-node filters of a
+    // device-planned layer, which run beside its big nodes' path; the main
+    // stream waits for it (ev_join) before the layer's placement
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr; // untimed
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
     hipEvent_t ev_e = nullptr, ev_f = nullptr, ev_g = nullptr;
     // scratch reused across layers and calls
@@ -1497,6 +1502,37 @@ void seed_frontier(pmdd_ctx *ctx, DevFrontier &f, int d, int64_t n_nodes, int64_
     }
     upload_offsets(ctx, f);
 }
+
+// Launches forked onto the context's side stream: at construction the side
+// stream waits for everything queued so far on the main stream; join() makes
+// the main stream wait for everything queued since on the side stream.  The
+// destructor joins too, so that when an error unwinds a layer, a later wait
+// on the main stream also covers the side stream's launches.
+struct SideFork {
+    pmdd_ctx *ctx;
+    hipStream_t main;
+    bool open = false;
+    SideFork(pmdd_ctx *c, hipStream_t m) : ctx(c), main(m) {
+        hip_check(hipEventRecord(c->ev_fork, m), "hipEventRecord(fork)");
+        hip_check(hipStreamWaitEvent(c->side, c->ev_fork, 0), "hipStreamWaitEvent(fork)");
+        open = true;
+    }
+    SideFork(const SideFork &) = delete;
+    SideFork &operator=(const SideFork &) = delete;
+    void join() {
+        if (open) {
+            open = false;
+            hip_check(hipEventRecord(ctx->ev_join, ctx->side), "hipEventRecord(join)");
+            hip_check(hipStreamWaitEvent(main, ctx->ev_join, 0), "hipStreamWaitEvent(join)");
+        }
+    }
+    ~SideFork() {
+        if (open) {
+            (void)hipEventRecord(ctx->ev_join, ctx->side);
+            (void)hipStreamWaitEvent(main, ctx->ev_join, 0);
+        }
+    }
+};
 
 // ---------------------------------------------------------------------------
 // One enumeration
@@ -2760,6 +2796,36 @@ struct Run {
         src.run_off = ctx->plan_run_off[side].as<int64_t>();
         int32_t *pts = ctx->surv_pts.as<int32_t>();
         uint8_t *alive = ctx->alive.as<uint8_t>();
+        const pmdd_tuning &T = ctx->tuning[d];
+        const int64_t counts[3] = {n_tiny, n_mid, n_small};
+        const DevBuf *lists[3] = {&ctx->which_s, &ctx->which_m, &ctx->which_a};
+        int filters = 0;
+        // the chunk filters of the nodes of at most a chunk, per tier
+        auto tiers = [&](hipStream_t q) {
+            for (int k = 0; k < 3; ++k) {
+                if (counts[k] <= 0) {
+                    continue;
+                }
+                K.filter_tier[k][threads_index(T.filter_threads[k])](
+                    dim3(static_cast<unsigned>(counts[k])), q, lists[k]->as<int32_t>(),
+                    cb.chunks.as<ChunkDesc>(), &src, pts, C, alive, cb.nsurv.as<int32_t>(),
+                    cb.nalive.as<int32_t>(), d_tests);
+                hip_check(hipGetLastError(), "k_filter_chunks");
+                ++filters;
+            }
+        };
+        // With big nodes in the layer, the tiers run on the side stream
+        // beside the big nodes' gather, merges and cross waves: both read
+        // only the chunk tables and the previous frontier, and write only
+        // their own chunks' slots, flags and counts.  (Without big nodes,
+        // one tier beside the others measured no faster.)  Phase timing
+        // keeps every launch on the main stream, where its events bracket
+        // them.
+        std::unique_ptr<SideFork> fork;
+        if (!timing && n_bign > 0 && n_tiny + n_mid + n_small > 0) {
+            fork.reset(new SideFork(ctx, s));
+            tiers(ctx->side);
+        }
         int32_t *sorted = nullptr;
         if (n_bign > 0) {
             // big nodes: parent runs gathered and merged into key order, then
@@ -2785,20 +2851,8 @@ struct Run {
         }
         const size_t m1 = mark();
         span(m0, m1, &st->kernel_expand_td_s);
-        const pmdd_tuning &T = ctx->tuning[d];
-        const int64_t counts[3] = {n_tiny, n_mid, n_small};
-        const DevBuf *lists[3] = {&ctx->which_s, &ctx->which_m, &ctx->which_a};
-        int filters = 0;
-        for (int k = 0; k < 3; ++k) {
-            if (counts[k] <= 0) {
-                continue;
-            }
-            K.filter_tier[k][threads_index(T.filter_threads[k])](
-                dim3(static_cast<unsigned>(counts[k])), s, lists[k]->as<int32_t>(),
-                cb.chunks.as<ChunkDesc>(), &src, pts, C, alive, cb.nsurv.as<int32_t>(),
-                cb.nalive.as<int32_t>(), d_tests);
-            hip_check(hipGetLastError(), "k_filter_chunks");
-            ++filters;
+        if (!fork) {
+            tiers(s);
         }
         const size_t m2 = mark();
         span(m1, m2, &st->kernel_filter_s);
@@ -2827,6 +2881,9 @@ struct Run {
                 hip_check(hipGetLastError(), "k_append_front");
             }
             st->cross_launches += waves;
+        }
+        if (fork) {
+            fork->join(); // the placement reads every chunk's counts
         }
         const size_t m3 = mark();
         span(m1, m3, &st->kernel_dominance_s);
@@ -3626,6 +3683,9 @@ int pmdd_ctx_create(int device, pmdd_ctx **out, char *err, size_t errlen) {
         const TuneTable &tt = tune_table();
         std::copy(tt.t, tt.t + 9, c->tuning);
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_a), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_c), "hipEventCreate");
@@ -3649,8 +3709,10 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
         return;
     }
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream != nullptr) {
-        (void)hipStreamSynchronize(ctx->stream);
+    for (hipStream_t q : {ctx->side, ctx->stream}) {
+        if (q != nullptr) {
+            (void)hipStreamSynchronize(q);
+        }
     }
     ctx->trim(); // release scratch before the stream
     if (ctx->ev_a != nullptr) {
@@ -3662,7 +3724,7 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
     if (ctx->ev_c != nullptr) {
         (void)hipEventDestroy(ctx->ev_c);
     }
-    for (hipEvent_t e : {ctx->ev_d, ctx->ev_e, ctx->ev_f, ctx->ev_g}) {
+    for (hipEvent_t e : {ctx->ev_d, ctx->ev_e, ctx->ev_f, ctx->ev_g, ctx->ev_fork, ctx->ev_join}) {
         if (e != nullptr) {
             (void)hipEventDestroy(e);
         }
@@ -3673,8 +3735,10 @@ void pmdd_ctx_destroy(pmdd_ctx *ctx) {
     for (hipEvent_t e : ctx->ev_late) {
         (void)hipEventDestroy(e);
     }
-    if (ctx->stream != nullptr) {
-        (void)hipStreamDestroy(ctx->stream);
+    for (hipStream_t q : {ctx->side, ctx->stream}) {
+        if (q != nullptr) {
+            (void)hipStreamDestroy(q);
+        }
     }
     if (ctx->h_rb != nullptr) {
         (void)hipHostFree(ctx->h_rb);
