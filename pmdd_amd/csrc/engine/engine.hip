@@ -359,6 +359,11 @@ struct pmdd_ctx {
     // stream waits for it (ev_join) before the layer's placement
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr; // untimed
+    // whether layers fork onto the side stream: not in the contexts of a
+    // server with several slots, whose streams already keep the chip busy
+    // and share its few hardware queues (a fork's event wait in a shared
+    // queue holds back the other slots' launches behind it)
+    bool use_side = true;
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
     hipEvent_t ev_e = nullptr, ev_f = nullptr, ev_g = nullptr;
     // scratch reused across layers and calls
@@ -2822,7 +2827,7 @@ struct Run {
         // keeps every launch on the main stream, where its events bracket
         // them.
         std::unique_ptr<SideFork> fork;
-        if (!timing && n_bign > 0 && n_tiny + n_mid + n_small > 0) {
+        if (!timing && ctx->use_side && n_bign > 0 && n_tiny + n_mid + n_small > 0) {
             fork.reset(new SideFork(ctx, s));
             tiers(ctx->side);
         }
@@ -4203,6 +4208,7 @@ struct pmdd_server {
         if (rc != PMDD_OK) {
             fail(rc, e);
         }
+        c->use_side = false; // runs beside the slots
         pack_ctx[d] = c;
         {
             AcctScope scope(&c->acct);
@@ -4470,6 +4476,7 @@ int pmdd_server_create(const pmdd_server_config *cfg, pmdd_server **out, char *e
                 }
                 fail(rc, e);
             }
+            c->use_side = cfg->slots == 1;
             srv->ctxs.push_back(c);
         }
         if (cfg->memory_budget > 0) {
