@@ -207,8 +207,10 @@ typedef struct pmdd_stats {
     double *std_frontier_survivors_per_layer;
 } pmdd_stats;
 
-/* Per-device engine context: one HIP stream (default priority) and the device
- * buffers its enumerations reuse.  Those buffers (pass scratch and frontier
+/* Per-device engine context: two HIP streams (default priority; the second
+ * runs the small-node filters of a layer with big nodes beside them, joined
+ * before the layer's placement, and every call ends with both idle) and the
+ * device buffers its enumerations reuse.  Those buffers (pass scratch and frontier
  * buffers) stay at the largest size any call needed until pmdd_ctx_destroy,
  * so repeated enumerations allocate nothing.  Calls on one context must not
  * overlap; use one context per thread (or the server below). */
@@ -313,7 +315,8 @@ const char *pmdd_engine_version(void);
  *
  * No counterpart in the reference (which enumerates one instance per
  * process).  A server owns `slots` worker contexts on one device, each with
- * its own stream and resident scratch.  Submitted requests queue in arrival
+ * its own stream and resident scratch (with two slots or more, a slot's
+ * layers stay on its one stream).  Submitted requests queue in arrival
  * order; a request is admitted when a slot is free and its memory estimate
  * fits the budget next to the requests already running, so requests that
  * arrive while others are on the GPU overlap with them.  A request whose
