@@ -124,3 +124,31 @@ def test_narrow_layers_take_the_big_node_path(engine):
         host = _run(engine, dd, True, method=method)
         for k in EXACT:
             assert dev.stats[k] == host.stats[k], (k, dev.stats[k], host.stats[k])
+
+
+LAUNCHES = ("filter_launches", "cross_launches", "frontier_check_launches",
+            "state_dominance_launches")
+
+
+@pytest.mark.parametrize("name,kind,sd", [("KP_p-3_n-30_ins-2.dat", 1, 0),
+                                          ("KP_p-3_n-30_ins-2.dat", 1, 1),
+                                          ("bp-100-20-3-10-10192.dat", 2, 0)])
+def test_two_stream_layers_match_one_stream(engine, name, kind, sd):
+    """A layer with big nodes runs its small-node filters on the context's
+    second stream, beside the big nodes' gather, merges and cross waves,
+    unless phase timing is on (then every launch stays on one stream).  Both
+    give the oracle's frontier, the same launches and the same statistics;
+    method 1 of the knapsack meets big nodes (cross launches) and small ones
+    (filter launches) in its sweep."""
+    dd = DecisionDiagram.from_file(instance_path(name), kind, state_dominance=bool(sd))
+    ref = oracle.enumerate_dd(dd, 1, 0, threads=4)
+    for method in (1, 3):
+        print(f"case two streams {name} sd={sd} method={method}", flush=True)
+        two = engine.enumerate(dd, method=method, state_dominance=sd)
+        one = engine.enumerate(dd, method=method, state_dominance=sd, phase_timing=True)
+        assert two.points.shape == ref.shape and (two.points == ref).all()
+        assert one.points.shape == ref.shape and (one.points == ref).all()
+        if method == 1 and kind == 1:
+            assert two.stats["cross_launches"] > 0 and two.stats["filter_launches"] > 0
+        for k in EXACT + LAUNCHES:
+            assert two.stats[k] == one.stats[k], (k, two.stats[k], one.stats[k])
