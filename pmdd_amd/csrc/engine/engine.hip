@@ -372,6 +372,7 @@ struct pmdd_ctx {
     DevBuf surv_pts, alive, counter, join_meta, keep_a, keep_b;
     DevBuf sort_a, sort_b, big_base, big_node, run_start, grp_run;
     DevBuf join_x, join_y, join_ctr, front_count, merge_a, merge_b;
+    DevBuf join_flag; // a batch's product flags for the check against R's later blocks
     // device-planned layers: per sweep direction the plan of its next layer
     // (arc run offsets, candidate counts, workgroup prefixes); the block the
     // host reads back (headers and scores) and its pinned mirror; the first
@@ -426,7 +427,7 @@ struct pmdd_ctx {
                           &plan_run_off[0], &plan_run_off[1], &plan_gamma[0], &plan_gamma[1],
                           &plan_part[0], &plan_part[1], &plan_rb, &node_chunk, &scan_part,
                           &step_stats, &run_ctr, &sd_flag, &sd_pos, &sd_active, &sd_cmp,
-                          &batch_ctr}) {
+                          &batch_ctr, &join_flag}) {
             f(*b);
         }
         for (DevFrontier &fr : frontier_pool) {
@@ -989,6 +990,12 @@ struct Launch {
     void (*refill_flags)(dim3, hipStream_t, const int32_t *, int64_t,
                          const unsigned long long *, const int32_t *, int64_t, int64_t, int64_t,
                          const int64_t *, uint8_t *, unsigned long long *);
+    // the flag check over blocks of blen rows from lo, one grid row per block
+    void (*refill_flag_rows)(dim3, hipStream_t, const int32_t *, int64_t,
+                             const unsigned long long *, const int32_t *, int64_t, int64_t,
+                             int64_t, int64_t, uint8_t *, unsigned long long *);
+    void (*flag_append)(dim3, hipStream_t, const int32_t *, int64_t, const unsigned long long *,
+                        const uint8_t *, int32_t *, int64_t, unsigned long long *);
     // persistent chunk filters of a queued pass, one per tier (nodes of at
     // most kSmallNode candidates, of at most mid_cap, of at most CH), with
     // the grid that keeps every compute unit full for each
@@ -1022,6 +1029,13 @@ constexpr int kFilterThreadsBig = 512;
 // 8 per SIMD, which its register use allows (4096 measured 4% slower, 16384
 // no faster).
 constexpr int64_t kRefillBlock = 2048;
+// A batch checks its products against R's first kSerialBlocks blocks one
+// launch each, each launch keeping only the survivors; the few products left
+// then meet every later block in one launch (a grid row per block, deaths
+// flagged), at most kTailWaves waves per block, and the unflagged ones are
+// appended.
+constexpr int kSerialBlocks = 1;
+constexpr int64_t kTailWaves = 2048;
 constexpr int64_t kRefillWaves = 8192;
 // Largest coupling batch (products) once the first batches have built R.
 constexpr int64_t kJoinBatchCap = int64_t(1) << 25;
@@ -1094,14 +1108,29 @@ template <int D> struct LaunchImpl {
                           unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D, false>), g, dim3(pmdd_k::kWave), 0, s,
                            X, xs, n_in, R, rs, lo, hi, static_cast<const int64_t *>(nullptr), Y,
-                           ys, n_out, nullptr, pt);
+                           ys, n_out, nullptr, pt, int64_t(0));
     }
     static void refill_flags(dim3 g, hipStream_t s, const int32_t *X, int64_t xs,
                              const unsigned long long *n_in, const int32_t *R, int64_t rs,
                              int64_t lo, int64_t hi, const int64_t *hi_dev, uint8_t *flag,
                              unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D, true>), g, dim3(pmdd_k::kWave), 0, s, X,
-                           xs, n_in, R, rs, lo, hi, hi_dev, nullptr, 0, nullptr, flag, pt);
+                           xs, n_in, R, rs, lo, hi, hi_dev, nullptr, 0, nullptr, flag, pt,
+                           int64_t(0));
+    }
+    static void refill_flag_rows(dim3 g, hipStream_t s, const int32_t *X, int64_t xs,
+                                 const unsigned long long *n_in, const int32_t *R, int64_t rs,
+                                 int64_t lo, int64_t hi, int64_t blen, uint8_t *flag,
+                                 unsigned long long *pt) {
+        hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D, true>), g, dim3(pmdd_k::kWave), 0, s, X,
+                           xs, n_in, R, rs, lo, hi, static_cast<const int64_t *>(nullptr), nullptr,
+                           0, nullptr, flag, pt, blen);
+    }
+    static void flag_append(dim3 g, hipStream_t s, const int32_t *X, int64_t xs,
+                            const unsigned long long *n_in, const uint8_t *flag, int32_t *Y,
+                            int64_t ys, unsigned long long *n_out) {
+        hipLaunchKernelGGL((pmdd_k::k_flag_append<D>), g, dim3(pmdd_k::kBlock), 0, s, X, xs, n_in,
+                           flag, Y, ys, n_out);
     }
     static void stage(dim3 g, hipStream_t s, const int32_t *which, const ChunkDesc *c,
                       pmdd_k::SortedSrc src, int32_t *out, int64_t stride, uint8_t *alive,
@@ -1163,6 +1192,8 @@ template <int D> struct LaunchImpl {
         L.join_products = join_products;
         L.refill_vs = refill_vs;
         L.refill_flags = refill_flags;
+        L.refill_flag_rows = refill_flag_rows;
+        L.flag_append = flag_append;
         L.filter_dense = filter<pmdd_k::SortedSrc, false>;
         L.chunk_cap = CH;
         L.mid_cap = CH > kMidNode ? kMidNode : 0;
@@ -3237,6 +3268,9 @@ struct Run {
                 bounds.push_back(r_count);
             }
             const int nbk = std::max<int>(0, static_cast<int>(bounds.size()) - 1);
+            const int serial = nbk > kSerialBlocks + 1 ? kSerialBlocks : nbk;
+            const int64_t tail_rows =
+                serial < nbk ? ceil_div(r_count - bounds[serial], kRefillBlock) : 0;
             // counters: [0] products, [1 + b] survivors of R block b
             std::vector<unsigned long long> ctr(1 + static_cast<size_t>(nbk), 0);
             ctr[0] = static_cast<unsigned long long>(bprod);
@@ -3255,28 +3289,58 @@ struct Run {
                     ceil_div(bprod, pmdd_k::kWave), ctx->tuning[d].refill_waves));
                 int32_t *in = ctx->join_x.as<int32_t>();
                 int32_t *outb = ctx->join_y.as<int32_t>();
-                for (int b = 0; b < nbk; ++b) {
+                for (int b = 0; b < serial; ++b) {
                     K.refill_vs(dim3(grid), s, in, bprod, dc + b, R.pts.as<int32_t>(), R.cap,
                                 bounds[b], bounds[b + 1], outb, bprod, dc + 1 + b, t_check);
                     hip_check(hipGetLastError(), "k_refill_vs_block");
+                    std::swap(in, outb);
+                }
+                if (serial < nbk) {
+                    // R's rows from bounds[serial] on, in grid rows of
+                    // kRefillBlock (the last one shorter, ending at r_count)
+                    ctx->join_flag.ensure(static_cast<size_t>(bprod), "hipMalloc(product flags)");
+                    uint8_t *pf = ctx->join_flag.as<uint8_t>();
+                    hipLaunchKernelGGL(pmdd_k::k_fill_u8,
+                                       dim3(static_cast<unsigned>(ceil_div(bprod, 256))), dim3(256),
+                                       0, s, pf, bprod, uint8_t(1));
+                    hip_check(hipGetLastError(), "k_fill_u8");
+                    const unsigned gx = static_cast<unsigned>(
+                        std::min<int64_t>(ceil_div(bprod, pmdd_k::kWave), kTailWaves));
+                    K.refill_flag_rows(dim3(gx, static_cast<unsigned>(tail_rows)), s, in, bprod,
+                                       dc + serial, R.pts.as<int32_t>(), R.cap, bounds[serial],
+                                       r_count, kRefillBlock, pf, t_check);
+                    hip_check(hipGetLastError(), "k_refill_vs_block");
+                    K.flag_append(dim3(static_cast<unsigned>(
+                                      std::min<int64_t>(ceil_div(bprod, kBlock), 4096))),
+                                  s, in, bprod, dc + serial, pf, outb, bprod, dc + nbk);
+                    hip_check(hipGetLastError(), "k_flag_append");
                     std::swap(in, outb);
                 }
                 F = in;
                 const size_t m2 = mark();
                 span(m1, m2, &st->kernel_frontier_check_s);
                 span(m1, m2, &st->kernel_dominance_s);
-                st->frontier_check_launches += nbk;
+                st->frontier_check_launches += serial + (serial < nbk ? 1 : 0);
             }
             // the first of the batch's two waits: the products past R size
             // the batch's own filter
             d2h(ctx, ctr, ctx->join_ctr, ctr.size(), "join counters");
             flush_late();
             const int64_t ytotal = static_cast<int64_t>(ctr[nbk]);
-            for (int b = 0; b < nbk; ++b) {
+            for (int b = 0; b < serial; ++b) {
                 const int64_t n_in = static_cast<int64_t>(ctr[b]);
                 const int64_t n_out = static_cast<int64_t>(ctr[b + 1]);
                 st->frontier_check_bytes +=
                     static_cast<int64_t>(d) * 4 * (n_in + n_out + bounds[b + 1] - bounds[b]);
+            }
+            if (serial < nbk) {
+                // every later block reads the products and their flags and
+                // the block; the append reads them once and writes the rest
+                const int64_t n_in = static_cast<int64_t>(ctr[serial]);
+                const int64_t rows = bounds[nbk] - bounds[serial];
+                st->frontier_check_bytes +=
+                    (static_cast<int64_t>(d) * 4 + 2) * n_in * tail_rows +
+                    static_cast<int64_t>(d) * 4 * (rows + n_in + ytotal) + n_in;
             }
             if (debug_join) {
                 std::fprintf(stderr, "join batch: nodes=%d products=%lld after_R=%lld", Kn,

@@ -1084,6 +1084,39 @@ __global__ __launch_bounds__(kBlock) void k_count_kept(const int64_t *__restrict
     }
 }
 
+// Y gets the points X[q], q < *n_in, whose flag is still 1 (order not
+// kept), and *n_out counts them; one atomic per wave (grid-stride).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_flag_append(
+    const int32_t *__restrict__ X, int64_t xstride, const unsigned long long *__restrict__ n_in,
+    const uint8_t *__restrict__ flag, int32_t *__restrict__ Y, int64_t ystride,
+    unsigned long long *__restrict__ n_out) {
+    const int64_t n = static_cast<int64_t>(*n_in);
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (kWave - lane));
+    const int64_t step = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kBlock; b0 < n; b0 += step) {
+        const int64_t q = b0 + threadIdx.x;
+        const bool keep = q < n && flag[q] != 0;
+        const uint64_t mask = __ballot(keep);
+        if (mask == 0) {
+            continue;
+        }
+        unsigned long long base = 0;
+        if (lane == 0) {
+            base = atomicAdd(n_out, static_cast<unsigned long long>(__popcll(mask)));
+        }
+        base = __shfl(base, 0);
+        if (keep) {
+            const int64_t at = static_cast<int64_t>(base) + __popcll(mask & below);
+#pragma unroll
+            for (int o = 0; o < D; ++o) {
+                Y[o * ystride + at] = X[o * xstride + q];
+            }
+        }
+    }
+}
+
 // Fill a byte array (used instead of memset so the engine never records a
 // memset node when captured into a graph).
 __global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
@@ -1275,10 +1308,11 @@ __global__ __launch_bounds__(kBlock) void k_join_products(JoinSrc src, int64_t n
 template <int D, bool FLAGS>
 __global__ __launch_bounds__(kWave) void k_refill_vs_block(
     const int32_t *__restrict__ X, int64_t xstride, const unsigned long long *__restrict__ n_in,
-    const int32_t *__restrict__ R, int64_t rstride, int64_t lo, int64_t hi_arg,
+    const int32_t *__restrict__ R, int64_t rstride, int64_t lo_arg, int64_t hi_arg,
     const int64_t *__restrict__ hi_dev,
     int32_t *__restrict__ Y, int64_t ystride, unsigned long long *__restrict__ n_out,
-    uint8_t *__restrict__ flag, unsigned long long *__restrict__ pair_tests) {
+    uint8_t *__restrict__ flag, unsigned long long *__restrict__ pair_tests,
+    int64_t blen = 0) {
     constexpr int DP = Pad<D>::DP;
     constexpr int kSub = 64;                  // rows between two refills
     constexpr int kSubPerTile = kWave / kSub; // sub-tiles per staged tile
@@ -1291,8 +1325,14 @@ __global__ __launch_bounds__(kWave) void k_refill_vs_block(
     const int64_t G = gridDim.x;
     const int64_t w = blockIdx.x;
     // the block ends at hi_arg, or at *hi_dev when that is set (a block whose
-    // length is known on the device only)
-    const int64_t hi = hi_dev != nullptr ? *hi_dev : hi_arg;
+    // length is known on the device only).  With blen > 0 (FLAGS only), row
+    // y of the grid takes the y-th block of blen rows from lo_arg instead.
+    int64_t lo = lo_arg;
+    int64_t hi = hi_dev != nullptr ? *hi_dev : hi_arg;
+    if (FLAGS && blen > 0) {
+        lo = lo_arg + static_cast<int64_t>(blockIdx.y) * blen;
+        hi = min(hi, lo + blen);
+    }
     if (w * kWave >= n || hi <= lo) {
         return; // an empty block kills nothing (flags stay, nothing appended)
     }
