@@ -1030,10 +1030,13 @@ constexpr int kFilterThreadsBig = 512;
 // no faster).
 constexpr int64_t kRefillBlock = 2048;
 // A batch checks its products against R's first kSerialBlocks blocks one
-// launch each, each launch keeping only the survivors; the few products left
+// launch each, each launch keeping only the survivors; the products left
 // then meet every later block in one launch (a grid row per block, deaths
 // flagged), at most kTailWaves waves per block, and the unflagged ones are
-// appended.
+// appended.  One block (R's first 512 points) first measured fastest: with
+// none, products that R's first points kill within a few rows also walk
+// every later block; with more, each later launch is a few waves walking
+// one block alone.  2048 tail waves beat 512, 1024 and 4096.
 constexpr int kSerialBlocks = 1;
 constexpr int64_t kTailWaves = 2048;
 constexpr int64_t kRefillWaves = 8192;
