@@ -993,7 +993,7 @@ struct Launch {
     // the flag check over blocks of blen rows from lo, one grid row per block
     void (*refill_flag_rows)(dim3, hipStream_t, const int32_t *, int64_t,
                              const unsigned long long *, const int32_t *, int64_t, int64_t,
-                             int64_t, int64_t, uint8_t *, unsigned long long *);
+                             int64_t, const int64_t *, int64_t, uint8_t *, unsigned long long *);
     void (*flag_append)(dim3, hipStream_t, const int32_t *, int64_t, const unsigned long long *,
                         const uint8_t *, int32_t *, int64_t, unsigned long long *);
     // persistent chunk filters of a queued pass, one per tier (nodes of at
@@ -1123,11 +1123,10 @@ template <int D> struct LaunchImpl {
     }
     static void refill_flag_rows(dim3 g, hipStream_t s, const int32_t *X, int64_t xs,
                                  const unsigned long long *n_in, const int32_t *R, int64_t rs,
-                                 int64_t lo, int64_t hi, int64_t blen, uint8_t *flag,
-                                 unsigned long long *pt) {
+                                 int64_t lo, int64_t hi, const int64_t *hi_dev, int64_t blen,
+                                 uint8_t *flag, unsigned long long *pt) {
         hipLaunchKernelGGL((pmdd_k::k_refill_vs_block<D, true>), g, dim3(pmdd_k::kWave), 0, s, X,
-                           xs, n_in, R, rs, lo, hi, static_cast<const int64_t *>(nullptr), nullptr,
-                           0, nullptr, flag, pt, blen);
+                           xs, n_in, R, rs, lo, hi, hi_dev, nullptr, 0, nullptr, flag, pt, blen);
     }
     static void flag_append(dim3 g, hipStream_t s, const int32_t *X, int64_t xs,
                             const unsigned long long *n_in, const uint8_t *flag, int32_t *Y,
@@ -3311,7 +3310,8 @@ struct Run {
                         std::min<int64_t>(ceil_div(bprod, pmdd_k::kWave), kTailWaves));
                     K.refill_flag_rows(dim3(gx, static_cast<unsigned>(tail_rows)), s, in, bprod,
                                        dc + serial, R.pts.as<int32_t>(), R.cap, bounds[serial],
-                                       r_count, kRefillBlock, pf, t_check);
+                                       r_count, static_cast<const int64_t *>(nullptr),
+                                       kRefillBlock, pf, t_check);
                     hip_check(hipGetLastError(), "k_refill_vs_block");
                     K.flag_append(dim3(static_cast<unsigned>(
                                       std::min<int64_t>(ceil_div(bprod, kBlock), 4096))),
@@ -3506,11 +3506,20 @@ struct Run {
                 M.ensure(static_cast<size_t>(SM) * d * 4, "hipMalloc(frontier merge)");
                 Rn.ensure(static_cast<size_t>(SM) * d * 4, "hipMalloc(frontier merge)");
                 const size_t e0 = mark();
+                // B (its length a device count, at most ytotal) in grid rows
+                // of kRefillBlock points (more when ytotal would ask for over
+                // 64 rows), so that R's points, which mostly outlast B, walk
+                // its blocks side by side; rows past B's end exit at once
+                const int64_t blen = std::max<int64_t>(kRefillBlock, ceil_div(ytotal, 64));
+                const int64_t brows = std::max<int64_t>(1, ceil_div(ytotal, blen));
                 const unsigned rgrid = static_cast<unsigned>(std::min<int64_t>(
-                    ceil_div(r_count, pmdd_k::kWave), ctx->tuning[d].refill_waves));
-                K.refill_flags(dim3(rgrid), s, R.pts.as<int32_t>(), R.cap,
-                               ctx->join_ctr.as<unsigned long long>(), B.pts.as<int32_t>(), B.cap,
-                               0, 0, bc + 2, ctx->keep_b.as<uint8_t>(), t_check);
+                    ceil_div(r_count, pmdd_k::kWave),
+                    brows > 1 ? kTailWaves : ctx->tuning[d].refill_waves));
+                K.refill_flag_rows(dim3(rgrid, static_cast<unsigned>(brows)), s,
+                                   R.pts.as<int32_t>(), R.cap,
+                                   ctx->join_ctr.as<unsigned long long>(), B.pts.as<int32_t>(),
+                                   B.cap, 0, 0, bc + 2, blen, ctx->keep_b.as<uint8_t>(),
+                                   t_check);
                 hip_check(hipGetLastError(), "k_refill_vs_block");
                 const size_t e1 = mark();
                 span(e0, e1, &st->kernel_frontier_check_s);
