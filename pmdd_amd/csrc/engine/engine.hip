@@ -1038,6 +1038,9 @@ constexpr int64_t kRefillBlock = 2048;
 // every later block; with more, each later launch is a few waves walking
 // one block alone.  2048 tail waves beat 512, 1024 and 4096.
 constexpr int kSerialBlocks = 1;
+// Workgroups per state-dominance target, each taking every kSdSplit-th tile
+// of the comparison labels.
+constexpr unsigned kSdSplit = 4;
 constexpr int64_t kTailWaves = 2048;
 constexpr int64_t kRefillWaves = 8192;
 // Largest coupling batch (products) once the first batches have built R.
@@ -1081,8 +1084,8 @@ template <int D> struct LaunchImpl {
     static void state_dom(dim3 g, hipStream_t s, const int32_t *t, const int64_t *count,
                           const int64_t *off, const int32_t *pts, int64_t stride, uint8_t *keep,
                           unsigned long long *pt) {
-        hipLaunchKernelGGL((pmdd_k::k_state_dominance<D>), g, dim3(kBlock), 0, s, t, count, off,
-                           pts, stride, keep, pt);
+        hipLaunchKernelGGL((pmdd_k::k_state_dominance<D>), dim3(g.x, kSdSplit), dim3(kBlock), 0,
+                           s, t, count, off, pts, stride, keep, pt);
     }
     static void compact_nodes(dim3 g, hipStream_t s, const int64_t *oi, const int32_t *src,
                               int64_t ss, const uint8_t *keep, const int64_t *oo, int32_t *dst,

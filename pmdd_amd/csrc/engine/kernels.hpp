@@ -996,6 +996,7 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
         m[o] = valid ? pts[o * stride + b0 + i] : 0;
     }
     bool dead = !valid;
+    volatile const uint8_t *vkeep = keep;
     // frontiers are in descending key order: the last label of this block has
     // the smallest key, and a comparison label can only be >= a label whose
     // key does not exceed its own
@@ -1008,9 +1009,17 @@ __global__ __launch_bounds__(kBlock) void k_state_dominance(
         }
         const int64_t c0 = off[cmp];
         const int64_t cn = off[cmp + 1] - c0;
-        for (int64_t tb = 0; tb < cn; tb += kBlock) {
+        // the gridDim.y workgroups of a target take every gridDim.y-th tile
+        // of the comparison labels, and each drops the labels the others
+        // have flagged (flags only go from 1 to 0)
+        const int64_t first = static_cast<int64_t>(blockIdx.y) * kBlock;
+        const int64_t step = static_cast<int64_t>(gridDim.y) * kBlock;
+        for (int64_t tb = first; tb < cn; tb += step) {
             if (key_at<D>(pts, stride, c0 + tb) < block_min) {
                 break;
+            }
+            if (gridDim.y > 1 && valid && !dead) {
+                dead = vkeep[b0 + i] == 0;
             }
             if (!__syncthreads_or(!dead)) {
                 break;
