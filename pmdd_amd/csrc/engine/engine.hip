@@ -1021,14 +1021,14 @@ constexpr int64_t kNarrowLayer = 256;
 // a step of 1024 candidates tests too many peers that die anyway)
 constexpr int kFilterThreadsBig = 512;
 
-// Coupling: blocks of R after the first 2048 points hold kRefillBlock points
-// (a wave's last products walk a whole block with few lanes busy, so shorter
-// blocks drain faster: 2048 measured 5% faster than 8192 over the coupling
-// check, 4096 about as fast, 1024 slower);
+// Coupling: blocks of R after the first 2048 points hold kRefillBlock points,
+// and so do the grid rows of the merged tail launch and of the check of R
+// against a batch frontier (4096 measured fastest there, against 1024, 2048
+// and 8192; when every block was its own launch, 2048 had beaten 8192);
 // k_refill_vs_block runs at most kRefillWaves one-wave workgroups: 32 per CU,
 // 8 per SIMD, which its register use allows (4096 measured 4% slower, 16384
 // no faster).
-constexpr int64_t kRefillBlock = 2048;
+constexpr int64_t kRefillBlock = 4096;
 // A batch checks its products against R's first kSerialBlocks blocks one
 // launch each, each launch keeping only the survivors; the products left
 // then meet every later block in one launch (a grid row per block, deaths
